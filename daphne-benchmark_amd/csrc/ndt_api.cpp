@@ -1,0 +1,956 @@
+/*
+ * ndt_api.cpp - host side of the C ABI (include/ndt_hip.h): contexts, device
+ * memory, grid-build and alignment orchestration, and the compat-mode host
+ * controller.  Compiled by g++ with -ffp-contract=off: the compat controller
+ * restates the reference's host arithmetic (src/CPU/ndt_mapping/kernel.cpp) with
+ * the same libm calls, so its decisions match the reference given the same sums.
+ */
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ndt_hip.h"
+#include "ndt_internal.h"
+
+using namespace ndt;
+
+/* ------------------------------------------------------------------ errors */
+
+static thread_local std::string g_last_error;
+
+static int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (hipError_t)(expr);                                                        \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,                   \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+#define LAUNCH_TRY(expr)                                                                           \
+    do {                                                                                           \
+        int e_ = (expr);                                                                           \
+        if (e_ != 0)                                                                               \
+            return fail(NDT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString((hipError_t)e_)); \
+    } while (0)
+
+extern "C" const char* ndt_last_error(void) { return g_last_error.c_str(); }
+extern "C" const char* ndt_version(void) { return "ndt-mi355x 0.1 (gfx950)"; }
+
+/* ------------------------------------------------------------------ device buffers */
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t want) {
+        if (want <= bytes && p) return 0;
+        release();
+        size_t b = want ? want : 1;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,
+                        std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
+        }
+        bytes = b;
+        return 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+#define BUF_TRY(buf, bytes)                \
+    do {                                   \
+        int e_ = (buf).ensure(bytes);      \
+        if (e_) return e_;                 \
+    } while (0)
+
+struct ndt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0;
+    char name[256] = {0};
+    /* workspaces */
+    DevBuf pts, desc, wgmap, states, sync, partials, out, guesses, small, partials2;
+    double last_ms = 0.0;
+    int32_t last_launches = 0;
+    int64_t last_pairs = 0;
+};
+
+struct ndt_grid {
+    ndt_ctx* ctx = nullptr;
+    ndt_params params;
+    ndt_grid_info info;
+    /* ndt */
+    NdtGridDev dev;
+    DevBuf slots, cand, icov_f, mean_c, icov_c;
+    /* all occupied voxels (diagnostics) */
+    DevBuf all_mean, all_icov, all_cov, all_count, all_valid, all_key;
+    /* compat */
+    CompatGridDev cdev;
+    int bits[3] = {0, 0, 0};
+    int64_t origin_cell[3] = {0, 0, 0};
+};
+
+struct ndt_scans {
+    ndt_ctx* ctx = nullptr;
+    int32_t S = 0;
+    std::vector<int64_t> counts, offsets;
+    int64_t total = 0;
+    DevBuf pts;
+};
+
+/* ------------------------------------------------------------------ params */
+
+extern "C" void ndt_params_default(ndt_params* p, int32_t mode) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->mode = mode;
+    p->resolution = 1.0f;
+    p->step_size = 0.1;
+    p->outlier_ratio = 0.55;
+    if (mode == NDT_MODE_COMPAT) {
+        /* src/CPU/ndt_mapping/kernel.cpp:37-41,56-57 */
+        p->trans_eps = 0.1;
+        p->max_iterations = 0;
+    } else {
+        p->trans_eps = 0.01;
+        p->max_iterations = 30;
+    }
+    p->min_points = 6;
+    p->min_eig_ratio = 0.01;
+    p->point_precision = NDT_POINT_FP32;
+    p->sum_precision = NDT_SUM_FP64;
+}
+
+static int check_params(const ndt_params* p) {
+    if (!p) return fail(NDT_ERR_INVALID, "params is null");
+    if (p->mode != NDT_MODE_COMPAT && p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "unknown mode");
+    if (!(p->resolution > 0.0f) || !std::isfinite(p->resolution)) return fail(NDT_ERR_INVALID, "resolution must be > 0");
+    if (!(p->step_size > 0) || !(p->trans_eps > 0)) return fail(NDT_ERR_INVALID, "step_size and trans_eps must be > 0");
+    if (p->max_iterations < 0) return fail(NDT_ERR_INVALID, "max_iterations must be >= 0");
+    if (!(p->outlier_ratio > 0) || !(p->outlier_ratio < 1)) return fail(NDT_ERR_INVALID, "outlier_ratio must be in (0, 1)");
+    if (p->sum_precision < 0 || p->sum_precision > 2) return fail(NDT_ERR_INVALID, "bad sum_precision");
+    if (p->point_precision < 0 || p->point_precision > 1) return fail(NDT_ERR_INVALID, "bad point_precision");
+    if (p->block_size != 0 && p->block_size != 128 && p->block_size != 256 && p->block_size != 512)
+        return fail(NDT_ERR_INVALID, "block_size must be 0, 128, 256 or 512");
+    if (p->points_per_wg < 0) return fail(NDT_ERR_INVALID, "points_per_wg must be >= 0");
+    return 0;
+}
+
+/* ------------------------------------------------------------------ context */
+
+extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
+    if (!out) return fail(NDT_ERR_INVALID, "out is null");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) return fail(NDT_ERR_INVALID, "no such device");
+    HIP_TRY(hipSetDevice(device));
+    ndt_ctx* c = new (std::nothrow) ndt_ctx();
+    if (!c) return fail(NDT_ERR_NOMEM, "host allocation failed");
+    c->device = device;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(NDT_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
+    }
+    c->cus = prop.multiProcessorCount;
+    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    if (!prop.cooperativeLaunch) {
+        delete c;
+        return fail(NDT_ERR_UNSUPPORTED, "device lacks cooperative launch");
+    }
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        delete c;
+        return fail(NDT_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int ndt_ctx_destroy(ndt_ctx* c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+extern "C" int ndt_ctx_device_info(ndt_ctx* c, int32_t* cus, char* name, int32_t name_len) {
+    if (!c) return fail(NDT_ERR_INVALID, "ctx is null");
+    if (cus) *cus = c->cus;
+    if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", c->name);
+    return 0;
+}
+
+extern "C" int ndt_ctx_last_timing(ndt_ctx* c, double* ms, int32_t* launches, int64_t* pairs) {
+    if (!c) return fail(NDT_ERR_INVALID, "ctx is null");
+    if (ms) *ms = c->last_ms;
+    if (launches) *launches = c->last_launches;
+    if (pairs) *pairs = c->last_pairs;
+    return 0;
+}
+
+static int ceil_log2(int64_t v) {
+    int b = 0;
+    while ((1ll << b) < v) b++;
+    return b;
+}
+
+static inline float ord2f(uint32_t u) {
+    uint32_t bits = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* runs of equal keys in a sorted array: flags -> ranks -> starts; returns count */
+static int sorted_runs(ndt_ctx* c, const uint64_t* keys, int64_t n, int shift, uint64_t limit, DevBuf& flags,
+                       DevBuf& ranks, DevBuf& starts, DevBuf& scratch, int64_t* nruns) {
+    BUF_TRY(flags, sizeof(uint32_t) * (n + 1));
+    BUF_TRY(ranks, sizeof(uint32_t) * (n + 1));
+    BUF_TRY(scratch, scan_scratch_bytes(n) + 512);
+    LAUNCH_TRY(launch_run_heads(keys, n, shift, limit, flags.as<uint32_t>(), c->stream));
+    uint32_t* total = (uint32_t*)((char*)scratch.p + ((scan_scratch_bytes(n) + 255) & ~(size_t)255));
+    LAUNCH_TRY(exclusive_scan_u32(flags.as<uint32_t>(), ranks.as<uint32_t>(), n, total, scratch.p, c->stream));
+    uint32_t h_total = 0;
+    HIP_TRY(hipMemcpyAsync(&h_total, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    BUF_TRY(starts, sizeof(uint32_t) * (h_total + 1));
+    LAUNCH_TRY(launch_run_starts(flags.as<uint32_t>(), ranks.as<uint32_t>(), n, starts.as<uint32_t>(), c->stream));
+    *nruns = h_total;
+    return 0;
+}
+
+/* count of sorted keys below limit (non-sentinel prefix) */
+static int count_below(ndt_ctx* c, const uint64_t* keys, int64_t n, uint64_t limit, int64_t* out) {
+    /* binary search on the host over a few device reads */
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) / 2;
+        uint64_t k = 0;
+        HIP_TRY(hipMemcpyAsync(&k, keys + mid, sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (k < limit) lo = mid + 1;
+        else hi = mid;
+    }
+    *out = lo;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ grid build */
+
+static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
+    const double res = (double)g->params.resolution;
+    BUF_TRY(c->pts, sizeof(float4) * m);
+    HIP_TRY(hipMemcpyAsync(c->pts.p, map, sizeof(float4) * m, hipMemcpyHostToDevice, c->stream));
+    DevBuf mm;
+    BUF_TRY(mm, 64);
+    uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    HIP_TRY(hipMemcpyAsync(mm.p, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_bbox(c->pts.as<float4>(), m, mm.as<uint32_t>(), c->stream));
+    uint32_t h_mm[6];
+    HIP_TRY(hipMemcpyAsync(h_mm, mm.p, sizeof(h_mm), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (h_mm[0] == 0xFFFFFFFFu) return fail(NDT_ERR_INVALID, "map has no finite points");
+    int64_t lo_cell[3], hi_cell[3];
+    for (int k = 0; k < 3; k++) {
+        lo_cell[k] = (int64_t)std::floor((double)ord2f(h_mm[k]) / res);
+        hi_cell[k] = (int64_t)std::floor((double)ord2f(h_mm[3 + k]) / res);
+        g->origin_cell[k] = lo_cell[k] - 1;
+        g->bits[k] = std::max(1, ceil_log2(hi_cell[k] - lo_cell[k] + 3));
+    }
+    const int B = g->bits[0] + g->bits[1] + g->bits[2];
+    if (B + 5 > 62) return fail(NDT_ERR_UNSUPPORTED, "map extent too large for the cell key");
+    DevBuf keys, vals, keys_t, vals_t, scratch, flags, ranks, starts, sc2;
+    BUF_TRY(keys, sizeof(uint64_t) * m);
+    BUF_TRY(vals, sizeof(uint32_t) * m);
+    BUF_TRY(keys_t, sizeof(uint64_t) * m);
+    BUF_TRY(vals_t, sizeof(uint32_t) * m);
+    BUF_TRY(scratch, radix_sort_scratch_bytes(m));
+    LAUNCH_TRY(launch_keys_ndt(c->pts.as<float4>(), m, res, g->origin_cell, g->bits, keys.as<uint64_t>(),
+                               vals.as<uint32_t>(), c->stream));
+    LAUNCH_TRY(radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), keys_t.as<uint64_t>(), vals_t.as<uint32_t>(), m,
+                                B + 1, scratch.p, c->stream));
+    const uint64_t limit = 1ull << B;
+    int64_t nvox = 0, nvalid_pts = 0;
+    int e = sorted_runs(c, keys.as<uint64_t>(), m, 0, limit, flags, ranks, starts, sc2, &nvox);
+    if (e) return e;
+    e = count_below(c, keys.as<uint64_t>(), m, limit, &nvalid_pts);
+    if (e) return e;
+    if (nvox == 0) return fail(NDT_ERR_INVALID, "map has no finite points");
+    BUF_TRY(g->all_mean, sizeof(double) * 3 * nvox);
+    BUF_TRY(g->all_icov, sizeof(double) * 9 * nvox);
+    BUF_TRY(g->all_cov, sizeof(double) * 9 * nvox);
+    BUF_TRY(g->all_count, sizeof(int32_t) * nvox);
+    BUF_TRY(g->all_valid, sizeof(uint32_t) * (nvox + 1));
+    BUF_TRY(g->all_key, sizeof(uint64_t) * nvox);
+    LAUNCH_TRY(launch_voxel_stats_ndt(c->pts.as<float4>(), vals.as<uint32_t>(), keys.as<uint64_t>(), starts.as<uint32_t>(),
+                                      nvox, nvalid_pts, g->params.min_points, g->params.min_eig_ratio, g->bits,
+                                      g->all_mean.as<double>(), g->all_icov.as<double>(), g->all_cov.as<double>(),
+                                      g->all_count.as<int32_t>(), g->all_valid.as<uint32_t>(), g->all_key.as<uint64_t>(),
+                                      c->stream));
+    /* compact the valid voxels */
+    DevBuf vrank, sc3;
+    BUF_TRY(vrank, sizeof(uint32_t) * (nvox + 1));
+    BUF_TRY(sc3, scan_scratch_bytes(nvox) + 512);
+    uint32_t* vtotal = (uint32_t*)((char*)sc3.p + ((scan_scratch_bytes(nvox) + 255) & ~(size_t)255));
+    LAUNCH_TRY(exclusive_scan_u32(g->all_valid.as<uint32_t>(), vrank.as<uint32_t>(), nvox, vtotal, sc3.p, c->stream));
+    uint32_t h_nvalid = 0;
+    HIP_TRY(hipMemcpyAsync(&h_nvalid, vtotal, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t nv = h_nvalid;
+    g->info.map_points = m;
+    g->info.voxels = nvox;
+    g->info.valid_voxels = nv;
+    BUF_TRY(g->mean_c, sizeof(double) * 3 * (nv + 1));
+    BUF_TRY(g->icov_c, sizeof(double) * 6 * (nv + 1));
+    BUF_TRY(g->icov_f, sizeof(float4) * 2 * (nv + 1));
+    DevBuf vkey_c;
+    BUF_TRY(vkey_c, sizeof(uint64_t) * (nv + 1));
+    LAUNCH_TRY(launch_compact_voxels(g->all_valid.as<uint32_t>(), vrank.as<uint32_t>(), nvox, g->all_mean.as<double>(),
+                                     g->all_icov.as<double>(), g->all_key.as<uint64_t>(), g->mean_c.as<double>(),
+                                     g->icov_c.as<double>(), g->icov_f.as<float4>(), vkey_c.as<uint64_t>(), c->stream));
+    /* dilation: candidate lists of every query cell */
+    const int64_t ne = 27 * nv;
+    int64_t nq = 0;
+    if (ne > 0) {
+        BUF_TRY(keys, sizeof(uint64_t) * ne);
+        BUF_TRY(vals, sizeof(uint32_t) * ne);
+        BUF_TRY(keys_t, sizeof(uint64_t) * ne);
+        BUF_TRY(vals_t, sizeof(uint32_t) * ne);
+        BUF_TRY(scratch, radix_sort_scratch_bytes(ne));
+        LAUNCH_TRY(launch_dilate_emit(vkey_c.as<uint64_t>(), nv, g->bits, keys.as<uint64_t>(), vals.as<uint32_t>(), c->stream));
+        LAUNCH_TRY(radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), keys_t.as<uint64_t>(), vals_t.as<uint32_t>(),
+                                    ne, B + 5, scratch.p, c->stream));
+        e = sorted_runs(c, keys.as<uint64_t>(), ne, 5, ~0ull, flags, ranks, starts, sc2, &nq);
+        if (e) return e;
+    }
+    int log2cap = std::max(4, ceil_log2(2 * std::max<int64_t>(nq, 1)));
+    int64_t cap = 1ll << log2cap;
+    BUF_TRY(g->slots, sizeof(HashSlot) * cap);
+    LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
+    BUF_TRY(g->cand, sizeof(float4) * (ne + 1));
+    if (nq > 0) {
+        LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, 5, g->slots.as<HashSlot>(),
+                                      log2cap, c->stream));
+        LAUNCH_TRY(launch_cand_fill(vals.as<uint32_t>(), ne, g->mean_c.as<double>(), g->cand.as<float4>(), c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    g->info.query_cells = nq;
+    g->info.candidates = ne;
+    g->info.hash_capacity = cap;
+    NdtGridDev& d = g->dev;
+    d.slots = g->slots.as<HashSlot>();
+    d.cand = g->cand.as<float4>();
+    d.icov_f = g->icov_f.as<float4>();
+    d.mean_d = g->mean_c.as<double>();
+    d.icov_d = g->icov_c.as<double>();
+    d.log2cap = log2cap;
+    for (int k = 0; k < 3; k++) {
+        d.bits[k] = g->bits[k];
+        d.origin_cell[k] = g->origin_cell[k];
+    }
+    d.resolution = g->params.resolution;
+    d.inv_resolution = 1.0f / g->params.resolution;
+    d.resolution_d = res;
+    return 0;
+}
+
+static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
+    /* initCompute, src/CPU/ndt_mapping/kernel.cpp:1302-1373 */
+    for (int64_t i = 0; i < m; i++)
+        if (!std::isfinite(map[4 * i]) || !std::isfinite(map[4 * i + 1]) || !std::isfinite(map[4 * i + 2]))
+            return fail(NDT_ERR_INVALID, "compat mode needs finite map points");
+    const float res = g->params.resolution;
+    BUF_TRY(c->pts, sizeof(float4) * m);
+    HIP_TRY(hipMemcpyAsync(c->pts.p, map, sizeof(float4) * m, hipMemcpyHostToDevice, c->stream));
+    DevBuf mm;
+    BUF_TRY(mm, 64);
+    uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    HIP_TRY(hipMemcpyAsync(mm.p, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_bbox(c->pts.as<float4>(), m, mm.as<uint32_t>(), c->stream));
+    uint32_t h_mm[6];
+    HIP_TRY(hipMemcpyAsync(h_mm, mm.p, sizeof(h_mm), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    CompatGridDev& d = g->cdev;
+    int64_t ncells = 1;
+    for (int k = 0; k < 3; k++) {
+        float mn = ord2f(h_mm[k]), mx = ord2f(h_mm[3 + k]);
+        d.minv[k] = mn - 0.01f;
+        d.maxv[k] = mx + 0.01f;
+        float span = (d.maxv[k] - d.minv[k]) / res + 1;
+        if (!(span < 2.0e9f)) return fail(NDT_ERR_UNSUPPORTED, "dense grid too large");
+        d.dims[k] = (int)span;
+        ncells *= d.dims[k];
+    }
+    if (ncells > 0x7fffffffll) return fail(NDT_ERR_UNSUPPORTED, "dense grid exceeds the reference's int cell index");
+    d.resolution = res;
+    g->info.dense_cells = ncells;
+    for (int k = 0; k < 3; k++) {
+        g->info.dims[k] = d.dims[k];
+        g->info.min_corner[k] = d.minv[k];
+        g->info.max_corner[k] = d.maxv[k];
+    }
+    DevBuf keys, vals, keys_t, vals_t, scratch, flags, ranks, starts, sc2;
+    BUF_TRY(keys, sizeof(uint64_t) * m);
+    BUF_TRY(vals, sizeof(uint32_t) * m);
+    BUF_TRY(keys_t, sizeof(uint64_t) * m);
+    BUF_TRY(vals_t, sizeof(uint32_t) * m);
+    BUF_TRY(scratch, radix_sort_scratch_bytes(m));
+    LAUNCH_TRY(launch_keys_compat(c->pts.as<float4>(), m, d.minv, d.dims, res, keys.as<uint64_t>(), vals.as<uint32_t>(),
+                                  c->stream));
+    LAUNCH_TRY(radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), keys_t.as<uint64_t>(), vals_t.as<uint32_t>(), m,
+                                std::max(1, ceil_log2(ncells + 1)), scratch.p, c->stream));
+    int64_t nvox = 0;
+    int e = sorted_runs(c, keys.as<uint64_t>(), m, 0, (uint64_t)ncells, flags, ranks, starts, sc2, &nvox);
+    if (e) return e;
+    BUF_TRY(g->all_mean, sizeof(double) * 3 * nvox);
+    BUF_TRY(g->all_icov, sizeof(double) * 9 * nvox);
+    BUF_TRY(g->all_count, sizeof(int32_t) * nvox);
+    BUF_TRY(g->all_key, sizeof(uint64_t) * nvox);
+    LAUNCH_TRY(launch_voxel_stats_compat(c->pts.as<float4>(), vals.as<uint32_t>(), keys.as<uint64_t>(),
+                                         starts.as<uint32_t>(), nvox, m, (double)(size_t)ncells, g->all_mean.as<double>(),
+                                         g->all_icov.as<double>(), g->all_count.as<int32_t>(), g->all_key.as<uint64_t>(),
+                                         c->stream));
+    int log2cap = std::max(4, ceil_log2(2 * std::max<int64_t>(nvox, 1)));
+    int64_t cap = 1ll << log2cap;
+    BUF_TRY(g->slots, sizeof(HashSlot) * cap);
+    LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
+    LAUNCH_TRY(launch_hash_insert_ids(g->all_key.as<uint64_t>(), nvox, g->slots.as<HashSlot>(), log2cap, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    d.slots = g->slots.as<HashSlot>();
+    d.mean = g->all_mean.as<double>();
+    d.icov = g->all_icov.as<double>();
+    d.log2cap = log2cap;
+    g->info.map_points = m;
+    g->info.voxels = nvox;
+    g->info.valid_voxels = nvox;
+    g->info.hash_capacity = cap;
+    return 0;
+}
+
+extern "C" int ndt_grid_build(ndt_ctx* c, const float* map, int64_t m, const ndt_params* params, ndt_grid** out) {
+    if (!c || !map || !out) return fail(NDT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (m <= 0) return fail(NDT_ERR_INVALID, "empty map");
+    if (m > 0x7fffffffll) return fail(NDT_ERR_UNSUPPORTED, "map larger than 2^31 points");
+    int e = check_params(params);
+    if (e) return e;
+    HIP_TRY(hipSetDevice(c->device));
+    ndt_grid* g = new (std::nothrow) ndt_grid();
+    if (!g) return fail(NDT_ERR_NOMEM, "host allocation failed");
+    g->ctx = c;
+    g->params = *params;
+    std::memset(&g->info, 0, sizeof(g->info));
+    std::memset(&g->dev, 0, sizeof(g->dev));
+    std::memset(&g->cdev, 0, sizeof(g->cdev));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    e = (params->mode == NDT_MODE_COMPAT) ? grid_build_compat(c, map, m, g) : grid_build_ndt(c, map, m, g);
+    if (e) {
+        delete g;
+        return e;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    g->info.build_ms = ms;
+    *out = g;
+    return 0;
+}
+
+extern "C" int ndt_grid_info_get(const ndt_grid* g, ndt_grid_info* info) {
+    if (!g || !info) return fail(NDT_ERR_INVALID, "null argument");
+    *info = g->info;
+    return 0;
+}
+
+extern "C" int ndt_grid_destroy(ndt_grid* g) {
+    if (!g) return 0;
+    (void)hipSetDevice(g->ctx->device);
+    delete g;
+    return 0;
+}
+
+extern "C" int ndt_grid_voxels(const ndt_grid* g, int64_t cap, int64_t* keys, int32_t* n, double* mean, double* icov,
+                               int32_t* valid) {
+    if (!g) return fail(NDT_ERR_INVALID, "grid is null");
+    ndt_ctx* c = g->ctx;
+    HIP_TRY(hipSetDevice(c->device));
+    const int64_t k = std::min<int64_t>(cap, g->info.voxels);
+    if (k <= 0) return 0;
+    std::vector<uint64_t> hk(k);
+    HIP_TRY(hipMemcpy(hk.data(), g->all_key.p, sizeof(uint64_t) * k, hipMemcpyDeviceToHost));
+    if (keys) {
+        for (int64_t i = 0; i < k; i++) {
+            if (g->params.mode == NDT_MODE_COMPAT) {
+                keys[3 * i] = (int64_t)hk[i];
+                keys[3 * i + 1] = keys[3 * i + 2] = 0;
+            } else {
+                uint64_t key = hk[i];
+                keys[3 * i] = (int64_t)(key & ((1ull << g->bits[0]) - 1)) + g->origin_cell[0];
+                keys[3 * i + 1] = (int64_t)((key >> g->bits[0]) & ((1ull << g->bits[1]) - 1)) + g->origin_cell[1];
+                keys[3 * i + 2] = (int64_t)(key >> (g->bits[0] + g->bits[1])) + g->origin_cell[2];
+            }
+        }
+    }
+    if (n) HIP_TRY(hipMemcpy(n, g->all_count.p, sizeof(int32_t) * k, hipMemcpyDeviceToHost));
+    if (mean) HIP_TRY(hipMemcpy(mean, g->all_mean.p, sizeof(double) * 3 * k, hipMemcpyDeviceToHost));
+    if (icov) HIP_TRY(hipMemcpy(icov, g->all_icov.p, sizeof(double) * 9 * k, hipMemcpyDeviceToHost));
+    if (valid) {
+        if (g->params.mode == NDT_MODE_COMPAT) {
+            for (int64_t i = 0; i < k; i++) valid[i] = 1;
+        } else {
+            std::vector<uint32_t> hv(k);
+            HIP_TRY(hipMemcpy(hv.data(), g->all_valid.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < k; i++) valid[i] = (int32_t)hv[i];
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ ndt alignment */
+
+static AlignParams make_align_params(const ndt_params* p, int32_t S) {
+    AlignParams a;
+    std::memset(&a, 0, sizeof(a));
+    a.step_size = p->step_size;
+    a.trans_eps = p->trans_eps;
+    gauss_consts((double)p->resolution, p->outlier_ratio, &a.d1, &a.d2);
+    for (int k = 0; k < 3; k++) a.origin[k] = p->origin[k];
+    a.max_iterations = p->max_iterations;
+    a.point_fp64 = p->point_precision == NDT_POINT_FP64;
+    a.sum_mode = a.point_fp64 ? NDT_SUM_FP64 : p->sum_precision;
+    a.nscans = S;
+    return a;
+}
+
+/* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host. */
+static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
+                        const int64_t* counts, int32_t S, const double* guesses, const ndt_params* p,
+                        ndt_result* results) {
+    if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
+    const int block = p->block_size ? p->block_size : 256;
+    AlignParams prm = make_align_params(p, S);
+    int bpc = 0;
+    LAUNCH_TRY(align_kernel_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
+    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
+    const int64_t capacity = (int64_t)bpc * c->cus;
+    int64_t total_pts = 0;
+    for (int32_t s = 0; s < S; s++) total_pts += counts[s];
+    int64_t ppw = p->points_per_wg ? p->points_per_wg : 512;
+    /* spread a large batch over every co-resident workgroup */
+    ppw = std::max<int64_t>(ppw, (total_pts + capacity - 1) / capacity);
+    BUF_TRY(c->guesses, sizeof(double) * 16 * S);
+    BUF_TRY(c->states, sizeof(ScanState) * S);
+    BUF_TRY(c->sync, sizeof(GroupSync) * S);
+    BUF_TRY(c->out, sizeof(ScanOut) * S);
+    BUF_TRY(c->desc, sizeof(ScanDesc) * S);
+    HIP_TRY(hipMemcpyAsync(c->guesses.p, guesses, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
+    std::vector<ScanDesc> desc(S);
+    std::vector<int32_t> wgmap;
+    std::vector<std::pair<int32_t, int32_t>> launches; /* [first scan, end scan) */
+    int64_t used = 0;
+    int32_t first = 0;
+    for (int32_t s = 0; s < S; s++) {
+        int64_t G = std::max<int64_t>(1, (counts[s] + ppw - 1) / ppw);
+        G = std::min<int64_t>(G, std::min<int64_t>(MAX_GROUP, capacity));
+        if (used + G > capacity) {
+            launches.push_back({first, s});
+            first = s;
+            used = 0;
+        }
+        desc[s].offset = offsets[s];
+        desc[s].n = (int32_t)counts[s];
+        desc[s].first_wg = (int32_t)used;
+        desc[s].group = (int32_t)G;
+        for (int64_t w = 0; w < G; w++) wgmap.push_back(s);
+        used += G;
+    }
+    launches.push_back({first, S});
+    BUF_TRY(c->wgmap, sizeof(int32_t) * wgmap.size());
+    BUF_TRY(c->partials, sizeof(double) * 32 * std::max<int64_t>(capacity, 1));
+    HIP_TRY(hipMemcpyAsync(c->desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->wgmap.p, wgmap.data(), sizeof(int32_t) * wgmap.size(), hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), S, prm, c->states.as<ScanState>(), c->sync.as<GroupSync>(),
+                                  c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    int64_t wg_base = 0;
+    for (auto& L : launches) {
+        int32_t total = 0;
+        for (int32_t s = L.first; s < L.second; s++) total += desc[s].group;
+        if (total == 0) continue;
+        LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>(), c->wgmap.as<int32_t>() + wg_base, total, block, prm,
+                                c->states.as<ScanState>(), c->sync.as<GroupSync>(), c->partials.as<double>(),
+                                c->out.as<ScanOut>(), c->stream));
+        wg_base += total;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    std::vector<ScanOut> out(S);
+    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = (int32_t)launches.size();
+    int64_t pairs = 0;
+    int worst = 0;
+    for (int32_t s = 0; s < S; s++) {
+        ndt_result& r = results[s];
+        std::memset(&r, 0, sizeof(r));
+        for (int k = 0; k < 16; k++) {
+            r.final_transformation_f64[k] = out[s].final_m[k];
+            r.final_transformation[k] = (float)out[s].final_m[k];
+        }
+        r.score = out[s].score;
+        r.trans_probability = counts[s] > 0 ? out[s].score / (double)counts[s] : 0.0;
+        r.iterations = out[s].iterations;
+        r.converged = out[s].converged;
+        r.passes = out[s].passes;
+        r.status = out[s].status;
+        pairs += (int64_t)out[s].pairs;
+        if (r.status) worst = r.status;
+    }
+    c->last_pairs = pairs;
+    if (worst) return fail(worst, "a scan's device-side wait exceeded its bound");
+    return 0;
+}
+
+static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const int64_t* counts, int32_t S,
+                      std::vector<int64_t>& offsets, int64_t* total) {
+    offsets.assign(S, 0);
+    int64_t t = 0;
+    for (int32_t s = 0; s < S; s++) {
+        if (!scans[s] || counts[s] <= 0) return fail(NDT_ERR_INVALID, "empty or null scan");
+        if (counts[s] > 0x7fffffffll) return fail(NDT_ERR_UNSUPPORTED, "scan larger than 2^31 points");
+        offsets[s] = t;
+        t += counts[s];
+    }
+    BUF_TRY(dst, sizeof(float4) * t);
+    for (int32_t s = 0; s < S; s++)
+        HIP_TRY(hipMemcpyAsync((char*)dst.p + sizeof(float4) * offsets[s], scans[s], sizeof(float4) * counts[s],
+                               hipMemcpyHostToDevice, c->stream));
+    *total = t;
+    return 0;
+}
+
+extern "C" int ndt_align_batch(ndt_ctx* c, const ndt_grid* g, const float* const* scans, const int64_t* counts,
+                               int32_t S, const double* guesses, const ndt_params* p, ndt_result* results) {
+    if (!c || !g || !scans || !counts || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
+    if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
+    int e = check_params(p);
+    if (e) return e;
+    if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    HIP_TRY(hipSetDevice(c->device));
+    std::vector<int64_t> offsets;
+    int64_t total = 0;
+    e = pack_scans(c, c->pts, scans, counts, S, offsets, &total);
+    if (e) return e;
+    return align_packed(c, g, c->pts.as<float4>(), offsets.data(), counts, S, guesses, p, results);
+}
+
+extern "C" int ndt_align(ndt_ctx* c, const ndt_grid* g, const float* scan, int64_t n, const double guess[16],
+                         const ndt_params* p, ndt_result* result) {
+    if (!scan) return fail(NDT_ERR_INVALID, "scan is null");
+    const float* scans[1] = {scan};
+    int64_t counts[1] = {n};
+    return ndt_align_batch(c, g, scans, counts, 1, guess, p, result);
+}
+
+extern "C" int ndt_scans_upload(ndt_ctx* c, const float* const* scans, const int64_t* counts, int32_t S,
+                                ndt_scans** out) {
+    if (!c || !scans || !counts || !out) return fail(NDT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
+    HIP_TRY(hipSetDevice(c->device));
+    ndt_scans* sc = new (std::nothrow) ndt_scans();
+    if (!sc) return fail(NDT_ERR_NOMEM, "host allocation failed");
+    sc->ctx = c;
+    sc->S = S;
+    sc->counts.assign(counts, counts + S);
+    int e = pack_scans(c, sc->pts, scans, counts, S, sc->offsets, &sc->total);
+    if (!e) {
+        hipError_t he = hipStreamSynchronize(c->stream);
+        if (he != hipSuccess) e = fail(NDT_ERR_HIP, hipGetErrorString(he));
+    }
+    if (e) {
+        delete sc;
+        return e;
+    }
+    *out = sc;
+    return 0;
+}
+
+extern "C" int ndt_scans_destroy(ndt_scans* sc) {
+    if (!sc) return 0;
+    (void)hipSetDevice(sc->ctx->device);
+    delete sc;
+    return 0;
+}
+
+extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* guesses,
+                               const ndt_params* p, ndt_result* results) {
+    if (!c || !g || !sc || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    HIP_TRY(hipSetDevice(c->device));
+    return align_packed(c, g, sc->pts.as<float4>(), sc->offsets.data(), sc->counts.data(), sc->S, guesses, p, results);
+}
+
+/* ------------------------------------------------------------------ single derivative pass */
+
+static int compat_pass(ndt_ctx* c, const ndt_grid* g, const float4* scan, int64_t n, const float M[16], const double p[6],
+                       double d1, double d2, double* red /* NRED_COMPAT */);
+
+extern "C" int ndt_derivatives(ndt_ctx* c, const ndt_grid* g, const float* scan, int64_t n, const double p[6],
+                               const float compat_transform[16], const ndt_params* params, double* score,
+                               double gradient[6], double hessian[36], int64_t* pairs) {
+    if (!c || !g || !scan || !p || !params) return fail(NDT_ERR_INVALID, "null argument");
+    if (n <= 0) return fail(NDT_ERR_INVALID, "empty scan");
+    int e = check_params(params);
+    if (e) return e;
+    if (params->mode != g->params.mode) return fail(NDT_ERR_INVALID, "params mode differs from the grid's");
+    HIP_TRY(hipSetDevice(c->device));
+    BUF_TRY(c->pts, sizeof(float4) * n);
+    HIP_TRY(hipMemcpyAsync(c->pts.p, scan, sizeof(float4) * n, hipMemcpyHostToDevice, c->stream));
+    if (params->mode == NDT_MODE_COMPAT) {
+        if (!compat_transform) return fail(NDT_ERR_INVALID, "compat mode needs the transform matrix");
+        double d1, d2;
+        gauss_consts((double)params->resolution, params->outlier_ratio, &d1, &d2);
+        double red[NRED_COMPAT];
+        e = compat_pass(c, g, c->pts.as<float4>(), n, compat_transform, p, d1, d2, red);
+        if (e) return e;
+        if (score) *score = red[0];
+        if (gradient)
+            for (int i = 0; i < 6; i++) gradient[i] = red[1 + i];
+        if (hessian)
+            for (int i = 0; i < 36; i++) hessian[i] = red[7 + i];
+        if (pairs) *pairs = (int64_t)red[NACC_COMPAT];
+        return 0;
+    }
+    AlignParams prm = make_align_params(params, 1);
+    const int nblocks = (int)std::min<int64_t>(1024, (n + 255) / 256);
+    BUF_TRY(c->partials2, sizeof(double) * NRED * nblocks + 64 * sizeof(double));
+    BUF_TRY(c->small, 64 * sizeof(double));
+    HIP_TRY(hipMemcpyAsync(c->small.p, p, sizeof(double) * 6, hipMemcpyHostToDevice, c->stream));
+    double* red_dev = c->partials2.as<double>() + (size_t)NRED * nblocks;
+    LAUNCH_TRY(launch_deriv_ndt(g->dev, c->pts.as<float4>(), n, c->small.as<double>(), prm, c->partials2.as<double>(),
+                                nblocks, c->stream));
+    LAUNCH_TRY(launch_reduce_partials(c->partials2.as<double>(), nblocks, NRED, red_dev, c->stream));
+    double red[NRED];
+    HIP_TRY(hipMemcpyAsync(red, red_dev, sizeof(red), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (score) *score = red[0];
+    if (gradient)
+        for (int i = 0; i < 6; i++) gradient[i] = red[1 + i];
+    if (hessian)
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) hessian[i * 6 + j] = red[7 + (i <= j ? tri_index(i, j) : tri_index(j, i))];
+    if (pairs) *pairs = (int64_t)red[NACC];
+    return 0;
+}
+
+/* ------------------------------------------------------------------ compat alignment (host controller) */
+
+static int compat_pass(ndt_ctx* c, const ndt_grid* g, const float4* scan, int64_t n, const float M[16], const double p[6],
+                       double d1, double d2, double* red) {
+    double jd[8][3], hd[15][3];
+    angle_tables(p, jd, hd); /* host libm, as the reference */
+    double jh[23 * 3];
+    for (int k = 0; k < 8; k++)
+        for (int q = 0; q < 3; q++) jh[k * 3 + q] = jd[k][q];
+    for (int k = 0; k < 15; k++)
+        for (int q = 0; q < 3; q++) jh[(8 + k) * 3 + q] = hd[k][q];
+    const int nblocks = (int)std::min<int64_t>(512, (n + 255) / 256);
+    BUF_TRY(c->partials2, sizeof(double) * NRED_COMPAT * (nblocks + 1));
+    double* red_dev = c->partials2.as<double>() + (size_t)NRED_COMPAT * nblocks;
+    LAUNCH_TRY(launch_deriv_compat(g->cdev, scan, n, M, jh, d1, d2, c->partials2.as<double>(), nblocks, c->stream));
+    LAUNCH_TRY(launch_reduce_partials(c->partials2.as<double>(), nblocks, NRED_COMPAT, red_dev, c->stream));
+    HIP_TRY(hipMemcpyAsync(red, red_dev, sizeof(double) * NRED_COMPAT, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+/* buildTransformationMatrix, kernel.cpp:949-1013 (rotation only; translation kept) */
+static void compat_build_matrix(float M[16], const double* t) {
+    const float q_ha = (float)(0.5f * t[3]);
+    const float q_w = (float)std::cos((double)q_ha);
+    const float q_x = (float)std::sin((double)q_ha);
+    const float q_y = 0.0f, q_z = 0.0f;
+    const float q_ha2 = (float)(0.5f * t[4]);
+    const float q_w2 = (float)std::cos((double)q_ha2);
+    const float q_x2 = 0.0f;
+    const float q_y2 = (float)std::sin((double)q_ha2);
+    const float q_z2 = 0.0f;
+    const float q_ha3 = (float)(0.5f * t[5]);
+    const float q_w3 = (float)std::cos((double)q_ha3);
+    const float q_x3 = 0.0f, q_y3 = 0.0f;
+    const float q_z3 = (float)std::sin((double)q_ha3);
+    const float r_x = q_w * q_w2 - q_x * q_x2 - q_y * q_y2 - q_z * q_z2;
+    const float r_y = q_w * q_x2 + q_x * q_w2 + q_y * q_z2 - q_z * q_y2;
+    const float r_z = q_w * q_y2 + q_y * q_w2 + q_z * q_x2 - q_x * q_z2;
+    const float r_w = q_w * q_z2 + q_z * q_w2 + q_x * q_y2 - q_y * q_x2;
+    const float r2_x = r_w * q_w3 - r_x * q_x3 - r_y * q_y3 - r_z * q_z3;
+    const float r2_y = r_w * q_x3 + r_x * q_w3 + r_y * q_z3 - r_z * q_y3;
+    const float r2_z = r_w * q_y3 + r_y * q_w3 + r_z * q_x3 - r_x * q_z3;
+    const float r2_w = r_w * q_z3 + r_z * q_w3 + r_x * q_y3 - r_y * q_x3;
+    const float tx = 2.0f * r2_x, ty = 2.0f * r2_y, tz = 2.0f * r2_z;
+    const float twx = tx * r2_w, twy = ty * r2_w, twz = tz * r2_w;
+    const float txx = tx * r2_x, txy = ty * r2_x, txz = tz * r2_x;
+    const float tyy = ty * r2_y, tyz = tz * r2_y, tzz = tz * r2_z;
+    M[12] = 0.0f; M[13] = 0.0f; M[14] = 0.0f; M[15] = 1.0f;
+    M[0] = 1.0f - (tyy + tzz);
+    M[1] = txy - twz;
+    M[2] = txz + twy;
+    M[4] = txy + twz;
+    M[5] = 1.0f - (txx + tzz);
+    M[6] = tyz - twx;
+    M[8] = txz - twy;
+    M[9] = tyz + twx;
+    M[10] = 1.0f - (txx + tyy);
+}
+
+/* eulerAngles, kernel.cpp:1146-1173 */
+static void compat_euler(const float M[16], double* result) {
+    const double PI_REF = 3.1415926535897932384626433832795;
+    double res[3];
+    res[0] = std::atan2((double)M[6], (double)M[10]);
+    double n1 = M[0], n2 = M[1];
+    double c2 = std::sqrt(n1 * n1 + n2 * n2);
+    if (res[0] > 0.0) {
+        res[0] -= PI_REF;
+        res[1] = std::atan2((double)(-M[2]), -c2);
+    } else {
+        res[1] = std::atan2((double)(-M[2]), c2);
+    }
+    double s1 = std::sin(res[0]), c1 = std::cos(res[0]);
+    res[2] = std::atan2(s1 * M[8] - c1 * M[4], c1 * M[5] - s1 * M[9]);
+    result[0] = -res[0];
+    result[1] = -res[1];
+    result[2] = -res[2];
+}
+
+extern "C" int ndt_compat_callback(ndt_ctx* c, const float* scan, int64_t n, const float guess[16], const float* map,
+                                   int64_t m, ndt_result* result) {
+    /* partial_points_callback -> ndt_align -> computeTransformation with the
+     * reference's parameters (kernel.cpp:1405-1414, :1375-1392, :1175-1272) */
+    if (!c || !scan || !guess || !map || !result) return fail(NDT_ERR_INVALID, "null argument");
+    if (n <= 0 || m <= 0) return fail(NDT_ERR_INVALID, "empty scan or map");
+    HIP_TRY(hipSetDevice(c->device));
+    ndt_params prm;
+    ndt_params_default(&prm, NDT_MODE_COMPAT);
+    ndt_grid* g = nullptr;
+    int e = ndt_grid_build(c, map, m, &prm, &g);
+    if (e) return e;
+    struct GridGuard {
+        ndt_grid* g;
+        ~GridGuard() { ndt_grid_destroy(g); }
+    } guard{g};
+    DevBuf scan_dev;
+    BUF_TRY(scan_dev, sizeof(float4) * n);
+    HIP_TRY(hipMemcpyAsync(scan_dev.p, scan, sizeof(float4) * n, hipMemcpyHostToDevice, c->stream));
+    double d1, d2;
+    gauss_consts((double)prm.resolution, prm.outlier_ratio, &d1, &d2);
+    float final_m[16];
+    std::memcpy(final_m, guess, sizeof(final_m));
+    double p[6], grad[6], H[36], red[NRED_COMPAT];
+    p[0] = final_m[3];
+    p[1] = final_m[7];
+    p[2] = final_m[11];
+    double ea[3];
+    compat_euler(final_m, ea);
+    p[3] = ea[0];
+    p[4] = ea[1];
+    p[5] = ea[2];
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    e = compat_pass(c, g, scan_dev.as<float4>(), n, final_m, p, d1, d2, red);
+    if (e) return e;
+    int passes = 1;
+    double pairs = red[NACC_COMPAT];
+    double score = red[0];
+    for (int i = 0; i < 6; i++) grad[i] = red[1 + i];
+    for (int i = 0; i < 36; i++) H[i] = red[7 + i];
+    const double step_size = prm.step_size, trans_eps = prm.trans_eps;
+    int converged = 0, nr_iterations = 0;
+    while (!converged) {
+        double neg_grad[6], delta_p[6];
+        for (int i = 0; i < 6; i++) neg_grad[i] = -grad[i];
+        solve6(delta_p, H, neg_grad);
+        double delta_p_norm = 1; /* kernel.cpp:1231 */
+        for (int i = 0; i < 6; i++) delta_p[i] /= delta_p_norm;
+        /* computeStepLengthMT (kernel.cpp:1015-1144): the interval starts converged,
+         * so only the first trial step a_t = clamp(1, 0.05, 0.1) is evaluated */
+        double step_init = delta_p_norm, step_max = step_size, step_min = trans_eps / 2;
+        double phi_0 = -score;
+        double d_phi_0 = -dot6(grad, delta_p);
+        double a_t = 0;
+        if (!(d_phi_0 >= 0 && d_phi_0 == 0)) {
+            if (d_phi_0 >= 0) {
+                d_phi_0 *= -1;
+                for (int i = 0; i < 6; i++) delta_p[i] *= -1;
+            }
+            (void)phi_0;
+            a_t = step_init;
+            a_t = dmin_std(a_t, step_max);
+            a_t = dmax_std(a_t, step_min);
+            double x_t[6];
+            for (int i = 0; i < 6; i++) x_t[i] = p[i] + delta_p[i] * a_t;
+            compat_build_matrix(final_m, x_t);
+            e = compat_pass(c, g, scan_dev.as<float4>(), n, final_m, x_t, d1, d2, red);
+            if (e) return e;
+            passes++;
+            pairs += red[NACC_COMPAT];
+            score = red[0];
+            for (int i = 0; i < 6; i++) grad[i] = red[1 + i];
+            for (int i = 0; i < 36; i++) H[i] = red[7 + i];
+        }
+        delta_p_norm = a_t;
+        for (int i = 0; i < 6; i++) delta_p[i] *= delta_p_norm;
+        for (int i = 0; i < 6; i++) p[i] = p[i] + delta_p[i];
+        if (nr_iterations > prm.max_iterations || (nr_iterations && (std::fabs(delta_p_norm) < trans_eps))) converged = 1;
+        nr_iterations++;
+    }
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 2 * passes;
+    c->last_pairs = (int64_t)pairs;
+    std::memset(result, 0, sizeof(*result));
+    for (int k = 0; k < 16; k++) {
+        result->final_transformation[k] = final_m[k];
+        result->final_transformation_f64[k] = final_m[k];
+    }
+    result->score = score;
+    result->trans_probability = score / (double)n;
+    result->iterations = nr_iterations;
+    result->converged = converged;
+    result->passes = passes;
+    return 0;
+}

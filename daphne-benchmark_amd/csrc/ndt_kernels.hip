@@ -1,0 +1,1554 @@
+/*
+ * ndt_kernels.hip - gfx950 kernels of the NDT scan-matching engine.
+ *
+ *  grid build   bbox reduce, voxel keys, stable LSD radix sort, run heads/starts,
+ *               per-voxel float64 statistics (ndt: two-pass covariance, Jacobi
+ *               eigenvalue clamp, inverse; compat: the reference's single-pass
+ *               normalisation), neighbourhood dilation, hash build
+ *  align        fused per-point kernel: transform, hashed cell lookup, candidate
+ *               list scan, score / gradient / Hessian per pair, wave64 shuffle +
+ *               LDS reductions; persistent cooperative kernel that keeps every
+ *               scan's Newton / More-Thuente state on the device
+ *  compat       the reference's per-pair arithmetic in float64 on the GPU
+ *
+ * The file is compiled with -ffp-contract=off so that float64 control code and
+ * the compat path keep the reference's rounding; the float32 hot loop opts into
+ * contraction explicitly.
+ */
+#include <hip/hip_runtime.h>
+#include <hip/hip_cooperative_groups.h>
+
+#include "ndt_internal.h"
+
+namespace ndt {
+
+constexpr int WAVE = 64;
+
+/* ------------------------------------------------------------------ helpers */
+
+__device__ inline uint64_t ld_agent_u64(const void* p) {
+    return __hip_atomic_load((uint64_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent_u64(void* p, uint64_t v) {
+    __hip_atomic_store((uint64_t*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline double ld_agent_f64(const double* p) { return __longlong_as_double((long long)ld_agent_u64(p)); }
+__device__ inline void st_agent_f64(double* p, double v) { st_agent_u64(p, (uint64_t)__double_as_longlong(v)); }
+__device__ inline int32_t ld_agent_i32(const int32_t* p) {
+    return __hip_atomic_load((int32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+    return v;
+}
+
+static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+/* ------------------------------------------------------------------ bbox */
+
+__global__ void k_bbox(const float4* __restrict__ pts, int64_t n, uint32_t* __restrict__ mm) {
+    uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float4 p = pts[i];
+        if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) continue;
+        uint32_t o[3] = {f2ord(p.x), f2ord(p.y), f2ord(p.z)};
+        for (int k = 0; k < 3; k++) {
+            lo[k] = o[k] < lo[k] ? o[k] : lo[k];
+            hi[k] = o[k] > hi[k] ? o[k] : hi[k];
+        }
+    }
+    for (int k = 0; k < 3; k++) {
+        for (int off = WAVE / 2; off > 0; off >>= 1) {
+            uint32_t a = __shfl_xor(lo[k], off, WAVE), b = __shfl_xor(hi[k], off, WAVE);
+            lo[k] = a < lo[k] ? a : lo[k];
+            hi[k] = b > hi[k] ? b : hi[k];
+        }
+    }
+    if ((threadIdx.x & (WAVE - 1)) == 0) {
+        for (int k = 0; k < 3; k++) {
+            atomicMin(&mm[k], lo[k]);
+            atomicMax(&mm[3 + k], hi[k]);
+        }
+    }
+}
+
+int launch_bbox(const float4* pts, int64_t n, uint32_t* mm, hipStream_t stream) {
+    int blocks = grid_for(n, 256);
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, stream, pts, n, mm);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ keys */
+
+struct KeyLayout {
+    int64_t o[3];
+    int b[3];
+};
+
+__global__ void k_keys_ndt(const float4* __restrict__ pts, int64_t n, double res, KeyLayout L,
+                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pts[i];
+    const uint64_t sentinel = 1ull << (L.b[0] + L.b[1] + L.b[2]);
+    uint64_t key = sentinel;
+    if (isfinite(p.x) && isfinite(p.y) && isfinite(p.z)) {
+        int64_t c[3] = {(int64_t)floor((double)p.x / res) - L.o[0], (int64_t)floor((double)p.y / res) - L.o[1],
+                        (int64_t)floor((double)p.z / res) - L.o[2]};
+        bool ok = true;
+        for (int k = 0; k < 3; k++) ok = ok && c[k] >= 0 && c[k] < (1ll << L.b[k]);
+        if (ok) key = ((uint64_t)c[2] << (L.b[0] + L.b[1])) | ((uint64_t)c[1] << L.b[0]) | (uint64_t)c[0];
+    }
+    keys[i] = key;
+    vals[i] = (uint32_t)i;
+}
+
+int launch_keys_ndt(const float4* pts, int64_t n, double resolution, const int64_t* origin_cell, const int* bits,
+                    uint64_t* keys, uint32_t* vals, hipStream_t stream) {
+    KeyLayout L;
+    for (int k = 0; k < 3; k++) {
+        L.o[k] = origin_cell[k];
+        L.b[k] = bits[k];
+    }
+    hipLaunchKernelGGL(k_keys_ndt, dim3(grid_for(n, 256)), dim3(256), 0, stream, pts, n, resolution, L, keys, vals);
+    return (int)hipGetLastError();
+}
+
+struct CompatLayout {
+    float minv[3];
+    int dims[3];
+    float res;
+};
+
+/* linearizeCoord / linearizeAddr, src/CPU/ndt_mapping/kernel.cpp:231-244 */
+__device__ inline int compat_linearize(const float* minv, const int* dims, float res, float x, float y, float z) {
+    int ix = (int)((x - minv[0]) / res);
+    int iy = (int)((y - minv[1]) / res);
+    int iz = (int)((z - minv[2]) / res);
+    return ix + dims[0] * (iy + dims[1] * iz);
+}
+
+__global__ void k_keys_compat(const float4* __restrict__ pts, int64_t n, CompatLayout L, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pts[i];
+    keys[i] = (uint64_t)(uint32_t)compat_linearize(L.minv, L.dims, L.res, p.x, p.y, p.z);
+    vals[i] = (uint32_t)i;
+}
+
+int launch_keys_compat(const float4* pts, int64_t n, const float* minv, const int* dims, float resolution,
+                       uint64_t* keys, uint32_t* vals, hipStream_t stream) {
+    CompatLayout L;
+    for (int k = 0; k < 3; k++) {
+        L.minv[k] = minv[k];
+        L.dims[k] = dims[k];
+    }
+    L.res = resolution;
+    hipLaunchKernelGGL(k_keys_compat, dim3(grid_for(n, 256)), dim3(256), 0, stream, pts, n, L, keys, vals);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ radix sort
+ * Stable LSD radix sort, 8-bit digits.  Per pass: per-tile digit histograms
+ * (digit-major), one exclusive scan, and a stable scatter in which each tile
+ * ranks its keys round by round with wave ballots. */
+
+constexpr int RS_BLOCK = 256;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_BLOCK * RS_ITEMS;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_hist(const uint64_t* __restrict__ keys, int64_t n, int shift,
+                                                      uint32_t* __restrict__ hist, int nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = base + r * RS_BLOCK + threadIdx.x;
+        if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                         uint64_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                         int64_t n, int shift, const uint32_t* __restrict__ offs,
+                                                         int nblocks) {
+    __shared__ uint32_t base[256];
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_BLOCK / WAVE][256];
+    const int tid = threadIdx.x;
+    const int wave = tid / WAVE, lane = tid % WAVE;
+    base[tid] = offs[(int64_t)tid * nblocks + blockIdx.x];
+    run[tid] = 0;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int64_t tbase = (int64_t)blockIdx.x * RS_TILE;
+    for (int r = 0; r < RS_ITEMS; r++) {
+        int64_t idx = tbase + r * RS_BLOCK + tid;
+        bool valid = idx < n;
+        uint64_t key = valid ? kin[idx] : 0;
+        uint32_t val = valid ? vin[idx] : 0;
+        uint32_t d = valid ? (uint32_t)((key >> shift) & 255u) : 256u;
+        uint64_t peers = __ballot(1);
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        uint32_t rank = (uint32_t)__popcll(peers & lt_mask);
+        for (int w = 0; w < RS_BLOCK / WAVE; w++) wcnt[w][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            uint32_t acc = run[tid];
+            for (int w = 0; w < RS_BLOCK / WAVE; w++) {
+                uint32_t c = wcnt[w][tid];
+                wcnt[w][tid] = acc;
+                acc += c;
+            }
+            run[tid] = acc;
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[d] + wcnt[wave][d] + rank;
+            kout[pos] = key;
+            vout[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ scan */
+
+constexpr int SC_BLOCK = 256;
+constexpr int SC_ITEMS = 16;
+constexpr int SC_TILE = SC_BLOCK * SC_ITEMS;
+
+/* exclusive block scan of one value per thread; returns the block total */
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* excl, uint32_t* lds_waves) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    uint32_t incl = v;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+        uint32_t t = __shfl_up(incl, off, WAVE);
+        if (lane >= off) incl += t;
+    }
+    if (lane == WAVE - 1) lds_waves[wave] = incl;
+    __syncthreads();
+    uint32_t wave_off = 0, total = 0;
+    for (int w = 0; w < (int)(blockDim.x / WAVE); w++) {
+        uint32_t c = lds_waves[w];
+        if (w < wave) wave_off += c;
+        total += c;
+    }
+    __syncthreads();
+    *excl = wave_off + incl - v;
+    return total;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n,
+                                                          uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t lw[SC_BLOCK / WAVE];
+    int64_t base = (int64_t)blockIdx.x * SC_TILE;
+    uint32_t s = 0;
+    for (int r = 0; r < SC_ITEMS; r++) {
+        int64_t idx = base + r * SC_BLOCK + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    uint32_t ex;
+    uint32_t total = block_exclusive_scan(s, &ex, lw);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ bsum, int64_t nb, uint32_t* __restrict__ total) {
+    __shared__ uint32_t lw[1024 / WAVE];
+    const int64_t per = (nb + blockDim.x - 1) / blockDim.x;
+    const int64_t b = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int64_t i = b; i < b + per && i < nb; i++) s += bsum[i];
+    uint32_t ex;
+    uint32_t tot = block_exclusive_scan(s, &ex, lw);
+    uint32_t run = ex;
+    for (int64_t i = b; i < b + per && i < nb; i++) {
+        uint32_t v = bsum[i];
+        bsum[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+__global__ __launch_bounds__(SC_BLOCK) void k_scan_down(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                        int64_t n, const uint32_t* __restrict__ bsum) {
+    __shared__ uint32_t lw[SC_BLOCK / WAVE];
+    /* blocked arrangement: thread t owns items [t*16, t*16+16) of the tile */
+    int64_t base = (int64_t)blockIdx.x * SC_TILE + (int64_t)threadIdx.x * SC_ITEMS;
+    uint32_t v[SC_ITEMS];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        v[k] = (base + k < n) ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t ex;
+    block_exclusive_scan(s, &ex, lw);
+    uint32_t run = ex + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+}
+
+size_t scan_scratch_bytes(int64_t n) {
+    int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+    return (size_t)(nb + 1) * sizeof(uint32_t);
+}
+
+int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* scratch,
+                       hipStream_t stream) {
+    if (n <= 0) return 0;
+    int64_t nb = (n + SC_TILE - 1) / SC_TILE;
+    uint32_t* bsum = (uint32_t*)scratch;
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BLOCK), 0, stream, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, stream, bsum, nb, total_dev);
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(SC_BLOCK), 0, stream, in, out, n, bsum);
+    return (int)hipGetLastError();
+}
+
+size_t radix_sort_scratch_bytes(int64_t n) {
+    int64_t nb = (n + RS_TILE - 1) / RS_TILE;
+    int64_t h = 256 * nb;
+    return (size_t)h * sizeof(uint32_t) + scan_scratch_bytes(h) + 256;
+}
+
+int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp, uint32_t* vals_tmp, int64_t n, int nbits,
+                     void* scratch, hipStream_t stream) {
+    if (n <= 1 || nbits <= 0) return 0;
+    int nb = (int)((n + RS_TILE - 1) / RS_TILE);
+    int64_t h = 256ll * nb;
+    uint32_t* hist = (uint32_t*)scratch;
+    void* sc = (void*)(((uintptr_t)(hist + h) + 255) & ~(uintptr_t)255);
+    uint64_t *ki = keys, *ko = keys_tmp;
+    uint32_t *vi = vals, *vo = vals_tmp;
+    int passes = (nbits + 7) / 8;
+    for (int p = 0; p < passes; p++) {
+        int shift = 8 * p;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_BLOCK), 0, stream, ki, n, shift, hist, nb);
+        int e = exclusive_scan_u32(hist, hist, h, nullptr, sc, stream);
+        if (e) return e;
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_BLOCK), 0, stream, ki, vi, ko, vo, n, shift, hist, nb);
+        uint64_t* tk = ki; ki = ko; ko = tk;
+        uint32_t* tv = vi; vi = vo; vo = tv;
+    }
+    if (ki != keys) {
+        hipMemcpyAsync(keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, stream);
+        hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream);
+    }
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ runs */
+
+__global__ void k_run_heads(const uint64_t* __restrict__ keys, int64_t n, int shift, uint64_t limit,
+                            uint32_t* __restrict__ flags) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    uint32_t f = 0;
+    if (k < limit) f = (i == 0) || ((keys[i - 1] >> shift) != (k >> shift));
+    flags[i] = f;
+}
+
+int launch_run_heads(const uint64_t* keys, int64_t n, int shift, uint64_t limit, uint32_t* flags,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_run_heads, dim3(grid_for(n, 256)), dim3(256), 0, stream, keys, n, shift, limit, flags);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_run_starts(const uint32_t* __restrict__ flags, const uint32_t* __restrict__ ranks, int64_t n,
+                             uint32_t* __restrict__ starts) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flags[i]) starts[ranks[i]] = (uint32_t)i;
+}
+
+int launch_run_starts(const uint32_t* flags, const uint32_t* ranks, int64_t n, uint32_t* starts, hipStream_t stream) {
+    hipLaunchKernelGGL(k_run_starts, dim3(grid_for(n, 256)), dim3(256), 0, stream, flags, ranks, n, starts);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ voxel statistics */
+
+/* cyclic Jacobi eigen-decomposition of a symmetric 3x3 matrix: ascending
+ * eigenvalues, eigenvectors in the columns of V */
+__device__ void sym3_eig(const double* Ain, double* ev, double* V) {
+    double a[3][3], v[3][3];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            a[r][c] = Ain[r * 3 + c];
+            v[r][c] = (r == c) ? 1.0 : 0.0;
+        }
+    for (int sweep = 0; sweep < 50; sweep++) {
+        double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= 1e-36 * diag || off == 0.0) break;
+        for (int pq = 0; pq < 3; pq++) {
+            int p = (pq == 2) ? 1 : 0;
+            int q = (pq == 0) ? 1 : 2;
+            if (a[p][q] == 0.0) continue;
+            double theta = (a[q][q] - a[p][p]) / (2.0 * a[p][q]);
+            double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+            double c = 1.0 / sqrt(t * t + 1.0);
+            double s = t * c;
+            for (int k = 0; k < 3; k++) {
+                double akp = a[k][p], akq = a[k][q];
+                a[k][p] = c * akp - s * akq;
+                a[k][q] = s * akp + c * akq;
+            }
+            for (int k = 0; k < 3; k++) {
+                double apk = a[p][k], aqk = a[q][k];
+                a[p][k] = c * apk - s * aqk;
+                a[q][k] = s * apk + c * aqk;
+            }
+            for (int k = 0; k < 3; k++) {
+                double vkp = v[k][p], vkq = v[k][q];
+                v[k][p] = c * vkp - s * vkq;
+                v[k][q] = s * vkp + c * vkq;
+            }
+        }
+    }
+    int idx[3] = {0, 1, 2};
+    double d[3] = {a[0][0], a[1][1], a[2][2]};
+    for (int i = 0; i < 2; i++)
+        for (int j = i + 1; j < 3; j++)
+            if (d[idx[j]] < d[idx[i]]) {
+                int t = idx[i];
+                idx[i] = idx[j];
+                idx[j] = t;
+            }
+    for (int i = 0; i < 3; i++) {
+        ev[i] = d[idx[i]];
+        for (int r = 0; r < 3; r++) V[r * 3 + i] = v[r][idx[i]];
+    }
+}
+
+__global__ void k_voxel_stats_ndt(const float4* __restrict__ pts, const uint32_t* __restrict__ sidx,
+                                  const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ seg_start,
+                                  int64_t nseg, int64_t nvalid_pts, int min_points, double eig_ratio,
+                                  double* __restrict__ mean_out, double* __restrict__ icov_out,
+                                  double* __restrict__ cov_out, int32_t* __restrict__ count,
+                                  uint32_t* __restrict__ valid, uint64_t* __restrict__ vkey) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    int64_t b = seg_start[s];
+    int64_t e = (s + 1 < nseg) ? (int64_t)seg_start[s + 1] : nvalid_pts;
+    int64_t n = e - b;
+    double sum[3] = {0, 0, 0};
+    for (int64_t k = b; k < e; k++) {
+        float4 p = pts[sidx[k]];
+        sum[0] += p.x;
+        sum[1] += p.y;
+        sum[2] += p.z;
+    }
+    double mean[3] = {sum[0] / (double)n, sum[1] / (double)n, sum[2] / (double)n};
+    double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = b; k < e; k++) {
+        float4 p = pts[sidx[k]];
+        double d[3] = {p.x - mean[0], p.y - mean[1], p.z - mean[2]};
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) C[r * 3 + c] += d[r] * d[c];
+    }
+    uint32_t ok = 0;
+    double icov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, cov[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (n >= min_points && n >= 2) {
+        for (int k = 0; k < 9; k++) C[k] /= (double)(n - 1);
+        double ev[3], V[9];
+        sym3_eig(C, ev, V);
+        if (ev[2] > 0 && isfinite(ev[2])) {
+            double lo = eig_ratio * ev[2];
+            for (int k = 0; k < 3; k++)
+                if (ev[k] < lo) ev[k] = lo;
+            for (int r = 0; r < 3; r++)
+                for (int c = 0; c < 3; c++) {
+                    double a = 0, bb = 0;
+                    for (int k = 0; k < 3; k++) {
+                        a += V[r * 3 + k] * ev[k] * V[c * 3 + k];
+                        bb += V[r * 3 + k] * (1.0 / ev[k]) * V[c * 3 + k];
+                    }
+                    cov[r * 3 + c] = a;
+                    icov[r * 3 + c] = bb;
+                }
+            ok = 1;
+        }
+    }
+    for (int c = 0; c < 3; c++) mean_out[3 * s + c] = mean[c];
+    for (int k = 0; k < 9; k++) {
+        icov_out[9 * s + k] = icov[k];
+        cov_out[9 * s + k] = cov[k];
+    }
+    count[s] = (int32_t)n;
+    valid[s] = ok;
+    vkey[s] = skeys[b];
+}
+
+int launch_voxel_stats_ndt(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
+                           const uint32_t* seg_start, int64_t nseg, int64_t nvalid_pts, int min_points,
+                           double min_eig_ratio, const int* bits, double* mean, double* icov, double* cov,
+                           int32_t* count, uint32_t* valid, uint64_t* vkey, hipStream_t stream) {
+    (void)bits;
+    hipLaunchKernelGGL(k_voxel_stats_ndt, dim3(grid_for(nseg, 128)), dim3(128), 0, stream, pts, sorted_idx,
+                       sorted_keys, seg_start, nseg, nvalid_pts, min_points, min_eig_ratio, mean, icov, cov, count,
+                       valid, vkey);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_compact_voxels(const uint32_t* __restrict__ valid, const uint32_t* __restrict__ rank, int64_t nvox,
+                                 const double* __restrict__ mean, const double* __restrict__ icov,
+                                 const uint64_t* __restrict__ vkey, double* __restrict__ mean_out,
+                                 double* __restrict__ icov_out, float4* __restrict__ icov_f,
+                                 uint64_t* __restrict__ vkey_out) {
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= nvox || !valid[v]) return;
+    int64_t r = rank[v];
+    for (int c = 0; c < 3; c++) mean_out[3 * r + c] = mean[3 * v + c];
+    const double* C = icov + 9 * v;
+    double u[6] = {C[0], C[1], C[2], C[4], C[5], C[8]};
+    for (int k = 0; k < 6; k++) icov_out[6 * r + k] = u[k];
+    icov_f[2 * r] = make_float4((float)u[0], (float)u[1], (float)u[2], (float)u[3]);
+    icov_f[2 * r + 1] = make_float4((float)u[4], (float)u[5], 0.0f, 0.0f);
+    vkey_out[r] = vkey[v];
+}
+
+int launch_compact_voxels(const uint32_t* valid, const uint32_t* rank, int64_t nvox, const double* mean,
+                          const double* icov, const uint64_t* vkey, double* mean_out, double* icov_out, float4* icov_f,
+                          uint64_t* vkey_out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_compact_voxels, dim3(grid_for(nvox, 256)), dim3(256), 0, stream, valid, rank, nvox, mean,
+                       icov, vkey, mean_out, icov_out, icov_f, vkey_out);
+    return (int)hipGetLastError();
+}
+
+struct Bits3 {
+    int b[3];
+};
+
+/* each valid voxel announces itself to the 27 query cells that see it; the
+ * composite key orders a query cell's list by probe index (z, then y, then x) */
+__global__ void k_dilate_emit(const uint64_t* __restrict__ vkey, int64_t nvalid, Bits3 B, uint64_t* __restrict__ keys,
+                              uint32_t* __restrict__ vals) {
+    int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= nvalid) return;
+    uint64_t k = vkey[r];
+    int64_t c[3];
+    c[0] = (int64_t)(k & ((1ull << B.b[0]) - 1));
+    c[1] = (int64_t)((k >> B.b[0]) & ((1ull << B.b[1]) - 1));
+    c[2] = (int64_t)(k >> (B.b[0] + B.b[1]));
+    int o = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++, o++) {
+                /* voxel = query + (dx, dy, dz) */
+                uint64_t qx = (uint64_t)(c[0] - dx), qy = (uint64_t)(c[1] - dy), qz = (uint64_t)(c[2] - dz);
+                uint64_t q = (qz << (B.b[0] + B.b[1])) | (qy << B.b[0]) | qx;
+                keys[27 * r + o] = (q << 5) | (uint64_t)o;
+                vals[27 * r + o] = (uint32_t)r;
+            }
+}
+
+int launch_dilate_emit(const uint64_t* vkey, int64_t nvalid, const int* bits, uint64_t* keys, uint32_t* vals,
+                       hipStream_t stream) {
+    Bits3 B;
+    for (int k = 0; k < 3; k++) B.b[k] = bits[k];
+    hipLaunchKernelGGL(k_dilate_emit, dim3(grid_for(nvalid, 256)), dim3(256), 0, stream, vkey, nvalid, B, keys, vals);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_hash_clear(HashSlot* slots, int64_t cap) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= cap) return;
+    slots[i].key = EMPTY_KEY;
+    slots[i].begin = 0;
+    slots[i].count = 0;
+}
+
+int launch_hash_clear(HashSlot* slots, int64_t cap, hipStream_t stream) {
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(cap, 256)), dim3(256), 0, stream, slots, cap);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_hash_insert(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ starts, int64_t nq,
+                              int64_t nentries, int shift, HashSlot* __restrict__ slots, int log2cap) {
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    uint32_t b = starts[q];
+    uint32_t e = (q + 1 < nq) ? starts[q + 1] : (uint32_t)nentries;
+    uint64_t key = skeys[b] >> shift;
+    uint64_t mask = (1ull << log2cap) - 1;
+    uint64_t h = hash_slot(key, log2cap);
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&slots[h].key, (unsigned long long)EMPTY_KEY,
+                                            (unsigned long long)key);
+        if (prev == EMPTY_KEY) {
+            slots[h].begin = b;
+            slots[h].count = e - b;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int64_t nq, int64_t nentries, int shift,
+                       HashSlot* slots, int log2cap, hipStream_t stream) {
+    hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(nq, 256)), dim3(256), 0, stream, sorted_keys, starts, nq, nentries,
+                       shift, slots, log2cap);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_hash_insert_ids(const uint64_t* __restrict__ vkey, int64_t nvox, HashSlot* __restrict__ slots,
+                                  int log2cap) {
+    int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    uint64_t key = vkey[v];
+    uint64_t mask = (1ull << log2cap) - 1;
+    uint64_t h = hash_slot(key, log2cap);
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&slots[h].key, (unsigned long long)EMPTY_KEY,
+                                            (unsigned long long)key);
+        if (prev == EMPTY_KEY) {
+            slots[h].begin = (uint32_t)v;
+            slots[h].count = 1;
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, int log2cap, hipStream_t stream) {
+    hipLaunchKernelGGL(k_hash_insert_ids, dim3(grid_for(nvox, 256)), dim3(256), 0, stream, vkey, nvox, slots, log2cap);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_cand_fill(const uint32_t* __restrict__ svals, int64_t n, const double* __restrict__ mean,
+                            float4* __restrict__ cand) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t v = svals[i];
+    cand[i] = make_float4((float)mean[3 * v], (float)mean[3 * v + 1], (float)mean[3 * v + 2], __uint_as_float(v));
+}
+
+int launch_cand_fill(const uint32_t* sorted_vals, int64_t nentries, const double* mean, float4* cand,
+                     hipStream_t stream) {
+    hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_vals, nentries, mean,
+                       cand);
+    return (int)hipGetLastError();
+}
+
+/* initCompute's scatter and normalisation (src/CPU/ndt_mapping/kernel.cpp:1335-1372)
+ * per occupied cell, summing its points in index order as the reference does. */
+__global__ void k_voxel_stats_compat(const float4* __restrict__ pts, const uint32_t* __restrict__ sidx,
+                                     const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ seg_start,
+                                     int64_t nseg, int64_t npts, double ncells, double* __restrict__ mean_out,
+                                     double* __restrict__ icov_out, int32_t* __restrict__ count,
+                                     uint64_t* __restrict__ vkey) {
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= nseg) return;
+    int64_t b = seg_start[s];
+    int64_t e = (s + 1 < nseg) ? (int64_t)seg_start[s + 1] : npts;
+    double m[3] = {0, 0, 0};
+    double S[3][3] = {{0, 0, 1.0}, {0, 1.0, 0}, {1.0, 0, 0}};
+    int n = 0;
+    for (int64_t k = b; k < e; k++) {
+        float4 p4 = pts[sidx[k]];
+        float p[3] = {p4.x, p4.y, p4.z};
+        m[0] += p[0];
+        m[1] += p[1];
+        m[2] += p[2];
+        n++;
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) S[r][c] += (double)(p[r] * p[c]);
+    }
+    double ps[3] = {m[0], m[1], m[2]};
+    m[0] /= n;
+    m[1] /= n;
+    m[2] /= n;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            S[r][c] = (S[r][c] - 2 * (ps[r] * m[c])) / ncells + m[r] * m[c];
+            S[r][c] *= (ncells - 1.0) / n;
+        }
+    /* invertMatrix, kernel.cpp:1277-1300 */
+    double t[3][3];
+    double det = S[0][0] * (S[2][2] * S[1][1] - S[2][1] * S[1][2]) - S[1][0] * (S[2][2] * S[0][1] - S[2][1] * S[0][2]) +
+                 S[2][0] * (S[1][2] * S[0][1] - S[1][1] * S[0][2]);
+    double invDet = 1.0 / det;
+    t[0][0] = S[2][2] * S[1][1] - S[2][1] * S[1][2];
+    t[0][1] = -(S[2][2] * S[0][1] - S[2][1] * S[0][2]);
+    t[0][2] = S[1][2] * S[0][1] - S[1][1] * S[0][2];
+    t[1][0] = -(S[2][2] * S[0][1] - S[2][0] * S[1][2]);
+    t[1][1] = S[2][2] * S[0][0] - S[2][1] * S[0][2];
+    t[1][2] = -(S[1][2] * S[0][0] - S[1][0] * S[0][2]);
+    t[2][0] = S[2][1] * S[1][0] - S[2][0] * S[1][1];
+    t[2][1] = -(S[2][1] * S[0][0] - S[2][0] * S[0][1]);
+    t[2][2] = S[1][1] * S[0][0] - S[1][0] * S[0][1];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) icov_out[9 * s + 3 * r + c] = t[r][c] * invDet;
+    for (int c = 0; c < 3; c++) mean_out[3 * s + c] = m[c];
+    count[s] = n;
+    vkey[s] = skeys[b];
+}
+
+int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
+                              const uint32_t* seg_start, int64_t nseg, int64_t npts, double ncells, double* mean,
+                              double* icov, int32_t* count, uint64_t* vkey, hipStream_t stream) {
+    hipLaunchKernelGGL(k_voxel_stats_compat, dim3(grid_for(nseg, 128)), dim3(128), 0, stream, pts, sorted_idx,
+                       sorted_keys, seg_start, nseg, npts, ncells, mean, icov, count, vkey);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ ndt per-point math */
+
+/* Per-pass values shared by every point of a scan.  The angular derivatives of
+ * T(p) x = Rx(roll) Ry(pitch) Rz(yaw) x + t are written with the rotation axes
+ * a3 = e_x, a4 = Rx e_y, a5 = Rx Ry e_z and the rotated point y = R x:
+ *   dT/dtheta_k = a_k x y,   d2T/dtheta_i dtheta_j = a_i x (a_j x y)  (i <= j),
+ * which is eq. 6.19 / 6.21 (the j_ang / h_ang tables, kernel.cpp:707-825) in
+ * closed form.  The tables' small-angle rule (cos = 1, sin = 0 below 1e-4) is kept
+ * by evaluating the axes and Ra (used for y in the derivatives) at the angles
+ * with small components set to zero; R and t (the transform) stay exact. */
+template <typename T>
+struct PassParams {
+    T R[9], t[3], Ra[9], a4[3], a5[3];
+    T res, res2, d1, d2, nhd2; /* nhd2 = -d2 / 2 */
+};
+
+__device__ __attribute__((noinline)) void pass_params_f64(const double* x, const AlignParams& prm, double res,
+                                                          double* out /* 32 */) {
+    double M[16];
+    pose_matrix(x, M);
+    double xa[6];
+    for (int k = 0; k < 6; k++) xa[k] = (k >= 3 && fabs(x[k]) < 10e-5) ? 0.0 : x[k];
+    double Ma[16];
+    pose_matrix(xa, Ma);
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) {
+            out[r * 3 + c] = M[r * 4 + c];
+            out[12 + r * 3 + c] = Ma[r * 4 + c];
+        }
+        out[9 + r] = M[r * 4 + 3];
+    }
+    const double c3 = cos(xa[3]), s3 = sin(xa[3]), c4 = cos(xa[4]), s4 = sin(xa[4]);
+    out[21] = 0.0;
+    out[22] = c3;
+    out[23] = s3;
+    out[24] = s4;
+    out[25] = -s3 * c4;
+    out[26] = c3 * c4;
+    out[27] = res;
+    out[28] = res * res;
+    out[29] = prm.d1;
+    out[30] = prm.d2;
+    out[31] = -prm.d2 / 2.0;
+}
+
+template <typename T>
+__device__ inline void make_pass_params(const double* x, const AlignParams& prm, double res, PassParams<T>& pp) {
+    double v[32];
+    pass_params_f64(x, prm, res, v);
+    T* o = (T*)&pp;
+    for (int k = 0; k < 32; k++) o[k] = (T)v[k];
+}
+
+__device__ inline float uniform_value(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline double uniform_value(double v) {
+    long long b = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(uint32_t)lo);
+}
+
+/* copies the LDS pass parameters into wave-uniform (scalar) registers */
+template <typename T>
+__device__ inline PassParams<T> load_uniform(const PassParams<T>& lds) {
+    PassParams<T> u;
+    const T* src = (const T*)&lds;
+    T* dst = (T*)&u;
+#pragma unroll
+    for (int k = 0; k < 32; k++) dst[k] = uniform_value(src[k]);
+    return u;
+}
+
+/* Accumulator of the 28 sums (score, gradient, Hessian upper triangle). */
+template <typename T, int SUM>
+struct Acc;
+
+template <typename T>
+struct Acc<T, NDT_SUM_FP64> {
+    double s[NACC];
+    __device__ void zero() {
+        for (int k = 0; k < NACC; k++) s[k] = 0.0;
+    }
+    __device__ void add(int k, T v) { s[k] += (double)v; }
+    __device__ double get(int k) const { return s[k]; }
+};
+
+template <typename T>
+struct Acc<T, NDT_SUM_FP32> {
+    float s[NACC];
+    __device__ void zero() {
+        for (int k = 0; k < NACC; k++) s[k] = 0.0f;
+    }
+    __device__ void add(int k, T v) { s[k] += (float)v; }
+    __device__ double get(int k) const { return (double)s[k]; }
+};
+
+template <typename T>
+struct Acc<T, NDT_SUM_COMP32> {
+    float s[NACC], c[NACC];
+    __device__ void zero() {
+        for (int k = 0; k < NACC; k++) s[k] = c[k] = 0.0f;
+    }
+    __device__ void add(int k, T v) {
+        /* Neumaier's compensated summation */
+        float x = (float)v;
+        float t = s[k] + x;
+        if (fabsf(s[k]) >= fabsf(x)) c[k] += (s[k] - t) + x;
+        else c[k] += (x - t) + s[k];
+        s[k] = t;
+    }
+    __device__ double get(int k) const { return (double)s[k] + (double)c[k]; }
+};
+
+__device__ inline bool hash_find(const HashSlot* __restrict__ slots, int log2cap, uint64_t key, uint32_t& begin,
+                                 uint32_t& count) {
+    uint64_t mask = (1ull << log2cap) - 1;
+    uint64_t h = hash_slot(key, log2cap);
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        HashSlot s = slots[h];
+        if (s.key == key) {
+            begin = s.begin;
+            count = s.count;
+            return true;
+        }
+        if (s.key == EMPTY_KEY) return false;
+        h = (h + 1) & mask;
+    }
+    return false;
+}
+
+/* One scan point: transform, hashed lookup of its cell's candidate list, and the
+ * score / gradient / Hessian (Magnusson 2009 eqs. 6.9-6.13) of every candidate
+ * voxel whose mean lies within one resolution.  Returns the number of pairs. */
+template <typename T, int SUM>
+__device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, const PassParams<T>& u, Acc<T, SUM>& acc) {
+#pragma clang fp contract(fast)
+    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
+    const T xt0 = u.R[0] * x0 + u.R[1] * x1 + u.R[2] * x2 + u.t[0];
+    const T xt1 = u.R[3] * x0 + u.R[4] * x1 + u.R[5] * x2 + u.t[1];
+    const T xt2 = u.R[6] * x0 + u.R[7] * x1 + u.R[8] * x2 + u.t[2];
+    if (!(fabs(xt0) < (T)1e15) || !(fabs(xt1) < (T)1e15) || !(fabs(xt2) < (T)1e15)) return 0;
+    int64_t c0, c1, c2;
+    if constexpr (sizeof(T) == 4) {
+        c0 = (int64_t)floorf(xt0 * g.inv_resolution);
+        c1 = (int64_t)floorf(xt1 * g.inv_resolution);
+        c2 = (int64_t)floorf(xt2 * g.inv_resolution);
+    } else {
+        c0 = (int64_t)floor(xt0 / g.resolution_d);
+        c1 = (int64_t)floor(xt1 / g.resolution_d);
+        c2 = (int64_t)floor(xt2 / g.resolution_d);
+    }
+    c0 -= g.origin_cell[0];
+    c1 -= g.origin_cell[1];
+    c2 -= g.origin_cell[2];
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= (1ll << g.bits[0]) || c1 >= (1ll << g.bits[1]) || c2 >= (1ll << g.bits[2]))
+        return 0;
+    const uint64_t key = ((uint64_t)c2 << (g.bits[0] + g.bits[1])) | ((uint64_t)c1 << g.bits[0]) | (uint64_t)c0;
+    uint32_t begin, count;
+    if (!hash_find(g.slots, g.log2cap, key, begin, count)) return 0;
+    /* point Jacobian columns J3 = e_x x ya, J4 = a4 x ya, J5 = a5 x ya */
+    const T y0 = u.Ra[0] * x0 + u.Ra[1] * x1 + u.Ra[2] * x2;
+    const T y1 = u.Ra[3] * x0 + u.Ra[4] * x1 + u.Ra[5] * x2;
+    const T y2 = u.Ra[6] * x0 + u.Ra[7] * x1 + u.Ra[8] * x2;
+    const T J3y = -y2, J3z = y1;
+    const T J4x = u.a4[1] * y2 - u.a4[2] * y1, J4y = u.a4[2] * y0 - u.a4[0] * y2, J4z = u.a4[0] * y1 - u.a4[1] * y0;
+    const T J5x = u.a5[1] * y2 - u.a5[2] * y1, J5y = u.a5[2] * y0 - u.a5[0] * y2, J5z = u.a5[0] * y1 - u.a5[1] * y0;
+    int pairs = 0;
+    for (uint32_t ci = begin; ci < begin + count; ci++) {
+        const float4 cd = g.cand[ci];
+        const uint32_t v = __float_as_uint(cd.w);
+        T d0, d1, d2;
+        if constexpr (sizeof(T) == 4) {
+            d0 = xt0 - cd.x;
+            d1 = xt1 - cd.y;
+            d2 = xt2 - cd.z;
+            if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
+        } else {
+            d0 = xt0 - g.mean_d[3 * v];
+            d1 = xt1 - g.mean_d[3 * v + 1];
+            d2 = xt2 - g.mean_d[3 * v + 2];
+            if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) continue;
+        }
+        T c00, c01, c02, c11, c12, c22;
+        if constexpr (sizeof(T) == 4) {
+            const float4 a = g.icov_f[2 * v], b = g.icov_f[2 * v + 1];
+            c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
+        } else {
+            const double* C = g.icov_d + 6 * v;
+            c00 = C[0]; c01 = C[1]; c02 = C[2]; c11 = C[3]; c12 = C[4]; c22 = C[5];
+        }
+        /* q = C x' */
+        const T q0 = c00 * d0 + c01 * d1 + c02 * d2;
+        const T q1 = c01 * d0 + c11 * d1 + c12 * d2;
+        const T q2 = c02 * d0 + c12 * d1 + c22 * d2;
+        const T xCx = d0 * q0 + d1 * q1 + d2 * q2;
+        T e;
+        if constexpr (sizeof(T) == 4) e = expf(u.nhd2 * xCx);
+        else e = exp(u.nhd2 * xCx);
+        const T e2 = u.d2 * e;
+        /* invalid-value guard, kernel.cpp:456-457 */
+        if (e2 > (T)1 || e2 < (T)0 || e2 != e2) continue;
+        pairs++;
+        const T f = e2 * u.d1;
+        const T m = -u.d2;
+        /* x'^T C J_k */
+        const T u3 = q1 * J3y + q2 * J3z;
+        const T u4 = q0 * J4x + q1 * J4y + q2 * J4z;
+        const T u5 = q0 * J5x + q1 * J5y + q2 * J5z;
+        /* C J_k */
+        const T k30 = c01 * J3y + c02 * J3z, k31 = c11 * J3y + c12 * J3z, k32 = c12 * J3y + c22 * J3z;
+        const T k40 = c00 * J4x + c01 * J4y + c02 * J4z;
+        const T k41 = c01 * J4x + c11 * J4y + c12 * J4z;
+        const T k42 = c02 * J4x + c12 * J4y + c22 * J4z;
+        const T k50 = c00 * J5x + c01 * J5y + c02 * J5z;
+        const T k51 = c01 * J5x + c11 * J5y + c12 * J5z;
+        const T k52 = c02 * J5x + c12 * J5y + c22 * J5z;
+        /* J_b^T C J_a */
+        const T j33 = J3y * k31 + J3z * k32;
+        const T j34 = J4x * k30 + J4y * k31 + J4z * k32;
+        const T j35 = J5x * k30 + J5y * k31 + J5z * k32;
+        const T j44 = J4x * k40 + J4y * k41 + J4z * k42;
+        const T j45 = J5x * k40 + J5y * k41 + J5z * k42;
+        const T j55 = J5x * k50 + J5y * k51 + J5z * k52;
+        /* x'^T C H_ab = J_b . (q x a_a) */
+        const T h33 = q2 * J3y - q1 * J3z;
+        const T h34 = q2 * J4y - q1 * J4z;
+        const T h35 = q2 * J5y - q1 * J5z;
+        const T qa40 = q1 * u.a4[2] - q2 * u.a4[1], qa41 = q2 * u.a4[0] - q0 * u.a4[2], qa42 = q0 * u.a4[1] - q1 * u.a4[0];
+        const T qa50 = q1 * u.a5[2] - q2 * u.a5[1], qa51 = q2 * u.a5[0] - q0 * u.a5[2], qa52 = q0 * u.a5[1] - q1 * u.a5[0];
+        const T h44 = qa40 * J4x + qa41 * J4y + qa42 * J4z;
+        const T h45 = qa40 * J5x + qa41 * J5y + qa42 * J5z;
+        const T h55 = qa50 * J5x + qa51 * J5y + qa52 * J5z;
+        acc.add(0, -u.d1 * e);
+        acc.add(1, f * q0);
+        acc.add(2, f * q1);
+        acc.add(3, f * q2);
+        acc.add(4, f * u3);
+        acc.add(5, f * u4);
+        acc.add(6, f * u5);
+        const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
+        acc.add(7, f * (mu0 * q0 + c00));
+        acc.add(8, f * (mu0 * q1 + c01));
+        acc.add(9, f * (mu0 * q2 + c02));
+        acc.add(10, f * (mu0 * u3 + k30));
+        acc.add(11, f * (mu0 * u4 + k40));
+        acc.add(12, f * (mu0 * u5 + k50));
+        acc.add(13, f * (mu1 * q1 + c11));
+        acc.add(14, f * (mu1 * q2 + c12));
+        acc.add(15, f * (mu1 * u3 + k31));
+        acc.add(16, f * (mu1 * u4 + k41));
+        acc.add(17, f * (mu1 * u5 + k51));
+        acc.add(18, f * (mu2 * q2 + c22));
+        acc.add(19, f * (mu2 * u3 + k32));
+        acc.add(20, f * (mu2 * u4 + k42));
+        acc.add(21, f * (mu2 * u5 + k52));
+        acc.add(22, f * (mu3 * u3 + h33 + j33));
+        acc.add(23, f * (mu3 * u4 + h34 + j34));
+        acc.add(24, f * (mu3 * u5 + h35 + j35));
+        acc.add(25, f * (mu4 * u4 + h44 + j44));
+        acc.add(26, f * (mu4 * u5 + h45 + j45));
+        acc.add(27, f * (m * u5 * u5 + h55 + j55));
+    }
+    return pairs;
+}
+
+/* Reduces one value per thread per accumulator over the workgroup in a fixed
+ * order; thread k < NACC ends with sum k in out[k] (LDS). */
+template <int BLOCK>
+__device__ inline void block_reduce_acc(double* vals /* NRED per thread, registers */, double (*lds)[NRED],
+                                        double* out) {
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+#pragma unroll
+    for (int k = 0; k < NRED; k++) {
+        double v = wave_sum(vals[k]);
+        if (lane == 0) lds[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED) {
+        double s = 0.0;
+        for (int w = 0; w < BLOCK / WAVE; w++) s += lds[w][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+/* ------------------------------------------------------------------ the Newton / line-search controller
+ * Runs on one thread of the workgroup that completes a scan's pass.  Consumes the
+ * 28 sums of the pass just evaluated at st.x_t and either chooses the next pose
+ * to evaluate or finishes the scan.  Same semantics as computeTransformation /
+ * computeStepLengthMT (src/CPU/ndt_mapping/kernel.cpp:1015-1144, :1175-1272) in
+ * their textbook form (line search live, translation updated, true convergence). */
+
+__device__ __attribute__((noinline)) void ctl_finish(ScanState& st, const AlignParams& prm, int converged, ScanOut* out) {
+    st.converged = converged;
+    st.done = 1;
+    st.phase = PH_DONE;
+    double M[16];
+    pose_matrix(st.p, M);
+    M[3] += prm.origin[0];
+    M[7] += prm.origin[1];
+    M[11] += prm.origin[2];
+    for (int k = 0; k < 16; k++) {
+        st.final_m[k] = M[k];
+        out->final_m[k] = M[k];
+    }
+    out->score = st.score;
+    out->iterations = st.iter;
+    out->converged = converged;
+    out->passes = st.passes;
+    out->status = st.status;
+    out->pairs = st.pairs;
+}
+
+__device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const AlignParams& prm, ScanOut* out) {
+    const double mu = 1.e-4;
+    for (;;) {
+        if (st.iter >= prm.max_iterations) {
+            ctl_finish(st, prm, 0, out);
+            return;
+        }
+        double H[36], ng[6], delta[6];
+        for (int i = 0; i < 6; i++) {
+            ng[i] = -st.grad[i];
+            for (int j = 0; j < 6; j++) H[i * 6 + j] = st.hess[i <= j ? tri_index(i, j) : tri_index(j, i)];
+        }
+        solve6(delta, H, ng);
+        double norm = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2] + delta[3] * delta[3] +
+                           delta[4] * delta[4] + delta[5] * delta[5]);
+        if (norm == 0 || norm != norm) {
+            ctl_finish(st, prm, norm == norm, out);
+            return;
+        }
+        for (int i = 0; i < 6; i++) st.dir[i] = delta[i] / norm;
+        st.phi_0 = -st.score;
+        st.d_phi_0 = -dot6(st.grad, st.dir);
+        if (st.d_phi_0 >= 0) {
+            if (st.d_phi_0 == 0) {
+                /* no descent direction: zero step */
+                st.iter++;
+                if (st.iter > 1 && 0.0 < prm.trans_eps) {
+                    ctl_finish(st, prm, 1, out);
+                    return;
+                }
+                continue;
+            }
+            st.d_phi_0 *= -1;
+            for (int i = 0; i < 6; i++) st.dir[i] *= -1;
+        }
+        st.a_l = 0;
+        st.a_u = 0;
+        st.f_l = psi_mt(st.a_l, st.phi_0, st.phi_0, st.d_phi_0, mu);
+        st.g_l = dpsi_mt(st.d_phi_0, st.d_phi_0, mu);
+        st.f_u = psi_mt(st.a_u, st.phi_0, st.phi_0, st.d_phi_0, mu);
+        st.g_u = dpsi_mt(st.d_phi_0, st.d_phi_0, mu);
+        st.interval_converged = 0;
+        st.open_interval = 1;
+        st.step_iterations = 0;
+        double a_t = norm;
+        a_t = dmin_std(a_t, prm.step_size);
+        a_t = dmax_std(a_t, prm.trans_eps / 2);
+        st.a_t = a_t;
+        for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
+        st.phase = PH_MT_FIRST;
+        return;
+    }
+}
+
+__device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* red, const AlignParams& prm, ScanOut* out) {
+    const double mu = 1.e-4, nu = 0.9;
+    st.passes++;
+    st.pairs += red[NACC];
+    if (st.phase == PH_INIT) {
+        st.score = red[0];
+        for (int i = 0; i < 6; i++) st.grad[i] = red[1 + i];
+        for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];
+        for (int i = 0; i < 6; i++) st.p[i] = st.x_t[i];
+        ctl_newton_start(st, prm, out);
+        return;
+    }
+    /* a line-search evaluation at x_t = p + a_t dir */
+    double g_t[6];
+    for (int i = 0; i < 6; i++) g_t[i] = red[1 + i];
+    double phi_t = -red[0];
+    double d_phi_t = -dot6(g_t, st.dir);
+    double psi_t = psi_mt(st.a_t, phi_t, st.phi_0, st.d_phi_0, mu);
+    double d_psi_t = dpsi_mt(d_phi_t, st.d_phi_0, mu);
+    if (st.phase == PH_MT_LOOP) {
+        if (st.open_interval && (psi_t <= 0 && d_psi_t >= 0)) {
+            st.open_interval = 0;
+            st.f_l = st.f_l + st.phi_0 - mu * st.d_phi_0 * st.a_l;
+            st.g_l = st.g_l + mu * st.d_phi_0;
+            st.f_u = st.f_u + st.phi_0 - mu * st.d_phi_0 * st.a_u;
+            st.g_u = st.g_u + mu * st.d_phi_0;
+        }
+        if (st.open_interval)
+            st.interval_converged = mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, psi_t, d_psi_t);
+        else
+            st.interval_converged = mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, phi_t, d_phi_t);
+        st.step_iterations++;
+    }
+    if (!st.interval_converged && st.step_iterations < 10 && !(psi_t <= 0 && d_phi_t <= -nu * st.d_phi_0)) {
+        double a_t;
+        if (st.open_interval)
+            a_t = mt_trial_value(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, psi_t, d_psi_t);
+        else
+            a_t = mt_trial_value(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, phi_t, d_phi_t);
+        a_t = dmin_std(a_t, prm.step_size);
+        a_t = dmax_std(a_t, prm.trans_eps / 2);
+        st.a_t = a_t;
+        for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
+        st.phase = PH_MT_LOOP;
+        return;
+    }
+    /* accept x_t: its pass supplies score, gradient and Hessian */
+    for (int i = 0; i < 6; i++) st.p[i] = st.p[i] + st.dir[i] * st.a_t;
+    st.score = red[0];
+    for (int i = 0; i < 6; i++) st.grad[i] = g_t[i];
+    for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];
+    st.iter++;
+    if (st.iter > 1 && fabs(st.a_t) < prm.trans_eps) {
+        ctl_finish(st, prm, 1, out);
+        return;
+    }
+    ctl_newton_start(st, prm, out);
+}
+
+__global__ void k_init_states(const double* __restrict__ guesses, int32_t S, AlignParams prm, ScanState* states,
+                              GroupSync* sync) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    ScanState st;
+    uint64_t* w = (uint64_t*)&st;
+    for (int k = 0; k < STATE_WORDS; k++) w[k] = 0;
+    const double* G = guesses + 16 * s;
+    double M[16];
+    for (int k = 0; k < 16; k++) M[k] = G[k];
+    M[3] -= prm.origin[0];
+    M[7] -= prm.origin[1];
+    M[11] -= prm.origin[2];
+    double ea[3];
+    pose_euler(M, ea);
+    st.x_t[0] = M[3];
+    st.x_t[1] = M[7];
+    st.x_t[2] = M[11];
+    st.x_t[3] = ea[0];
+    st.x_t[4] = ea[1];
+    st.x_t[5] = ea[2];
+    for (int k = 0; k < 6; k++) st.p[k] = st.x_t[k];
+    st.phase = PH_INIT;
+    states[s] = st;
+    sync[s].arrive = 0;
+    sync[s].gen = 0;
+    sync[s].abort = 0;
+}
+
+int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, GroupSync* sync,
+                       hipStream_t stream) {
+    hipLaunchKernelGGL(k_init_states, dim3(grid_for(S, 64)), dim3(64), 0, stream, guesses, S, prm, states, sync);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ persistent align kernel
+ * One cooperative launch aligns every scan of a batch.  Scan s is processed by a
+ * group of `group` consecutive workgroups that split its points evenly.  After
+ * each pass every workgroup publishes its 28 partial sums; the workgroup that
+ * arrives last reduces them in workgroup order (deterministic), runs the
+ * controller on the scan's state and releases the others.  All waits are bounded. */
+
+constexpr uint32_t SPIN_LIMIT = 1u << 24;
+#ifndef ALIGN_MIN_WAVES
+#define ALIGN_MIN_WAVES 4
+#endif
+
+template <typename T, int SUM, int BLOCK>
+__global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, const float4* __restrict__ pts,
+                                                 const ScanDesc* __restrict__ scans, const int32_t* __restrict__ wg_scan,
+                                                 AlignParams prm, ScanState* states, GroupSync* sync, double* partials,
+                                                 ScanOut* out) {
+    __shared__ PassParams<T> pp;
+    __shared__ double lds_red[BLOCK / WAVE][NRED];
+    __shared__ double red[NRED];
+    __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ int s_done, s_last, s_abort;
+    const int tid = threadIdx.x;
+    const int wg = blockIdx.x;
+    const int s = wg_scan[wg];
+    const ScanDesc sd = scans[s];
+    const int local = wg - sd.first_wg;
+    const int G = sd.group;
+    const int64_t pb = sd.offset + ((int64_t)sd.n * local) / G;
+    const int64_t pe = sd.offset + ((int64_t)sd.n * (local + 1)) / G;
+    ScanState* st = states + s;
+    GroupSync* gs = sync + s;
+    uint32_t my_gen = 0;
+    for (;;) {
+        if (tid == 0) {
+            int done = ld_agent_i32(&st->done);
+            s_done = done;
+            if (!done) {
+                double x[6];
+                for (int k = 0; k < 6; k++) x[k] = ld_agent_f64(&st->x_t[k]);
+                make_pass_params<T>(x, prm, g.resolution_d, pp);
+            }
+        }
+        __syncthreads();
+        if (s_done) break;
+        Acc<T, SUM> acc;
+        acc.zero();
+        int pairs = 0;
+        {
+            const PassParams<T> u = load_uniform(pp);
+            for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+        }
+        double v[NRED];
+#pragma unroll
+        for (int k = 0; k < NACC; k++) v[k] = acc.get(k);
+        v[NACC] = (double)pairs;
+        block_reduce_acc<BLOCK>(v, lds_red, red);
+        if (G > 1) {
+            if (tid < NRED) st_agent_f64(partials + (int64_t)wg * 32 + tid, red[tid]);
+            __syncthreads();
+            if (tid == 0) {
+                __threadfence();
+                uint32_t old = __hip_atomic_fetch_add(&gs->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                s_last = (old == (uint32_t)(G - 1));
+            }
+            __syncthreads();
+        } else {
+            if (tid == 0) s_last = 1;
+            __syncthreads();
+        }
+        if (s_last) {
+            if (tid < STATE_WORDS) sstate[tid] = ld_agent_u64((const uint64_t*)st + tid);
+            for (int k = tid + BLOCK; k < STATE_WORDS; k += BLOCK) sstate[k] = ld_agent_u64((const uint64_t*)st + k);
+            if (G > 1 && tid < NRED) {
+                double acc2 = 0.0;
+                for (int w = 0; w < G; w++) acc2 += ld_agent_f64(partials + (int64_t)(sd.first_wg + w) * 32 + tid);
+                red[tid] = acc2;
+            }
+            __syncthreads();
+            if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
+            __syncthreads();
+            for (int k = tid; k < STATE_WORDS; k += BLOCK) st_agent_u64((uint64_t*)st + k, sstate[k]);
+            __syncthreads();
+            if (tid == 0 && G > 1) {
+                __hip_atomic_store(&gs->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            if (tid == 0) {
+                uint32_t spins = 0;
+                int ab = 0;
+                while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
+                    __builtin_amdgcn_s_sleep(2);
+                    if (++spins > SPIN_LIMIT ||
+                        __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        ab = 1;
+                        break;
+                    }
+                }
+                s_abort = ab;
+                if (ab) {
+                    __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    out[s].status = NDT_ERR_TIMEOUT;
+                }
+            }
+            __syncthreads();
+            if (s_abort) break;
+        }
+        my_gen++;
+        __syncthreads();
+    }
+}
+
+template <typename T, int SUM, int BLOCK>
+static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, const int32_t* wg_scan,
+                          int32_t total_wg, AlignParams prm, ScanState* states, GroupSync* sync, double* partials,
+                          ScanOut* out, hipStream_t stream) {
+    void* args[] = {&g, &pts, &scans, &wg_scan, &prm, &states, &sync, &partials, &out};
+    return (int)hipLaunchCooperativeKernel((const void*)k_align<T, SUM, BLOCK>, dim3(total_wg), dim3(BLOCK), args, 0,
+                                           stream);
+}
+
+template <typename T, int SUM, int BLOCK>
+static int max_blocks_t(int* bpc) {
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align<T, SUM, BLOCK>, BLOCK, 0);
+}
+
+#define NDT_DISPATCH(BLOCKSZ, FP64, SUMMODE, CALL)                                         \
+    do {                                                                                  \
+        if (FP64) {                                                                       \
+            if (BLOCKSZ == 128) return CALL(double, NDT_SUM_FP64, 128);                    \
+            if (BLOCKSZ == 256) return CALL(double, NDT_SUM_FP64, 256);                    \
+            if (BLOCKSZ == 512) return CALL(double, NDT_SUM_FP64, 512);                    \
+        } else if (SUMMODE == NDT_SUM_FP32) {                                             \
+            if (BLOCKSZ == 128) return CALL(float, NDT_SUM_FP32, 128);                     \
+            if (BLOCKSZ == 256) return CALL(float, NDT_SUM_FP32, 256);                     \
+            if (BLOCKSZ == 512) return CALL(float, NDT_SUM_FP32, 512);                     \
+        } else if (SUMMODE == NDT_SUM_COMP32) {                                           \
+            if (BLOCKSZ == 128) return CALL(float, NDT_SUM_COMP32, 128);                   \
+            if (BLOCKSZ == 256) return CALL(float, NDT_SUM_COMP32, 256);                   \
+            if (BLOCKSZ == 512) return CALL(float, NDT_SUM_COMP32, 512);                   \
+        } else {                                                                          \
+            if (BLOCKSZ == 128) return CALL(float, NDT_SUM_FP64, 128);                     \
+            if (BLOCKSZ == 256) return CALL(float, NDT_SUM_FP64, 256);                     \
+            if (BLOCKSZ == 512) return CALL(float, NDT_SUM_FP64, 512);                     \
+        }                                                                                 \
+        return (int)hipErrorInvalidValue;                                                 \
+    } while (0)
+
+int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu) {
+#define NDT_MB(T, S, B) max_blocks_t<T, S, B>(blocks_per_cu)
+    NDT_DISPATCH(block_size, point_fp64, sum_mode, NDT_MB);
+#undef NDT_MB
+}
+
+int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, const int32_t* wg_scan, int32_t total_wg,
+                 int block_size, AlignParams prm, ScanState* states, GroupSync* sync, double* partials, ScanOut* out,
+                 hipStream_t stream) {
+#define NDT_LA(T, S, B) \
+    launch_align_t<T, S, B>(grid, pts, scans, wg_scan, total_wg, prm, states, sync, partials, out, stream)
+    NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
+#undef NDT_LA
+}
+
+/* ------------------------------------------------------------------ single derivative pass (ndt) */
+
+template <typename T, int SUM>
+__global__ __launch_bounds__(256, ALIGN_MIN_WAVES) void k_deriv_ndt(NdtGridDev g, const float4* __restrict__ pts, int64_t n,
+                                                   const double* __restrict__ pose, AlignParams prm,
+                                                   double* __restrict__ partials) {
+    __shared__ PassParams<T> pp;
+    __shared__ double lds_red[256 / WAVE][NRED];
+    __shared__ double red[NRED];
+    if (threadIdx.x == 0) {
+        double x[6];
+        for (int k = 0; k < 6; k++) x[k] = pose[k];
+        make_pass_params<T>(x, prm, g.resolution_d, pp);
+    }
+    __syncthreads();
+    Acc<T, SUM> acc;
+    acc.zero();
+    int pairs = 0;
+    const PassParams<T> u = load_uniform(pp);
+    for (int64_t i = blockIdx.x * 256ll + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+    double v[NRED];
+#pragma unroll
+    for (int k = 0; k < NACC; k++) v[k] = acc.get(k);
+    v[NACC] = (double)pairs;
+    block_reduce_acc<256>(v, lds_red, red);
+    if (threadIdx.x < NRED) partials[(int64_t)blockIdx.x * NRED + threadIdx.x] = red[threadIdx.x];
+}
+
+int launch_deriv_ndt(NdtGridDev grid, const float4* pts, int64_t n, const double* pose_params, AlignParams prm,
+                     double* partials, int nblocks, hipStream_t stream) {
+    if (prm.point_fp64)
+        hipLaunchKernelGGL((k_deriv_ndt<double, NDT_SUM_FP64>), dim3(nblocks), dim3(256), 0, stream, grid, pts, n,
+                           pose_params, prm, partials);
+    else if (prm.sum_mode == NDT_SUM_FP32)
+        hipLaunchKernelGGL((k_deriv_ndt<float, NDT_SUM_FP32>), dim3(nblocks), dim3(256), 0, stream, grid, pts, n,
+                           pose_params, prm, partials);
+    else if (prm.sum_mode == NDT_SUM_COMP32)
+        hipLaunchKernelGGL((k_deriv_ndt<float, NDT_SUM_COMP32>), dim3(nblocks), dim3(256), 0, stream, grid, pts, n,
+                           pose_params, prm, partials);
+    else
+        hipLaunchKernelGGL((k_deriv_ndt<float, NDT_SUM_FP64>), dim3(nblocks), dim3(256), 0, stream, grid, pts, n,
+                           pose_params, prm, partials);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partials, int nblocks, int width, double* __restrict__ out) {
+    int k = threadIdx.x;
+    if (k >= width) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partials[(int64_t)b * width + k];
+    out[k] = s;
+}
+
+int launch_reduce_partials(const double* partials, int nblocks, int width, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, stream, partials, nblocks, width, out);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ compat derivative pass
+ * voxelRadiusSearch + computePointDerivatives + updateDerivatives
+ * (src/CPU/ndt_mapping/kernel.cpp:246-286, :438-560) per scan point, in float64
+ * with the reference's operation order and its Hessian term. */
+
+struct CompatPassArgs {
+    float M[16];
+    double jh[23][3];
+    double d1, d2;
+};
+
+__device__ inline double cdot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ inline bool compat_lookup(const CompatGridDev& g, uint64_t key, uint32_t& vox) {
+    uint32_t count;
+    return hash_find(g.slots, g.log2cap, key, vox, count);
+}
+
+__global__ __launch_bounds__(256) void k_deriv_compat(CompatGridDev g, const float4* __restrict__ scan, int64_t n,
+                                                      CompatPassArgs a, double* __restrict__ partials) {
+    __shared__ double lds[256 / WAVE][NRED_COMPAT];
+    double acc[NRED_COMPAT];
+    for (int k = 0; k < NRED_COMPAT; k++) acc[k] = 0.0;
+    const double radius = (double)g.resolution;
+    /* which angular second-derivative vector sits in block (i, j), i, j >= 3
+     * (point_hessian_ layout, kernel.cpp:528-558): a b c / b d e / c e f */
+    const int hsel[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+        const float4 p4 = scan[idx];
+        const float in[3] = {p4.x, p4.y, p4.z};
+        float pt[3];
+        for (int row = 0; row < 3; row++)
+            pt[row] = a.M[row * 4 + 0] * in[0] + a.M[row * 4 + 1] * in[1] + a.M[row * 4 + 2] * in[2] + a.M[row * 4 + 3];
+        const double x[3] = {in[0], in[1], in[2]};
+        double pg[3][6];
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 6; c++) pg[r][c] = 0.0;
+        pg[0][0] = 1.0;
+        pg[1][1] = 1.0;
+        pg[2][2] = 1.0;
+        pg[1][3] = cdot(x, a.jh[0]);
+        pg[2][3] = cdot(x, a.jh[1]);
+        pg[0][4] = cdot(x, a.jh[2]);
+        pg[1][4] = cdot(x, a.jh[3]);
+        pg[2][4] = cdot(x, a.jh[4]);
+        pg[0][5] = cdot(x, a.jh[5]);
+        pg[1][5] = cdot(x, a.jh[6]);
+        pg[2][5] = cdot(x, a.jh[7]);
+        double hv[6][3];
+        hv[0][0] = 0; hv[0][1] = cdot(x, a.jh[8]); hv[0][2] = cdot(x, a.jh[9]);
+        hv[1][0] = 0; hv[1][1] = cdot(x, a.jh[10]); hv[1][2] = cdot(x, a.jh[11]);
+        hv[2][0] = 0; hv[2][1] = cdot(x, a.jh[12]); hv[2][2] = cdot(x, a.jh[13]);
+        hv[3][0] = cdot(x, a.jh[14]); hv[3][1] = cdot(x, a.jh[15]); hv[3][2] = cdot(x, a.jh[16]);
+        hv[4][0] = cdot(x, a.jh[17]); hv[4][1] = cdot(x, a.jh[18]); hv[4][2] = cdot(x, a.jh[19]);
+        hv[5][0] = cdot(x, a.jh[20]); hv[5][1] = cdot(x, a.jh[21]); hv[5][2] = cdot(x, a.jh[22]);
+        const float radiusFinal = (float)(radius + 0.001f);
+        /* the probe loops step in float like the reference; the step counters only
+         * bound them for coordinates so large that adding the resolution no longer
+         * changes the float (where the reference would not terminate) */
+        int kz = 0;
+        for (float z = (float)(pt[2] - radius); z <= pt[2] + radiusFinal && kz < 4; z += g.resolution, kz++) {
+            int ky = 0;
+            for (float y = (float)(pt[1] - radius); y <= pt[1] + radiusFinal && ky < 4; y += g.resolution, ky++) {
+                int kx = 0;
+                for (float xx = (float)(pt[0] - radius); xx <= pt[0] + radiusFinal && kx < 4; xx += g.resolution, kx++) {
+                    if ((xx < g.minv[0]) || (xx > g.maxv[0]) || (y < g.minv[1]) || (y > g.maxv[1]) || (z < g.minv[2]) ||
+                        (z > g.maxv[2]))
+                        continue;
+                    int lin = compat_linearize(g.minv, g.dims, g.resolution, xx, y, z);
+                    uint32_t v;
+                    if (!compat_lookup(g, (uint64_t)(uint32_t)lin, v)) continue; /* empty cell: NaN mean */
+                    const double* c = g.mean + 3 * v;
+                    float dx = (float)(c[0] - pt[0]);
+                    float dy = (float)(c[1] - pt[1]);
+                    float dz = (float)(c[2] - pt[2]);
+                    float dist = (float)sqrt((double)(dx * dx + dy * dy + dz * dz));
+                    if (!(dist < radius)) continue;
+                    acc[NACC_COMPAT] += 1.0;
+                    double xt[3] = {pt[0], pt[1], pt[2]};
+                    xt[0] -= c[0];
+                    xt[1] -= c[1];
+                    xt[2] -= c[2];
+                    const double* ci = g.icov + 9 * v;
+                    double xCx = ci[0] * xt[0] * xt[0] + ci[4] * xt[1] * xt[1] + ci[8] * xt[2] * xt[2] +
+                                 (ci[1] + ci[3]) * xt[0] * xt[1] + (ci[2] + ci[6]) * xt[0] * xt[2] +
+                                 (ci[5] + ci[7]) * xt[1] * xt[2];
+                    double e = exp(-a.d2 * (xCx) / 2);
+                    double score_inc = -a.d1 * e;
+                    e = a.d2 * e;
+                    if (e > 1 || e < 0 || e != e) continue;
+                    acc[0] += score_inc;
+                    e *= a.d1;
+                    for (int i = 0; i < 6; i++) {
+                        double cov[3];
+                        for (int r = 0; r < 3; r++) {
+                            cov[r] = 0;
+                            for (int cc = 0; cc < 3; cc++) cov[r] += ci[r * 3 + cc] * pg[cc][i];
+                        }
+                        double xc = cdot(xt, cov);
+                        acc[1 + i] += xc * e;
+                        for (int j = 0; j < 6; j++) {
+                            double colVec[3] = {pg[0][j], pg[1][j], pg[2][j]};
+                            double hb[3] = {0.0, 0.0, 0.0};
+                            if (i >= 3 && j >= 3) {
+                                const int hs = hsel[i - 3][j - 3];
+                                hb[0] = hv[hs][0];
+                                hb[1] = hv[hs][1];
+                                hb[2] = hv[hs][2];
+                            }
+                            double cvh[3] = {colVec[0] + hb[0], colVec[1] + hb[1], colVec[2] + hb[2]};
+                            double mp[3];
+                            for (int r = 0; r < 3; r++) {
+                                mp[r] = 0;
+                                for (int cc = 0; cc < 3; cc++) mp[r] += ci[r * 3 + cc] * cvh[cc];
+                            }
+                            acc[7 + i * 6 + j] += e * (-a.d2 * xc * cdot(xt, mp) + cdot(colVec, cov));
+                        }
+                    }
+                }
+            }
+        }
+    }
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    for (int k = 0; k < NRED_COMPAT; k++) {
+        double v = wave_sum(acc[k]);
+        if (lane == 0) lds[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED_COMPAT) {
+        double s = 0.0;
+        for (int w = 0; w < 256 / WAVE; w++) s += lds[w][threadIdx.x];
+        partials[(int64_t)blockIdx.x * NRED_COMPAT + threadIdx.x] = s;
+    }
+}
+
+int launch_deriv_compat(CompatGridDev grid, const float4* scan, int64_t n, const float* M16, const double* jh,
+                        double d1, double d2, double* partials, int nblocks, hipStream_t stream) {
+    CompatPassArgs a;
+    for (int k = 0; k < 16; k++) a.M[k] = M16[k];
+    for (int k = 0; k < 23; k++)
+        for (int c = 0; c < 3; c++) a.jh[k][c] = jh[k * 3 + c];
+    a.d1 = d1;
+    a.d2 = d2;
+    hipLaunchKernelGGL(k_deriv_compat, dim3(nblocks), dim3(256), 0, stream, grid, scan, n, a, partials);
+    return (int)hipGetLastError();
+}
+
+}  // namespace ndt

@@ -1,0 +1,557 @@
+"""Host-side Python interface of the MI355X NDT scan-matching engine.
+
+Mirrors the reference's testcase interface for the ndt_mapping kernel
+(src/CPU/ndt_mapping/kernel.cpp): the DAPHNE data files (read_next_testcases
+:143-217, parseResult :178-190), ``partial_points_callback`` (:1405-1414) and the
+output check (check_next_outputs :1436-1496).  Compute goes through the C ABI of
+``libndthip.so`` (include/ndt_hip.h); there is no Python or CPU fallback: a
+missing library raises.
+
+Synthetic inputs come from ``libndt_scene.so`` (include/ndt_scene.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import gzip
+import os
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+
+# ---------------------------------------------------------------------------
+# synthetic scenes (include/ndt_scene.h)
+# ---------------------------------------------------------------------------
+
+
+class SceneCfg(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("extent_x", C.c_double),
+        ("extent_y", C.c_double),
+        ("n_boxes", C.c_int32),
+        ("n_cylinders", C.c_int32),
+        ("box_side_min", C.c_double),
+        ("box_side_max", C.c_double),
+        ("box_h_min", C.c_double),
+        ("box_h_max", C.c_double),
+        ("cyl_radius", C.c_double),
+        ("cyl_height", C.c_double),
+        ("ground_sigma", C.c_double),
+        ("origin", C.c_double * 3),
+    ]
+
+
+class ScanCfg(C.Structure):
+    _fields_ = [
+        ("n_points", C.c_int64),
+        ("max_range", C.c_double),
+        ("noise_sigma", C.c_double),
+        ("sensor_height", C.c_double),
+        ("sensor_margin", C.c_double),
+        ("perturb_xyz", C.c_double * 3),
+        ("perturb_yaw_deg", C.c_double),
+        ("perturb_rollpitch_deg", C.c_double),
+    ]
+
+
+_scene_lib = None
+
+
+def scene_lib():
+    global _scene_lib
+    if _scene_lib is None:
+        path = os.path.join(PKG_DIR, "libndt_scene.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+        lib = C.CDLL(path)
+        lib.ndt_scene_cfg_default.argtypes = [C.POINTER(SceneCfg), C.c_uint64]
+        lib.ndt_scan_cfg_default.argtypes = [C.POINTER(ScanCfg)]
+        lib.ndt_scene_map.argtypes = [C.POINTER(SceneCfg), C.c_int64, C.c_void_p]
+        lib.ndt_scene_map.restype = C.c_int
+        lib.ndt_scene_scan.argtypes = [C.POINTER(SceneCfg), C.c_void_p, C.c_int64, C.POINTER(ScanCfg),
+                                       C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.ndt_scene_scan.restype = C.c_int
+        lib.ndt_scene_pose.argtypes = [C.c_double] * 6 + [C.c_void_p]
+        _scene_lib = lib
+    return _scene_lib
+
+
+def scene_cfg(seed: int, extent: float = 100.0, n_boxes: int | None = None, n_cylinders: int | None = None,
+              origin=(0.0, 0.0, 0.0)) -> SceneCfg:
+    """Scene of ``extent`` x ``extent`` metres.  Object counts default to the
+    100 m x 100 m densities (40 boxes, 60 cylinders) scaled by area."""
+    cfg = SceneCfg()
+    scene_lib().ndt_scene_cfg_default(C.byref(cfg), seed)
+    scale = (extent * extent) / (100.0 * 100.0)
+    cfg.extent_x = cfg.extent_y = extent
+    cfg.n_boxes = int(round(40 * scale)) if n_boxes is None else n_boxes
+    cfg.n_cylinders = int(round(60 * scale)) if n_cylinders is None else n_cylinders
+    for i in range(3):
+        cfg.origin[i] = origin[i]
+    return cfg
+
+
+def scan_cfg(n_points: int = 2000, perturb_xyz: float = 0.5, perturb_yaw_deg: float = 2.0,
+             perturb_rollpitch_deg: float = 0.0, max_range: float = 40.0, noise: float = 0.01) -> ScanCfg:
+    cfg = ScanCfg()
+    scene_lib().ndt_scan_cfg_default(C.byref(cfg))
+    cfg.n_points = n_points
+    cfg.max_range = max_range
+    cfg.noise_sigma = noise
+    for i in range(3):
+        cfg.perturb_xyz[i] = perturb_xyz
+    cfg.perturb_yaw_deg = perturb_yaw_deg
+    cfg.perturb_rollpitch_deg = perturb_rollpitch_deg
+    return cfg
+
+
+def make_map(cfg: SceneCfg, m: int) -> np.ndarray:
+    out = np.empty((m, 4), dtype=np.float32)
+    rc = scene_lib().ndt_scene_map(C.byref(cfg), m, out.ctypes.data)
+    if rc:
+        raise ValueError(f"ndt_scene_map failed ({rc})")
+    return out
+
+
+def make_scan(scene: SceneCfg, map_xyzw: np.ndarray, cfg: ScanCfg, scan_seed: int):
+    """Returns (scan float32 (n,4), gt_pose float64 (4,4), guess_pose float64 (4,4))."""
+    map_xyzw = np.ascontiguousarray(map_xyzw, dtype=np.float32)
+    scan = np.empty((cfg.n_points, 4), dtype=np.float32)
+    gt = np.empty((4, 4), dtype=np.float64)
+    guess = np.empty((4, 4), dtype=np.float64)
+    rc = scene_lib().ndt_scene_scan(C.byref(scene), map_xyzw.ctypes.data, map_xyzw.shape[0], C.byref(cfg),
+                                    scan_seed, scan.ctypes.data, gt.ctypes.data, guess.ctypes.data)
+    if rc:
+        raise ValueError(f"ndt_scene_scan failed ({rc})")
+    return scan, gt, guess
+
+
+def pose(x, y, z, roll, pitch, yaw) -> np.ndarray:
+    out = np.empty((4, 4), dtype=np.float64)
+    scene_lib().ndt_scene_pose(x, y, z, roll, pitch, yaw, out.ctypes.data)
+    return out
+
+
+# Workloads named in the benchmark configuration (sizes, perturbations, seeds).
+WORKLOADS = {
+    # plain single-scan NDT, CPU-runnable: 100 m scene, 100 k map, 2 k scan
+    "cfg0": dict(extent=100.0, map_points=100_000, scans=1, scan_points=(2000, 2000),
+                 perturb_xyz=0.5, perturb_yaw_deg=2.0, perturb_rollpitch_deg=0.0),
+    # plain NDT at real scale: 300 m scene, 1 M map, 1000 sequential 20 k scans
+    "cfg1": dict(extent=300.0, map_points=1_000_000, scans=1000, scan_points=(20000, 20000),
+                 perturb_xyz=0.5, perturb_yaw_deg=2.0, perturb_rollpitch_deg=0.0),
+    # batched: 512 scans, point counts log-uniform in [1 k, 60 k]
+    "cfg2": dict(extent=300.0, map_points=1_000_000, scans=512, scan_points=(1000, 60000),
+                 perturb_xyz=1.0, perturb_yaw_deg=3.0, perturb_rollpitch_deg=0.5),
+    # the batch of cfg2 with UTM-sized world offsets
+    "cfg3": dict(extent=300.0, map_points=1_000_000, scans=512, scan_points=(1000, 60000),
+                 perturb_xyz=1.0, perturb_yaw_deg=3.0, perturb_rollpitch_deg=0.5,
+                 origin=(4.2e5, 5.6e6, 80.0)),
+}
+
+
+@dataclass
+class Workload:
+    name: str
+    map_xyzw: np.ndarray
+    scans: list
+    gt: np.ndarray        # (S, 4, 4) float64
+    guesses: np.ndarray   # (S, 4, 4) float64
+    origin: tuple = (0.0, 0.0, 0.0)
+    scene: SceneCfg | None = None
+    meta: dict = field(default_factory=dict)
+
+
+def scan_sizes(lo: int, hi: int, count: int, seed: int) -> np.ndarray:
+    if lo == hi:
+        return np.full(count, lo, dtype=np.int64)
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
+
+
+def make_workload(name: str, seed: int = 1, scans: int | None = None, map_points: int | None = None,
+                  scan_points=None) -> Workload:
+    """Builds a named workload (optionally shrunk: fewer scans / points) from seeds."""
+    w = dict(WORKLOADS[name])
+    if scans is not None:
+        w["scans"] = scans
+    if map_points is not None:
+        w["map_points"] = map_points
+    if scan_points is not None:
+        w["scan_points"] = scan_points
+    origin = w.get("origin", (0.0, 0.0, 0.0))
+    scene = scene_cfg(seed, extent=w["extent"], origin=origin)
+    mp = make_map(scene, w["map_points"])
+    sizes = scan_sizes(w["scan_points"][0], w["scan_points"][1], w["scans"], seed + 7)
+    out_scans, gts, guesses = [], [], []
+    for s in range(w["scans"]):
+        cfg = scan_cfg(int(sizes[s]), w["perturb_xyz"], w["perturb_yaw_deg"], w["perturb_rollpitch_deg"])
+        sc, gt, gu = make_scan(scene, mp, cfg, s)
+        out_scans.append(sc)
+        gts.append(gt)
+        guesses.append(gu)
+    return Workload(name, mp, out_scans, np.stack(gts), np.stack(guesses), tuple(origin), scene,
+                    dict(seed=seed, **w))
+
+
+# ---------------------------------------------------------------------------
+# DAPHNE data files (src/CPU/ndt_mapping/kernel.cpp:143-217, :178-190)
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class Testcase:
+    guess: np.ndarray   # (4, 4) float32
+    scan: np.ndarray    # (n, 4) float32
+    map: np.ndarray     # (m, 4) float32
+
+
+@dataclass
+class CallbackResult:
+    """Same fields as the reference's CallbackResult (datatypes.h:51-55)."""
+    converged: bool
+    final_transformation: np.ndarray  # (4, 4) float32
+    fitness_score: float = 0.0
+
+
+def _open(path, mode):
+    return gzip.open(path, mode) if str(path).endswith(".gz") else open(path, mode)
+
+
+def write_ndt_input(path, cases) -> None:
+    with _open(path, "wb") as f:
+        f.write(struct.pack("<i", len(cases)))
+        for c in cases:
+            f.write(np.ascontiguousarray(c.guess, dtype="<f4").tobytes())
+            for cloud in (c.scan, c.map):
+                cloud = np.ascontiguousarray(cloud, dtype="<f4").reshape(-1, 4)
+                f.write(struct.pack("<i", cloud.shape[0]))
+                f.write(cloud.tobytes())
+
+
+def read_ndt_input(path) -> list:
+    with _open(path, "rb") as f:
+        data = f.read()
+    (t,) = struct.unpack_from("<i", data, 0)
+    pos, cases = 4, []
+    for _ in range(t):
+        guess = np.frombuffer(data, "<f4", 16, pos).reshape(4, 4).copy()
+        pos += 64
+        clouds = []
+        for _k in range(2):
+            (n,) = struct.unpack_from("<i", data, pos)
+            pos += 4
+            clouds.append(np.frombuffer(data, "<f4", 4 * n, pos).reshape(n, 4).copy())
+            pos += 16 * n
+        cases.append(Testcase(guess, clouds[0], clouds[1]))
+    return cases
+
+
+def write_ndt_output(path, results) -> None:
+    with _open(path, "wb") as f:
+        for r in results:
+            f.write(np.ascontiguousarray(r.final_transformation, dtype="<f4").tobytes())
+            f.write(struct.pack("<d", r.fitness_score))
+            f.write(struct.pack("<B", 1 if r.converged else 0))
+
+
+def read_ndt_output(path) -> list:
+    with _open(path, "rb") as f:
+        data = f.read()
+    out = []
+    for i in range(len(data) // 73):
+        rec = data[73 * i: 73 * (i + 1)]
+        m = np.frombuffer(rec, "<f4", 16, 0).reshape(4, 4).copy()
+        (fit,) = struct.unpack_from("<d", rec, 64)
+        out.append(CallbackResult(bool(rec[72]), m, fit))
+    return out
+
+
+MAX_TRANSLATION_EPS = 0.001   # kernel.cpp:18
+MAX_EPS = 2.0                 # kernel.cpp:20
+
+
+def check_outputs(results, references):
+    """The reference's acceptance test (check_next_outputs, kernel.cpp:1436-1496).
+    Returns (ok, max_delta)."""
+    error = False
+    max_delta = 0.0
+    origin = np.array([0.724, 0.447, 0.525, 1.0], dtype=np.float32)
+    for res, ref in zip(results, references):
+        if bool(res.converged) != bool(ref.converged):
+            error = True
+        a = np.asarray(res.final_transformation, dtype=np.float32)
+        b = np.asarray(ref.final_transformation, dtype=np.float32)
+        if np.any(np.isnan(a) != np.isnan(b)):
+            error = True
+        for h in range(4):
+            delta = float(np.float32(abs(np.float32(a[h, 3]) - np.float32(b[h, 3]))))
+            if delta > max_delta:
+                max_delta = delta
+                if delta > MAX_TRANSLATION_EPS:
+                    error = True
+        rp = np.zeros(4, dtype=np.float32)
+        fp = np.zeros(4, dtype=np.float32)
+        for h in range(4):
+            for w in range(4):
+                rp[h] = np.float32(rp[h] + np.float32(a[h, w] * origin[w]))
+                fp[h] = np.float32(fp[h] + np.float32(b[h, w] * origin[w]))
+        for w in range(4):
+            delta = float(abs(rp[w] - fp[w]))
+            if delta > max_delta:
+                max_delta = delta
+                if delta > MAX_EPS:
+                    error = True
+    return (not error), max_delta
+
+
+# ---------------------------------------------------------------------------
+# C ABI of libndthip.so (include/ndt_hip.h)
+# ---------------------------------------------------------------------------
+
+NDT_MODE_COMPAT, NDT_MODE_NDT = 0, 1
+NDT_POINT_FP32, NDT_POINT_FP64 = 0, 1
+NDT_SUM_FP32, NDT_SUM_COMP32, NDT_SUM_FP64 = 0, 1, 2
+
+
+class NdtParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("resolution", C.c_float), ("step_size", C.c_double), ("trans_eps", C.c_double),
+                ("max_iterations", C.c_int32), ("outlier_ratio", C.c_double), ("min_points", C.c_int32),
+                ("min_eig_ratio", C.c_double), ("point_precision", C.c_int32), ("sum_precision", C.c_int32),
+                ("origin", C.c_double * 3), ("block_size", C.c_int32), ("points_per_wg", C.c_int32)]
+
+
+class NdtResult(C.Structure):
+    _fields_ = [("final_transformation", C.c_float * 16), ("final_transformation_f64", C.c_double * 16),
+                ("fitness_score", C.c_double), ("score", C.c_double), ("trans_probability", C.c_double),
+                ("iterations", C.c_int32), ("converged", C.c_int32), ("passes", C.c_int32), ("status", C.c_int32)]
+
+
+class NdtGridInfo(C.Structure):
+    _fields_ = [("map_points", C.c_int64), ("voxels", C.c_int64), ("valid_voxels", C.c_int64),
+                ("query_cells", C.c_int64), ("candidates", C.c_int64), ("hash_capacity", C.c_int64),
+                ("dense_cells", C.c_int64), ("dims", C.c_int32 * 3), ("min_corner", C.c_float * 3),
+                ("max_corner", C.c_float * 3), ("build_ms", C.c_double)]
+
+
+class NdtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ndt error {code}: {msg}")
+        self.code = code
+
+
+_hip_lib = None
+HIP_LIB_PATH = os.path.join(PKG_DIR, "libndthip.so")
+
+
+def hip_lib():
+    """Loads libndthip.so; raises when it is missing (no fallback path exists)."""
+    global _hip_lib
+    if _hip_lib is None:
+        if not os.path.exists(HIP_LIB_PATH):
+            raise RuntimeError(f"{HIP_LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(HIP_LIB_PATH)
+        P = C.c_void_p
+        L.ndt_params_default.argtypes = [C.POINTER(NdtParams), C.c_int32]
+        L.ndt_last_error.restype = C.c_char_p
+        L.ndt_version.restype = C.c_char_p
+        L.ndt_ctx_create.argtypes = [C.c_int32, C.POINTER(P)]
+        L.ndt_ctx_destroy.argtypes = [P]
+        L.ndt_ctx_device_info.argtypes = [P, C.POINTER(C.c_int32), C.c_char_p, C.c_int32]
+        L.ndt_grid_build.argtypes = [P, P, C.c_int64, C.POINTER(NdtParams), C.POINTER(P)]
+        L.ndt_grid_info_get.argtypes = [P, C.POINTER(NdtGridInfo)]
+        L.ndt_grid_destroy.argtypes = [P]
+        L.ndt_grid_voxels.argtypes = [P, C.c_int64, P, P, P, P, P]
+        L.ndt_align.argtypes = [P, P, P, C.c_int64, P, C.POINTER(NdtParams), C.POINTER(NdtResult)]
+        L.ndt_align_batch.argtypes = [P, P, P, P, C.c_int32, P, C.POINTER(NdtParams), P]
+        L.ndt_scans_upload.argtypes = [P, P, P, C.c_int32, C.POINTER(P)]
+        L.ndt_scans_destroy.argtypes = [P]
+        L.ndt_align_scans.argtypes = [P, P, P, P, C.POINTER(NdtParams), P]
+        L.ndt_compat_callback.argtypes = [P, P, C.c_int64, P, P, C.c_int64, C.POINTER(NdtResult)]
+        L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
+        L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        _hip_lib = L
+    return _hip_lib
+
+
+EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_ctx_create", "ndt_ctx_destroy",
+                    "ndt_ctx_device_info", "ndt_grid_build", "ndt_grid_info_get", "ndt_grid_destroy",
+                    "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
+                    "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing")
+
+
+def _check(rc):
+    if rc != 0:
+        raise NdtError(rc, hip_lib().ndt_last_error().decode())
+
+
+def params(mode=NDT_MODE_NDT, **kw) -> NdtParams:
+    p = NdtParams()
+    hip_lib().ndt_params_default(C.byref(p), mode)
+    for k, v in kw.items():
+        if k == "origin":
+            for i in range(3):
+                p.origin[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def _result(r: NdtResult) -> dict:
+    return dict(final_f32=np.ctypeslib.as_array(r.final_transformation).reshape(4, 4).copy(),
+                final_f64=np.ctypeslib.as_array(r.final_transformation_f64).reshape(4, 4).copy(),
+                score=r.score, trans_probability=r.trans_probability, iterations=r.iterations,
+                converged=bool(r.converged), passes=r.passes, status=r.status)
+
+
+class Context:
+    """One HIP device context (one stream)."""
+
+    def __init__(self, device: int = 0):
+        self.h = C.c_void_p()
+        _check(hip_lib().ndt_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            hip_lib().ndt_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_info(self):
+        cus = C.c_int32()
+        name = C.create_string_buffer(256)
+        _check(hip_lib().ndt_ctx_device_info(self.h, C.byref(cus), name, 256))
+        return cus.value, name.value.decode()
+
+    def last_timing(self):
+        ms, launches, pairs = C.c_double(), C.c_int32(), C.c_int64()
+        _check(hip_lib().ndt_ctx_last_timing(self.h, C.byref(ms), C.byref(launches), C.byref(pairs)))
+        return ms.value, launches.value, pairs.value
+
+    def compat_callback(self, scan, guess, map_xyzw) -> dict:
+        """partial_points_callback with the reference's semantics."""
+        scan = np.ascontiguousarray(scan, dtype=np.float32)
+        map_xyzw = np.ascontiguousarray(map_xyzw, dtype=np.float32)
+        guess = np.ascontiguousarray(guess, dtype=np.float32)
+        r = NdtResult()
+        _check(hip_lib().ndt_compat_callback(self.h, scan.ctypes.data, scan.shape[0], guess.ctypes.data,
+                                             map_xyzw.ctypes.data, map_xyzw.shape[0], C.byref(r)))
+        return _result(r)
+
+
+class Grid:
+    def __init__(self, ctx: Context, map_xyzw, prm: NdtParams):
+        self.ctx = ctx
+        self.params = prm
+        m = np.ascontiguousarray(map_xyzw, dtype=np.float32)
+        self.h = C.c_void_p()
+        _check(hip_lib().ndt_grid_build(ctx.h, m.ctypes.data, m.shape[0], C.byref(prm), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            hip_lib().ndt_grid_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        i = NdtGridInfo()
+        _check(hip_lib().ndt_grid_info_get(self.h, C.byref(i)))
+        return {k: (list(getattr(i, k)) if k in ("dims", "min_corner", "max_corner") else getattr(i, k))
+                for k, _ in NdtGridInfo._fields_}
+
+    def voxels(self) -> dict:
+        k = self.info()["voxels"]
+        keys = np.zeros((k, 3), np.int64)
+        n = np.zeros(k, np.int32)
+        mean = np.zeros((k, 3))
+        icov = np.zeros((k, 9))
+        valid = np.zeros(k, np.int32)
+        _check(hip_lib().ndt_grid_voxels(self.h, k, keys.ctypes.data, n.ctypes.data, mean.ctypes.data,
+                                         icov.ctypes.data, valid.ctypes.data))
+        return dict(key=keys, n=n, mean=mean, icov=icov, valid=valid)
+
+    def derivatives(self, scan, p, compat_transform=None, prm: NdtParams | None = None):
+        prm = prm or self.params
+        scan = np.ascontiguousarray(scan, dtype=np.float32)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        M = None if compat_transform is None else np.ascontiguousarray(compat_transform, dtype=np.float32)
+        score = C.c_double()
+        g = np.zeros(6)
+        H = np.zeros(36)
+        pairs = C.c_int64()
+        _check(hip_lib().ndt_derivatives(self.ctx.h, self.h, scan.ctypes.data, scan.shape[0], p.ctypes.data,
+                                         None if M is None else M.ctypes.data, C.byref(prm), C.byref(score),
+                                         g.ctypes.data, H.ctypes.data, C.byref(pairs)))
+        return score.value, g, H.reshape(6, 6), pairs.value
+
+    def align(self, scan, guess, prm: NdtParams | None = None) -> dict:
+        prm = prm or self.params
+        scan = np.ascontiguousarray(scan, dtype=np.float32)
+        guess = np.ascontiguousarray(guess, dtype=np.float64)
+        r = NdtResult()
+        _check(hip_lib().ndt_align(self.ctx.h, self.h, scan.ctypes.data, scan.shape[0], guess.ctypes.data,
+                                   C.byref(prm), C.byref(r)))
+        return _result(r)
+
+    def align_batch(self, scans, guesses, prm: NdtParams | None = None) -> list:
+        prm = prm or self.params
+        scans = [np.ascontiguousarray(s, dtype=np.float32) for s in scans]
+        S = len(scans)
+        ptrs = (C.c_void_p * S)(*[s.ctypes.data for s in scans])
+        counts = np.array([s.shape[0] for s in scans], dtype=np.int64)
+        guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(S, 16)
+        out = (NdtResult * S)()
+        _check(hip_lib().ndt_align_batch(self.ctx.h, self.h, ptrs, counts.ctypes.data, S, guesses.ctypes.data,
+                                         C.byref(prm), out))
+        return [_result(out[i]) for i in range(S)]
+
+
+class Scans:
+    """Scan batch resident in device memory."""
+
+    def __init__(self, ctx: Context, scans):
+        self.ctx = ctx
+        self.arrays = [np.ascontiguousarray(s, dtype=np.float32) for s in scans]
+        S = len(self.arrays)
+        ptrs = (C.c_void_p * S)(*[s.ctypes.data for s in self.arrays])
+        self.counts = np.array([s.shape[0] for s in self.arrays], dtype=np.int64)
+        self.h = C.c_void_p()
+        _check(hip_lib().ndt_scans_upload(ctx.h, ptrs, self.counts.ctypes.data, S, C.byref(self.h)))
+        self.S = S
+
+    def close(self):
+        if self.h:
+            hip_lib().ndt_scans_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def align(self, grid: Grid, guesses, prm: NdtParams | None = None) -> list:
+        prm = prm or grid.params
+        guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(self.S, 16)
+        out = (NdtResult * self.S)()
+        _check(hip_lib().ndt_align_scans(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm), out))
+        return [_result(out[i]) for i in range(self.S)]
+
+
+def partial_points_callback(ctx: Context, input_cloud, init_guess, target_cloud) -> CallbackResult:
+    """The reference's per-testcase callback (kernel.cpp:1405-1414), on the GPU."""
+    r = ctx.compat_callback(input_cloud, init_guess, target_cloud)
+    return CallbackResult(r["converged"], r["final_f32"], 0.0)
