@@ -92,6 +92,7 @@ struct ndt_ctx {
     double last_ms = 0.0;
     int32_t last_launches = 0;
     int64_t last_pairs = 0;
+    double last_profile[4] = {0, 0, 0, 0};
 };
 
 struct ndt_grid {
@@ -136,7 +137,10 @@ extern "C" void ndt_params_default(ndt_params* p, int32_t mode) {
     }
     p->min_points = 6;
     p->min_eig_ratio = 0.01;
-    p->point_precision = NDT_POINT_FP32;
+    /* float64 per-point arithmetic by default: NDT trajectories on sparse scenes are
+     * sensitive enough that float32 per-pair rounding moves some results by more
+     * than a centimetre; float32 modes remain selectable */
+    p->point_precision = NDT_POINT_FP64;
     p->sum_precision = NDT_SUM_FP64;
 }
 
@@ -174,7 +178,7 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
         return fail(NDT_ERR_HIP, std::string("hipGetDeviceProperties: ") + hipGetErrorString(e));
     }
     c->cus = prop.multiProcessorCount;
-    std::snprintf(c->name, sizeof(c->name), "%s (%s)", prop.name, prop.gcnArchName);
+    std::snprintf(c->name, sizeof(c->name), "%.120s (%.120s)", prop.name, prop.gcnArchName);
     if (!prop.cooperativeLaunch) {
         delete c;
         return fail(NDT_ERR_UNSUPPORTED, "device lacks cooperative launch");
@@ -203,6 +207,12 @@ extern "C" int ndt_ctx_device_info(ndt_ctx* c, int32_t* cus, char* name, int32_t
     if (!c) return fail(NDT_ERR_INVALID, "ctx is null");
     if (cus) *cus = c->cus;
     if (name && name_len > 0) std::snprintf(name, (size_t)name_len, "%s", c->name);
+    return 0;
+}
+
+extern "C" int ndt_ctx_last_profile(ndt_ctx* c, double phases_us[4]) {
+    if (!c || !phases_us) return fail(NDT_ERR_INVALID, "null argument");
+    for (int k = 0; k < 4; k++) phases_us[k] = c->last_profile[k];
     return 0;
 }
 
@@ -540,6 +550,7 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     std::memset(&a, 0, sizeof(a));
     a.step_size = p->step_size;
     a.trans_eps = p->trans_eps;
+    a.resolution_d = (double)p->resolution;
     gauss_consts((double)p->resolution, p->outlier_ratio, &a.d1, &a.d2);
     for (int k = 0; k < 3; k++) a.origin[k] = p->origin[k];
     a.max_iterations = p->max_iterations;
@@ -636,6 +647,13 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         if (r.status) worst = r.status;
     }
     c->last_pairs = pairs;
+    for (int k = 0; k < 4; k++) c->last_profile[k] = 0.0;
+    for (int32_t s = 0; s < S; s++) {
+        c->last_profile[0] += out[s].t_prologue;
+        c->last_profile[1] += out[s].t_compute;
+        c->last_profile[2] += out[s].t_barrier;
+        c->last_profile[3] += out[s].t_control;
+    }
     if (worst) return fail(worst, "a scan's device-side wait exceeded its bound");
     return 0;
 }

@@ -264,6 +264,7 @@ struct ScanState {
     double phi_0, d_phi_0, a_l, f_l, g_l, a_u, f_u, g_u, a_t;
     double final_m[16];
     double pairs;      /* (point, voxel) pairs summed over all passes */
+    double pp[32];     /* pass parameters of x_t (rotation, translation, axes, constants) */
     int32_t iter, converged;
     int32_t phase, done;
     int32_t step_iterations, open_interval;
@@ -283,6 +284,7 @@ struct ScanDesc {
 
 struct AlignParams {
     double step_size, trans_eps, d1, d2;
+    double resolution_d;
     double origin[3];
     int32_t max_iterations;
     int32_t point_fp64;
@@ -302,6 +304,7 @@ struct ScanOut {
     double final_m[16];
     double score;
     double pairs;
+    double t_prologue, t_compute, t_barrier, t_control; /* microseconds, workgroup 0 of the scan */
     int32_t iterations, converged, passes, status;
 };
 

@@ -354,8 +354,9 @@ int radix_sort_pairs(uint64_t* keys, uint32_t* vals, uint64_t* keys_tmp, uint32_
         uint32_t* tv = vi; vi = vo; vo = tv;
     }
     if (ki != keys) {
-        hipMemcpyAsync(keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, stream);
-        hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream);
+        hipError_t e = hipMemcpyAsync(keys, ki, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(vals, vi, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) return (int)e;
     }
     return (int)hipGetLastError();
 }
@@ -1011,6 +1012,65 @@ __device__ inline void block_reduce_acc(double* vals /* NRED per thread, registe
  * computeStepLengthMT (src/CPU/ndt_mapping/kernel.cpp:1015-1144, :1175-1272) in
  * their textbook form (line search live, translation updated, true convergence). */
 
+/* Gaussian elimination with partial pivoting (rows of A and b swapped together),
+ * unrolled so that the system stays in registers.  Same arithmetic as the
+ * reference's solve (kernel.cpp:293-343) apart from the right-hand-side swap,
+ * which the reference omits. */
+__device__ inline void solve6_pivot(double* result, const double* Ain, const double* bin) {
+    double A[6][6], b[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        b[i] = bin[i];
+#pragma unroll
+        for (int k = 0; k < 6; k++) A[i][k] = Ain[i * 6 + k];
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        double mx = fabs(A[j][j]);
+        int mi = j;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double v = fabs(A[i][j]);
+            if (v > mx) {
+                mi = i;
+                mx = v;
+            }
+        }
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            const bool sw = (mi == i);
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                double t = A[i][k];
+                A[i][k] = sw ? A[j][k] : t;
+                A[j][k] = sw ? t : A[j][k];
+            }
+            double t = b[i];
+            b[i] = sw ? b[j] : t;
+            b[j] = sw ? t : b[j];
+        }
+        if (mx == 0.0) A[j][j] = 0.001;
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
+            double pivot = A[i][j] / A[j][j];
+#pragma unroll
+            for (int k = 0; k < 6; k++) A[i][k] = A[i][k] - pivot * A[j][k];
+            b[i] = b[i] - pivot * b[j];
+        }
+    }
+    double r[6];
+    r[5] = b[5] / A[5][5];
+#pragma unroll
+    for (int i = 4; i >= 0; i--) {
+        double sum = 0.0;
+#pragma unroll
+        for (int j = i + 1; j < 6; j++) sum = sum + A[i][j] * r[j];
+        r[i] = (b[i] - sum) / A[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; i++) result[i] = r[i];
+}
+
 __device__ __attribute__((noinline)) void ctl_finish(ScanState& st, const AlignParams& prm, int converged, ScanOut* out) {
     st.converged = converged;
     st.done = 1;
@@ -1044,7 +1104,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
             ng[i] = -st.grad[i];
             for (int j = 0; j < 6; j++) H[i * 6 + j] = st.hess[i <= j ? tri_index(i, j) : tri_index(j, i)];
         }
-        solve6(delta, H, ng);
+        solve6_pivot(delta, H, ng);
         double norm = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2] + delta[3] * delta[3] +
                            delta[4] * delta[4] + delta[5] * delta[5]);
         if (norm == 0 || norm != norm) {
@@ -1081,6 +1141,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
         a_t = dmax_std(a_t, prm.trans_eps / 2);
         st.a_t = a_t;
         for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
+        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
         st.phase = PH_MT_FIRST;
         return;
     }
@@ -1129,6 +1190,7 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
         a_t = dmax_std(a_t, prm.trans_eps / 2);
         st.a_t = a_t;
         for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
+        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
         st.phase = PH_MT_LOOP;
         return;
     }
@@ -1167,6 +1229,7 @@ __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, Ali
     st.x_t[4] = ea[1];
     st.x_t[5] = ea[2];
     for (int k = 0; k < 6; k++) st.p[k] = st.x_t[k];
+    pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
     st.phase = PH_INIT;
     states[s] = st;
     sync[s].arrive = 0;
@@ -1213,18 +1276,14 @@ __global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, 
     ScanState* st = states + s;
     GroupSync* gs = sync + s;
     uint32_t my_gen = 0;
+    double prof[4] = {0, 0, 0, 0};
     for (;;) {
-        if (tid == 0) {
-            int done = ld_agent_i32(&st->done);
-            s_done = done;
-            if (!done) {
-                double x[6];
-                for (int k = 0; k < 6; k++) x[k] = ld_agent_f64(&st->x_t[k]);
-                make_pass_params<T>(x, prm, g.resolution_d, pp);
-            }
-        }
+        uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        if (tid == 0) s_done = ld_agent_i32(&st->done);
+        if (tid < 32) ((T*)&pp)[tid] = (T)ld_agent_f64(&st->pp[tid]);
         __syncthreads();
         if (s_done) break;
+        uint64_t t1 = __builtin_amdgcn_s_memrealtime();
         Acc<T, SUM> acc;
         acc.zero();
         int pairs = 0;
@@ -1237,6 +1296,8 @@ __global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, 
         for (int k = 0; k < NACC; k++) v[k] = acc.get(k);
         v[NACC] = (double)pairs;
         block_reduce_acc<BLOCK>(v, lds_red, red);
+        uint64_t t2 = __builtin_amdgcn_s_memrealtime();
+        uint64_t t3 = t2;
         if (G > 1) {
             if (tid < NRED) st_agent_f64(partials + (int64_t)wg * 32 + tid, red[tid]);
             __syncthreads();
@@ -1259,6 +1320,7 @@ __global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, 
                 red[tid] = acc2;
             }
             __syncthreads();
+            t3 = __builtin_amdgcn_s_memrealtime();
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
             __syncthreads();
             for (int k = tid; k < STATE_WORDS; k += BLOCK) st_agent_u64((uint64_t*)st + k, sstate[k]);
@@ -1291,6 +1353,20 @@ __global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, 
         }
         my_gen++;
         __syncthreads();
+        if (local == 0 && tid == 0) {
+            uint64_t t4 = __builtin_amdgcn_s_memrealtime();
+            prof[0] += (double)(t1 - t0);
+            prof[1] += (double)(t2 - t1);
+            prof[2] += (double)(t4 - t2) - (double)(s_last ? (t4 - t3) : 0);
+            prof[3] += (double)(s_last ? (t4 - t3) : 0);
+        }
+    }
+    if (local == 0 && tid == 0) {
+        /* s_memrealtime counts at 100 MHz */
+        out[s].t_prologue = prof[0] * 0.01;
+        out[s].t_compute = prof[1] * 0.01;
+        out[s].t_barrier = prof[2] * 0.01;
+        out[s].t_control = prof[3] * 0.01;
     }
 }
 

@@ -1,0 +1,247 @@
+/*
+ * kernel.cpp - the ndt_mapping testcase kernel of the DAPHNE benchmark on the
+ * MI355X engine.  Same data files, same per-testcase callback and the same output
+ * check as the reference (src/CPU/ndt_mapping/kernel.cpp): testcases are read from
+ * <data>/ndt_input.dat (:143-217) in blocks of p, each block is timed around
+ *     results[i] = partial_points_callback(scan[i], guess[i], map[i])   (:1416-1434)
+ * and compared with <data>/ndt_output.dat (:1436-1496).  The callback calls the C
+ * ABI of libndthip.so (ndt_compat_callback: grid build and alignment on the GPU
+ * with the reference's semantics).
+ *
+ * <data> is ../../../data as in the reference, or $DAPHNE_DATA_DIR when set.
+ * $NDT_HARNESS_MODE=ndt runs the textbook NDT mode instead (its results do not
+ * follow the reference's output file, which the check then reports).
+ */
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "benchmark.h"
+#include "ndt_hip.h"
+
+namespace {
+
+struct PointXYZI {
+    float data[4];
+};
+struct Matrix4f {
+    float data[4][4];
+};
+struct CallbackResult {
+    bool converged;
+    Matrix4f final_transformation;
+    double fitness_score;
+};
+typedef std::vector<PointXYZI> PointCloud;
+
+/* acceptance thresholds of the reference (kernel.cpp:18-20) */
+const double MAX_TRANSLATION_EPS = 0.001;
+const double MAX_EPS = 2;
+
+class IoError : public std::exception {
+public:
+    explicit IoError(const char* m) : msg(m) {}
+    const char* what() const noexcept override { return msg; }
+
+private:
+    const char* msg;
+};
+
+template <class T>
+void read_raw(std::ifstream& f, T* v, size_t count, const char* what) {
+    f.read(reinterpret_cast<char*>(v), (std::streamsize)(sizeof(T) * count));
+    if (!f) throw IoError(what);
+}
+
+class ndt_mapping : public kernel {
+public:
+    void init() override;
+    void run(int p = 1) override;
+    bool check_output() override;
+    ~ndt_mapping() override {
+        if (grid_) ndt_grid_destroy(grid_);
+        if (ctx_) ndt_ctx_destroy(ctx_);
+    }
+
+protected:
+    int read_next_testcases(int count) override;
+    void check_next_outputs(int count);
+    CallbackResult partial_points_callback(PointCloud& input_cloud, Matrix4f& init_guess, PointCloud& target_cloud);
+
+private:
+    int read_testcases = 0;
+    std::vector<PointCloud> scans, maps;
+    std::vector<Matrix4f> guesses;
+    std::vector<CallbackResult> results;
+    std::ifstream input_file, output_file;
+    bool error_so_far = false;
+    double max_delta = 0.0;
+    ndt_ctx* ctx_ = nullptr;
+    ndt_grid* grid_ = nullptr;
+    bool textbook = false;
+};
+
+std::string data_dir() {
+    const char* d = std::getenv("DAPHNE_DATA_DIR");
+    return d ? std::string(d) : std::string("../../../data");
+}
+
+void die_abi(const char* what) {
+    std::cerr << what << ": " << ndt_last_error() << std::endl;
+    std::exit(-4);
+}
+
+void ndt_mapping::init() {
+    std::cout << "init\n";
+    const std::string dir = data_dir();
+    input_file.open(dir + "/ndt_input.dat", std::ios::binary);
+    if (!input_file) {
+        std::cerr << "Error opening the testcase file" << std::endl;
+        std::exit(-3);
+    }
+    output_file.open(dir + "/ndt_output.dat", std::ios::binary);
+    if (!output_file) {
+        std::cerr << "Error opening the results file" << std::endl;
+        std::exit(-3);
+    }
+    try {
+        int32_t number = 0;
+        read_raw(input_file, &number, 1, "Error reading number of test cases");
+        testcases = (uint32_t)number;
+    } catch (IoError& e) {
+        std::cerr << e.what() << std::endl;
+        std::exit(-3);
+    }
+    const char* mode = std::getenv("NDT_HARNESS_MODE");
+    textbook = mode && std::strcmp(mode, "ndt") == 0;
+    if (ndt_ctx_create(0, &ctx_) != NDT_OK) die_abi("ndt_ctx_create");
+    error_so_far = false;
+    max_delta = 0.0;
+    std::cout << "done\n" << std::endl;
+}
+
+void read_cloud(std::ifstream& f, PointCloud& cloud) {
+    int32_t size = 0;
+    read_raw(f, &size, 1, "Error reading filtered scan");
+    if (size < 0) throw IoError("Error reading filtered scan");
+    cloud.resize((size_t)size);
+    if (size > 0) read_raw(f, cloud.data()->data, 4 * (size_t)size, "Error reading filtered scan");
+}
+
+int ndt_mapping::read_next_testcases(int count) {
+    scans.assign(count, PointCloud());
+    maps.assign(count, PointCloud());
+    guesses.assign(count, Matrix4f());
+    results.assign(count, CallbackResult());
+    int i;
+    for (i = 0; (i < count) && (read_testcases < (int)testcases); i++, read_testcases++) {
+        try {
+            read_raw(input_file, &guesses[i].data[0][0], 16, "Error reading initial guess");
+            read_cloud(input_file, scans[i]);
+            read_cloud(input_file, maps[i]);
+        } catch (IoError& e) {
+            std::cerr << e.what() << std::endl;
+            std::exit(-3);
+        }
+    }
+    return i;
+}
+
+CallbackResult ndt_mapping::partial_points_callback(PointCloud& input_cloud, Matrix4f& init_guess,
+                                                    PointCloud& target_cloud) {
+    CallbackResult result;
+    std::memset(&result, 0, sizeof(result));
+    ndt_result r;
+    if (!textbook) {
+        if (ndt_compat_callback(ctx_, input_cloud.data()->data, (int64_t)input_cloud.size(), &init_guess.data[0][0],
+                                target_cloud.data()->data, (int64_t)target_cloud.size(), &r) != NDT_OK)
+            die_abi("ndt_compat_callback");
+    } else {
+        ndt_params prm;
+        ndt_params_default(&prm, NDT_MODE_NDT);
+        if (grid_) ndt_grid_destroy(grid_);
+        grid_ = nullptr;
+        if (ndt_grid_build(ctx_, target_cloud.data()->data, (int64_t)target_cloud.size(), &prm, &grid_) != NDT_OK)
+            die_abi("ndt_grid_build");
+        double g[16];
+        for (int k = 0; k < 16; k++) g[k] = (&init_guess.data[0][0])[k];
+        if (ndt_align(ctx_, grid_, input_cloud.data()->data, (int64_t)input_cloud.size(), g, &prm, &r) != NDT_OK)
+            die_abi("ndt_align");
+    }
+    std::memcpy(&result.final_transformation.data[0][0], r.final_transformation, sizeof(float) * 16);
+    result.converged = r.converged != 0;
+    result.fitness_score = r.fitness_score;
+    return result;
+}
+
+void ndt_mapping::run(int p) {
+    pause_func();
+    while (read_testcases < (int)testcases) {
+        int count = read_next_testcases(p);
+        unpause_func();
+        for (int i = 0; i < count; i++) results[i] = partial_points_callback(scans[i], guesses[i], maps[i]);
+        pause_func();
+        check_next_outputs(count);
+    }
+}
+
+void ndt_mapping::check_next_outputs(int count) {
+    for (int i = 0; i < count; i++) {
+        CallbackResult reference;
+        try {
+            read_raw(output_file, &reference.final_transformation.data[0][0], 16, "Error reading reference result");
+            read_raw(output_file, &reference.fitness_score, 1, "Error reading reference result");
+            uint8_t conv = 0;
+            read_raw(output_file, &conv, 1, "Error reading reference result");
+            reference.converged = conv != 0;
+        } catch (IoError& e) {
+            std::cerr << e.what() << std::endl;
+            std::exit(-3);
+        }
+        const CallbackResult& res = results[i];
+        if (res.converged != reference.converged) error_so_far = true;
+        for (int h = 0; h < 4; h++) {
+            for (int w = 0; w < 4; w++)
+                if (std::isnan(res.final_transformation.data[h][w]) != std::isnan(reference.final_transformation.data[h][w]))
+                    error_so_far = true;
+            float delta = std::fabs(res.final_transformation.data[h][3] - reference.final_transformation.data[h][3]);
+            if (delta > max_delta) {
+                max_delta = delta;
+                if (delta > MAX_TRANSLATION_EPS) error_so_far = true;
+            }
+        }
+        const float origin[4] = {0.724f, 0.447f, 0.525f, 1.0f};
+        float res_pt[4] = {0, 0, 0, 0}, ref_pt[4] = {0, 0, 0, 0};
+        for (int h = 0; h < 4; h++)
+            for (int w = 0; w < 4; w++) {
+                res_pt[h] += res.final_transformation.data[h][w] * origin[w];
+                ref_pt[h] += reference.final_transformation.data[h][w] * origin[w];
+            }
+        for (int w = 0; w < 4; w++) {
+            float delta = std::fabs(res_pt[w] - ref_pt[w]);
+            if (delta > max_delta) {
+                max_delta = delta;
+                if (delta > MAX_EPS) error_so_far = true;
+            }
+        }
+    }
+}
+
+bool ndt_mapping::check_output() {
+    std::cout << "checking output \n";
+    input_file.close();
+    output_file.close();
+    std::cout << "max delta: " << max_delta << "\n";
+    return !error_so_far;
+}
+
+ndt_mapping the_kernel;
+
+}  // namespace
+
+kernel& myKernel = the_kernel;

@@ -373,6 +373,7 @@ def hip_lib():
         L.ndt_compat_callback.argtypes = [P, P, C.c_int64, P, P, C.c_int64, C.POINTER(NdtResult)]
         L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
+        L.ndt_ctx_last_profile.argtypes = [P, P]
         _hip_lib = L
     return _hip_lib
 
@@ -380,7 +381,8 @@ def hip_lib():
 EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_ctx_create", "ndt_ctx_destroy",
                     "ndt_ctx_device_info", "ndt_grid_build", "ndt_grid_info_get", "ndt_grid_destroy",
                     "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
-                    "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing")
+                    "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing",
+                    "ndt_ctx_last_profile")
 
 
 def _check(rc):
@@ -435,6 +437,12 @@ class Context:
         ms, launches, pairs = C.c_double(), C.c_int32(), C.c_int64()
         _check(hip_lib().ndt_ctx_last_timing(self.h, C.byref(ms), C.byref(launches), C.byref(pairs)))
         return ms.value, launches.value, pairs.value
+
+    def last_profile(self):
+        """(prologue, compute, barrier, control) microseconds of the last ndt alignment."""
+        out = np.zeros(4)
+        _check(hip_lib().ndt_ctx_last_profile(self.h, out.ctypes.data))
+        return out
 
     def compat_callback(self, scan, guess, map_xyzw) -> dict:
         """partial_points_callback with the reference's semantics."""

@@ -99,7 +99,7 @@ typedef struct ndt_scans ndt_scans;
 /* Fills the defaults of a mode: compat = the reference's members (outlier 0.55,
  * resolution 1, step 0.1, epsilon 0.1, max_iterations 0); ndt = resolution 1,
  * step 0.1, epsilon 0.01, 30 iterations, outlier 0.55, 6 points per voxel,
- * eigenvalue ratio 0.01, fp32 point math, fp64 sums. */
+ * eigenvalue ratio 0.01, fp64 point math, fp64 sums. */
 void ndt_params_default(ndt_params* params, int32_t mode);
 
 /* Message of the last failed call on this thread ("" if none). */
@@ -172,6 +172,12 @@ int ndt_derivatives(ndt_ctx* ctx, const ndt_grid* grid, const float* scan_xyzw, 
  * context: device milliseconds of the alignment kernel(s), measured with HIP
  * events on the context's stream, and the number of kernel launches. */
 int ndt_ctx_last_timing(ndt_ctx* ctx, double* align_kernel_ms, int32_t* launches, int64_t* pairs);
+
+/* Phase profile of the last ndt alignment on this context, in microseconds summed
+ * over scans, as seen by the first workgroup of each scan: pass-parameter load,
+ * point compute + workgroup reduction, waiting for the scan's other workgroups,
+ * and the Newton / line-search controller. */
+int ndt_ctx_last_profile(ndt_ctx* ctx, double phases_us[4]);
 
 #ifdef __cplusplus
 }

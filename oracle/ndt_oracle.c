@@ -81,6 +81,15 @@ void orc_angle_tables(const double p[6], double j[8][3], double h[15][3]) {
     h[14][0] = (-sx * sz + cx * sy * cz); h[14][1] = (-cx * sy * sz - sx * cz); h[14][2] = 0;
 }
 
+void orc_ndt_angle_tables(const double p[6], double j[8][3], double h[15][3]) {
+    /* The reference's tables, except h_ang_d1_[2] (kernel.cpp:786): the second
+     * derivative of Rx Ry Rz x with respect to pitch has -sin(pitch) in its third
+     * component, the reference (like the PCL code it derives from) stores +sin(pitch).
+     * The textbook mode uses the exact derivative (pinned by finite differences). */
+    orc_angle_tables(p, j, h);
+    h[6][2] = -h[6][2];
+}
+
 void orc_point_derivs(const double x[3], const double j[8][3], const double h[15][3],
                       double pg[3][6], double ph[18][6]) {
     /* point_gradient_ initialisation kernel.cpp:1191-1195, computePointDerivatives :495-560 */
@@ -126,6 +135,38 @@ void orc_solve6(double result[6], const double Ain[36], const double bin[6]) {
         if (max == 0.0) A[j][j] = ORC_SINGULAR_PIVOT;
         for (int i = j + 1; i < 6; i++) {
             pivot = A[i][j] / A[j][j];
+            for (int k = 0; k < 6; k++) A[i][k] = A[i][k] - pivot * A[j][k];
+            b[i] = b[i] - pivot * b[j];
+        }
+    }
+    result[5] = b[5] / A[5][5];
+    for (int i = 4; i >= 0; i--) {
+        double sum = 0.0;
+        for (int j = i + 1; j < 6; j++) sum = sum + A[i][j] * result[j];
+        result[i] = (b[i] - sum) / A[i][i];
+    }
+}
+
+void orc_solve6_pivot(double result[6], const double Ain[36], const double bin[6]) {
+    /* Gaussian elimination with partial pivoting that swaps the right-hand side
+     * together with the rows of A (the textbook mode's solve).  The reference's
+     * solve (kernel.cpp:306-313) swaps the rows of A only, which leaves b in the
+     * wrong order whenever a pivot row moves. */
+    double A[6][6], b[6];
+    memcpy(A, Ain, sizeof(A));
+    memcpy(b, bin, sizeof(b));
+    for (int j = 0; j < 6; j++) {
+        double max = fabs(A[j][j]);
+        int mi = j;
+        for (int i = j + 1; i < 6; i++)
+            if (fabs(A[i][j]) > max) { mi = i; max = fabs(A[i][j]); }
+        if (mi != j) {
+            for (int i = 0; i < 6; i++) { double t = A[mi][i]; A[mi][i] = A[j][i]; A[j][i] = t; }
+            double t = b[mi]; b[mi] = b[j]; b[j] = t;
+        }
+        if (max == 0.0) A[j][j] = ORC_SINGULAR_PIVOT;
+        for (int i = j + 1; i < 6; i++) {
+            double pivot = A[i][j] / A[j][j];
             for (int k = 0; k < 6; k++) A[i][k] = A[i][k] - pivot * A[j][k];
             b[i] = b[i] - pivot * b[j];
         }
@@ -911,7 +952,7 @@ static double ndt_pass(const orc_ndt_grid* g, const orc_ndt_params* prm, double 
     double Hu[36];
     memset(Hu, 0, sizeof(Hu));
     if (grad) memset(grad, 0, sizeof(double) * 6);
-    orc_angle_tables(p, jang, hang);
+    orc_ndt_angle_tables(p, jang, hang);
     orc_ndt_pose_matrix(p, M);
     const double res = g->resolution;
     double score = 0.0;
@@ -1077,7 +1118,7 @@ int orc_ndt_align(const orc_ndt_grid* g, const orc_ndt_params* prm, const float*
     while (iter < prm->max_iterations) {
         double neg_grad[6] = {-grad[0], -grad[1], -grad[2], -grad[3], -grad[4], -grad[5]};
         double delta_p[6], x_t[6];
-        orc_solve6(delta_p, H, neg_grad);
+        orc_solve6_pivot(delta_p, H, neg_grad);
         double norm = sqrt(delta_p[0] * delta_p[0] + delta_p[1] * delta_p[1] + delta_p[2] * delta_p[2] +
                            delta_p[3] * delta_p[3] + delta_p[4] * delta_p[4] + delta_p[5] * delta_p[5]);
         if (norm == 0 || norm != norm) {

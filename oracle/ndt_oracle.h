@@ -66,9 +66,14 @@ typedef struct orc_result {
 /* ---------------- shared helpers (pinned by the reference) ---------------- */
 void orc_gauss_consts(double resolution, double outlier_ratio, double* d1, double* d2);
 void orc_angle_tables(const double p[6], double jang[8][3], double hang[15][3]);
+/* The tables of the textbook mode: orc_angle_tables with h_ang_d1_[2] negated
+ * (the exact second derivative; see ndt_oracle.c). */
+void orc_ndt_angle_tables(const double p[6], double jang[8][3], double hang[15][3]);
 void orc_point_derivs(const double x[3], const double jang[8][3], const double hang[15][3],
                       double pg[3][6], double ph[18][6]);
 void orc_solve6(double result[6], const double A[36], const double b[6]);
+/* Textbook-mode solve: as orc_solve6 but the right-hand side is swapped with the rows. */
+void orc_solve6_pivot(double result[6], const double A[36], const double b[6]);
 int orc_mt_update_interval(double* a_l, double* f_l, double* g_l, double* a_u, double* f_u,
                            double* g_u, double a_t, double f_t, double g_t);
 double orc_mt_trial_value(double a_l, double f_l, double g_l, double a_u, double f_u, double g_u,

@@ -65,6 +65,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         L.orc_gauss_consts.argtypes = [D, D, P, P]
         L.orc_angle_tables.argtypes = [P, P, P]
+        L.orc_ndt_angle_tables.argtypes = [P, P, P]
         L.orc_point_derivs.argtypes = [P, P, P, P, P]
         L.orc_solve6.argtypes = [P, P, P]
         L.orc_mt_update_interval.argtypes = [P, P, P, P, P, P, D, D, D]
@@ -185,16 +186,18 @@ def gauss_consts(resolution=1.0, outlier=0.55):
     return d1.value, d2.value
 
 
-def angle_tables(p):
+def angle_tables(p, exact: bool = False):
+    """The reference's angle tables; exact=True gives the textbook mode's tables."""
     p = _f64(p)
     j = np.zeros((8, 3))
     h = np.zeros((15, 3))
-    lib().orc_angle_tables(p.ctypes.data, j.ctypes.data, h.ctypes.data)
+    fn = lib().orc_ndt_angle_tables if exact else lib().orc_angle_tables
+    fn(p.ctypes.data, j.ctypes.data, h.ctypes.data)
     return j, h
 
 
-def point_derivs(x, p):
-    j, h = angle_tables(p)
+def point_derivs(x, p, exact: bool = False):
+    j, h = angle_tables(p, exact)
     x = _f64(x)
     pg = np.zeros((3, 6))
     ph = np.zeros((18, 6))
