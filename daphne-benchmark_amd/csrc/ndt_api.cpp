@@ -658,6 +658,21 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     return 0;
 }
 
+/* Spreads the bits of a 10-bit value to every third bit (Morton interleave). */
+static inline uint32_t spread3(uint32_t v) {
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x030000FF;
+    v = (v | (v << 8)) & 0x0300F00F;
+    v = (v | (v << 4)) & 0x030C30C3;
+    v = (v | (v << 2)) & 0x09249249;
+    return v;
+}
+
+/* Copies scans into one device buffer, each scan's points reordered along a Morton
+ * curve of 1 m cells of its own (sensor-frame) coordinates: consecutive points,
+ * and so the points of one workgroup, then look up neighbouring voxels and hit the
+ * same cache lines.  A rigid transform keeps that locality.  The alignment does not
+ * depend on the point order beyond floating-point rounding of the sums. */
 static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const int64_t* counts, int32_t S,
                       std::vector<int64_t>& offsets, int64_t* total) {
     offsets.assign(S, 0);
@@ -669,9 +684,30 @@ static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const 
         t += counts[s];
     }
     BUF_TRY(dst, sizeof(float4) * t);
-    for (int32_t s = 0; s < S; s++)
-        HIP_TRY(hipMemcpyAsync((char*)dst.p + sizeof(float4) * offsets[s], scans[s], sizeof(float4) * counts[s],
-                               hipMemcpyHostToDevice, c->stream));
+    std::vector<float> staging((size_t)(4 * t));
+    std::vector<std::pair<uint32_t, uint32_t>> order;
+    for (int32_t s = 0; s < S; s++) {
+        const float* src = scans[s];
+        const int64_t n = counts[s];
+        order.resize((size_t)n);
+        for (int64_t i = 0; i < n; i++) {
+            uint32_t q[3];
+            for (int k = 0; k < 3; k++) {
+                float v = src[4 * i + k];
+                double cell = std::isfinite(v) ? std::floor((double)v) + 512.0 : 0.0;
+                q[k] = (uint32_t)std::min(1023.0, std::max(0.0, cell));
+            }
+            order[(size_t)i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), (uint32_t)i};
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                             return a.first < b.first;
+                         });
+        float* d = staging.data() + 4 * offsets[s];
+        for (int64_t i = 0; i < n; i++) std::memcpy(d + 4 * i, src + 4 * (size_t)order[(size_t)i].second, 16);
+    }
+    HIP_TRY(hipMemcpyAsync(dst.p, staging.data(), sizeof(float4) * t, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     *total = t;
     return 0;
 }

@@ -886,9 +886,23 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     const T J3y = -y2, J3z = y1;
     const T J4x = u.a4[1] * y2 - u.a4[2] * y1, J4y = u.a4[2] * y0 - u.a4[0] * y2, J4z = u.a4[0] * y1 - u.a4[1] * y0;
     const T J5x = u.a5[1] * y2 - u.a5[2] * y1, J5y = u.a5[2] * y0 - u.a5[0] * y2, J5z = u.a5[0] * y1 - u.a5[1] * y0;
+    /* Candidate filter first: branch-free over the (at most 27) list entries so
+     * that their loads are in flight together; the float test carries a margin in
+     * fp64 mode, where the exact test below decides. */
+    const float fx = (float)xt0, fy = (float)xt1, fz = (float)xt2;
+    const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.001f + 1e-3f;
+    uint32_t mask = 0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < count; k++) {
+        const float4 cd = g.cand[begin + k];
+        const float e0 = fx - cd.x, e1 = fy - cd.y, e2 = fz - cd.z;
+        mask |= (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
+    }
     int pairs = 0;
-    for (uint32_t ci = begin; ci < begin + count; ci++) {
-        const float4 cd = g.cand[ci];
+    while (mask) {
+        const uint32_t k = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1;
+        const float4 cd = g.cand[begin + k];
         const uint32_t v = __float_as_uint(cd.w);
         T d0, d1, d2;
         if constexpr (sizeof(T) == 4) {
@@ -1251,12 +1265,17 @@ int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanSt
  * controller on the scan's state and releases the others.  All waits are bounded. */
 
 constexpr uint32_t SPIN_LIMIT = 1u << 24;
-#ifndef ALIGN_MIN_WAVES
-#define ALIGN_MIN_WAVES 4
+/* waves per SIMD the register allocation must allow: the float64 variant needs
+ * about 170 VGPRs without spilling in the point loop, the float32 ones about 120 */
+#ifndef ALIGN_MIN_WAVES_F64
+#define ALIGN_MIN_WAVES_F64 2
+#endif
+#ifndef ALIGN_MIN_WAVES_F32
+#define ALIGN_MIN_WAVES_F32 3
 #endif
 
 template <typename T, int SUM, int BLOCK>
-__global__ __launch_bounds__(BLOCK, ALIGN_MIN_WAVES) void k_align(NdtGridDev g, const float4* __restrict__ pts,
+__global__ __launch_bounds__(BLOCK, sizeof(T) == 8 ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(NdtGridDev g, const float4* __restrict__ pts,
                                                  const ScanDesc* __restrict__ scans, const int32_t* __restrict__ wg_scan,
                                                  AlignParams prm, ScanState* states, GroupSync* sync, double* partials,
                                                  ScanOut* out) {
@@ -1424,7 +1443,7 @@ int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, cons
 /* ------------------------------------------------------------------ single derivative pass (ndt) */
 
 template <typename T, int SUM>
-__global__ __launch_bounds__(256, ALIGN_MIN_WAVES) void k_deriv_ndt(NdtGridDev g, const float4* __restrict__ pts, int64_t n,
+__global__ __launch_bounds__(256, sizeof(T) == 8 ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_deriv_ndt(NdtGridDev g, const float4* __restrict__ pts, int64_t n,
                                                    const double* __restrict__ pose, AlignParams prm,
                                                    double* __restrict__ partials) {
     __shared__ PassParams<T> pp;
