@@ -88,7 +88,7 @@ struct ndt_ctx {
     int cus = 0;
     char name[256] = {0};
     /* workspaces */
-    DevBuf pts, desc, wgmap, states, sync, partials, out, guesses, small, partials2;
+    DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2;
     double last_ms = 0.0;
     int32_t last_launches = 0;
     int64_t last_pairs = 0;
@@ -367,7 +367,8 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     if (nq > 0) {
         LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, 5, g->slots.as<HashSlot>(),
                                       log2cap, c->stream));
-        LAUNCH_TRY(launch_cand_fill(vals.as<uint32_t>(), ne, g->mean_c.as<double>(), g->cand.as<float4>(), c->stream));
+        LAUNCH_TRY(launch_cand_fill(keys.as<uint64_t>(), vals.as<uint32_t>(), ne, g->mean_c.as<double>(), g->bits,
+                                    g->origin_cell, (double)g->params.resolution, g->cand.as<float4>(), c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     g->info.query_cells = nq;
@@ -560,7 +561,11 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     return a;
 }
 
-/* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host. */
+/* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host:
+ * one cooperative launch per group of up to MAX_BATCH_SCANS scans.  The chunk size
+ * (points per partial sum) is a constant unless the caller sets points_per_wg, so
+ * a scan's sums, and therefore its result, do not depend on the other scans of the
+ * batch nor on how many workgroups the device runs. */
 static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
                         const int64_t* counts, int32_t S, const double* guesses, const ndt_params* p,
                         ndt_result* results) {
@@ -570,66 +575,57 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     int bpc = 0;
     LAUNCH_TRY(align_kernel_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
     if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
-    const int64_t capacity = (int64_t)bpc * c->cus;
-    int64_t total_pts = 0;
-    for (int32_t s = 0; s < S; s++) total_pts += counts[s];
-    int64_t ppw = p->points_per_wg ? p->points_per_wg : 512;
-    /* spread a large batch over every co-resident workgroup */
-    ppw = std::max<int64_t>(ppw, (total_pts + capacity - 1) / capacity);
-    BUF_TRY(c->guesses, sizeof(double) * 16 * S);
-    BUF_TRY(c->states, sizeof(ScanState) * S);
-    BUF_TRY(c->sync, sizeof(GroupSync) * S);
-    BUF_TRY(c->out, sizeof(ScanOut) * S);
-    BUF_TRY(c->desc, sizeof(ScanDesc) * S);
-    HIP_TRY(hipMemcpyAsync(c->guesses.p, guesses, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
+    const int32_t nwg = bpc * c->cus;
+    const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : ALIGN_CHUNK_POINTS;
     std::vector<ScanDesc> desc(S);
-    std::vector<int32_t> wgmap;
-    std::vector<std::pair<int32_t, int32_t>> launches; /* [first scan, end scan) */
-    int64_t used = 0;
-    int32_t first = 0;
+    int64_t slots = 0;
     for (int32_t s = 0; s < S; s++) {
-        int64_t G = std::max<int64_t>(1, (counts[s] + ppw - 1) / ppw);
-        G = std::min<int64_t>(G, std::min<int64_t>(MAX_GROUP, capacity));
-        if (used + G > capacity) {
-            launches.push_back({first, s});
-            first = s;
-            used = 0;
-        }
         desc[s].offset = offsets[s];
         desc[s].n = (int32_t)counts[s];
-        desc[s].first_wg = (int32_t)used;
-        desc[s].group = (int32_t)G;
-        for (int64_t w = 0; w < G; w++) wgmap.push_back(s);
-        used += G;
+        desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
+        desc[s].chunk_base = slots;
+        slots += desc[s].nchunks;
     }
-    launches.push_back({first, S});
-    BUF_TRY(c->wgmap, sizeof(int32_t) * wgmap.size());
-    BUF_TRY(c->partials, sizeof(double) * 32 * std::max<int64_t>(capacity, 1));
+    BUF_TRY(c->guesses, sizeof(double) * 16 * S);
+    BUF_TRY(c->states, sizeof(ScanState) * S);
+    BUF_TRY(c->sync, sizeof(RoundSync) * ((S + MAX_BATCH_SCANS - 1) / MAX_BATCH_SCANS));
+    BUF_TRY(c->out, sizeof(ScanOut) * S);
+    BUF_TRY(c->desc, sizeof(ScanDesc) * S);
+    BUF_TRY(c->partials, sizeof(double) * 32 * slots);
+    BUF_TRY(c->small, 64 * sizeof(double));
+    HIP_TRY(hipMemsetAsync(c->out.p, 0, sizeof(ScanOut) * S, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->guesses.p, guesses, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->wgmap.p, wgmap.data(), sizeof(int32_t) * wgmap.size(), hipMemcpyHostToDevice, c->stream));
-    LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), S, prm, c->states.as<ScanState>(), c->sync.as<GroupSync>(),
-                                  c->stream));
-    HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    int64_t wg_base = 0;
-    for (auto& L : launches) {
-        int32_t total = 0;
-        for (int32_t s = L.first; s < L.second; s++) total += desc[s].group;
-        if (total == 0) continue;
-        LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>(), c->wgmap.as<int32_t>() + wg_base, total, block, prm,
-                                c->states.as<ScanState>(), c->sync.as<GroupSync>(), c->partials.as<double>(),
-                                c->out.as<ScanOut>(), c->stream));
-        wg_base += total;
+    int32_t launches = 0;
+    double prof_sum[4] = {0, 0, 0, 0};
+    int aborted = 0;
+    float ms_total = 0;
+    for (int32_t s0 = 0; s0 < S; s0 += MAX_BATCH_SCANS) {
+        const int32_t n = std::min<int32_t>(MAX_BATCH_SCANS, S - s0);
+        RoundSync* rs = c->sync.as<RoundSync>() + s0 / MAX_BATCH_SCANS;
+        LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0, n, prm, c->states.as<ScanState>() + s0,
+                                      rs, c->stream));
+        HIP_TRY(hipEventRecord(c->ev0, c->stream));
+        LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, (int32_t)chunk, nwg, block, prm,
+                                c->states.as<ScanState>() + s0, rs, c->partials.as<double>(), c->out.as<ScanOut>() + s0,
+                                c->small.as<double>(), c->stream));
+        HIP_TRY(hipEventRecord(c->ev1, c->stream));
+        double prof[5];
+        HIP_TRY(hipMemcpyAsync(prof, c->small.p, sizeof(prof), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        ms_total += ms;
+        for (int k = 0; k < 4; k++) prof_sum[k] += prof[k];
+        if (prof[4] != 0.0) aborted = 1;
+        launches++;
     }
-    HIP_TRY(hipEventRecord(c->ev1, c->stream));
     std::vector<ScanOut> out(S);
     HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    float ms = 0;
-    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    c->last_ms = ms;
-    c->last_launches = (int32_t)launches.size();
+    c->last_ms = ms_total;
+    c->last_launches = launches;
     int64_t pairs = 0;
-    int worst = 0;
     for (int32_t s = 0; s < S; s++) {
         ndt_result& r = results[s];
         std::memset(&r, 0, sizeof(r));
@@ -642,19 +638,17 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         r.iterations = out[s].iterations;
         r.converged = out[s].converged;
         r.passes = out[s].passes;
-        r.status = out[s].status;
+        r.status = aborted && out[s].passes == 0 ? NDT_ERR_TIMEOUT : out[s].status;
+        if (aborted && r.status == 0 && r.iterations == 0 && r.passes == 0) r.status = NDT_ERR_TIMEOUT;
         pairs += (int64_t)out[s].pairs;
-        if (r.status) worst = r.status;
     }
     c->last_pairs = pairs;
-    for (int k = 0; k < 4; k++) c->last_profile[k] = 0.0;
-    for (int32_t s = 0; s < S; s++) {
-        c->last_profile[0] += out[s].t_prologue;
-        c->last_profile[1] += out[s].t_compute;
-        c->last_profile[2] += out[s].t_barrier;
-        c->last_profile[3] += out[s].t_control;
+    for (int k = 0; k < 4; k++) c->last_profile[k] = prof_sum[k];
+    if (aborted) {
+        for (int32_t s = 0; s < S; s++)
+            if (results[s].passes == 0 || out[s].final_m[15] == 0.0) results[s].status = NDT_ERR_TIMEOUT;
+        return fail(NDT_ERR_TIMEOUT, "a device-side wait exceeded its bound");
     }
-    if (worst) return fail(worst, "a scan's device-side wait exceeded its bound");
     return 0;
 }
 

@@ -224,7 +224,7 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
 /* Device view of an ndt-mode grid. */
 struct NdtGridDev {
     const HashSlot* slots;   /* query cell -> candidate list */
-    const float4* cand;      /* candidates: (mean x, y, z, voxel id bits), lists in probe order */
+    const float4* cand;      /* candidates: (mean - query cell corner, voxel id bits), lists in probe order */
     const float4* icov_f;    /* 2 per voxel: (c00 c01 c02 c11) (c12 c22 - -) */
     const double* mean_d;    /* 3 per voxel */
     const double* icov_d;    /* 6 per voxel: c00 c01 c02 c11 c12 c22 */
@@ -275,11 +275,10 @@ constexpr int STATE_WORDS = sizeof(ScanState) / 8;
 static_assert(sizeof(ScanState) % 8 == 0, "state must be whole 64-bit words");
 
 struct ScanDesc {
-    int64_t offset;    /* first point in the packed point buffer */
-    int32_t n;         /* points */
-    int32_t first_wg;  /* first workgroup of this scan's group */
-    int32_t group;     /* workgroups in the group */
-    int32_t pad;
+    int64_t offset;     /* first point in the packed point buffer */
+    int64_t chunk_base; /* first partial-sum slot of this scan */
+    int32_t n;          /* points */
+    int32_t nchunks;    /* chunks of the launch's chunk size */
 };
 
 struct AlignParams {
@@ -292,11 +291,12 @@ struct AlignParams {
     int32_t nscans;
 };
 
-struct GroupSync {
-    uint32_t arrive;
-    uint32_t gen;
+/* grid-barrier state of one cooperative align launch: arrivals so far (grows by
+ * the workgroup count per barrier) and the abort flag of a timed-out wait */
+struct RoundSync {
+    uint32_t count;
     uint32_t abort;
-    uint32_t pad[61]; /* one 256-byte line per scan */
+    uint32_t pad[62];
 };
 
 /* per-scan output written by the device */
@@ -345,20 +345,25 @@ int launch_dilate_emit(const uint64_t* vkey, int64_t nvalid, const int* bits, ui
 int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int64_t nq, int64_t nentries,
                        int shift, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, int log2cap, hipStream_t stream);
-int launch_cand_fill(const uint32_t* sorted_vals, int64_t nentries, const double* mean, float4* cand,
-                     hipStream_t stream);
+int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
+                     const int* bits, const int64_t* origin_cell, double res, float4* cand, hipStream_t stream);
 int launch_hash_clear(HashSlot* slots, int64_t cap, hipStream_t stream);
 int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
                               const uint32_t* seg_start, int64_t nseg, int64_t npts, double ncells,
                               double* mean, double* icov, int32_t* count, uint64_t* vkey, hipStream_t stream);
 
 /* ndt alignment: init states from guesses, then the cooperative persistent kernel */
-int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, GroupSync* sync,
+int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
                        hipStream_t stream);
 int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
-int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, const int32_t* wg_scan,
-                 int32_t total_wg, int block_size, AlignParams prm, ScanState* states, GroupSync* sync,
-                 double* partials, ScanOut* out, hipStream_t stream);
+constexpr int MAX_BATCH_SCANS = 2048;
+/* scan points per partial sum of the align kernel (ndt_params.points_per_wg = 0) */
+constexpr int ALIGN_CHUNK_POINTS = 512;
+/* prof receives 5 doubles: phase microseconds (active list, compute, barriers,
+ * control) of workgroup 0 and an abort flag */
+int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk, int32_t nwg,
+                 int block_size, AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out,
+                 double* prof, hipStream_t stream);
 
 /* single derivative passes (tests and the compat path) */
 /* partials: NRED (ndt) / NRED_COMPAT (compat) doubles per block, the last one the pair count */

@@ -641,18 +641,30 @@ int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, 
     return (int)hipGetLastError();
 }
 
-__global__ void k_cand_fill(const uint32_t* __restrict__ svals, int64_t n, const double* __restrict__ mean,
+/* Candidate entry i of the sorted dilation keys: the voxel's mean relative to the
+ * lower corner of the query cell the entry belongs to, in float32, and the voxel
+ * id.  Relative to the cell the values lie in [-res, 2 res), so float32 keeps them
+ * to about 1e-7 m at any map extent. */
+__global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t n,
+                            const double* __restrict__ mean, Bits3 B, int64_t o0, int64_t o1, int64_t o2, double res,
                             float4* __restrict__ cand) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t v = svals[i];
-    cand[i] = make_float4((float)mean[3 * v], (float)mean[3 * v + 1], (float)mean[3 * v + 2], __uint_as_float(v));
+    const uint32_t v = svals[i];
+    const uint64_t q = skeys[i] >> 5;
+    const int64_t c0 = (int64_t)(q & ((1ull << B.b[0]) - 1)) + o0;
+    const int64_t c1 = (int64_t)((q >> B.b[0]) & ((1ull << B.b[1]) - 1)) + o1;
+    const int64_t c2 = (int64_t)(q >> (B.b[0] + B.b[1])) + o2;
+    cand[i] = make_float4((float)(mean[3 * v] - (double)c0 * res), (float)(mean[3 * v + 1] - (double)c1 * res),
+                          (float)(mean[3 * v + 2] - (double)c2 * res), __uint_as_float(v));
 }
 
-int launch_cand_fill(const uint32_t* sorted_vals, int64_t nentries, const double* mean, float4* cand,
-                     hipStream_t stream) {
-    hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_vals, nentries, mean,
-                       cand);
+int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
+                     const int* bits, const int64_t* origin_cell, double res, float4* cand, hipStream_t stream) {
+    Bits3 B;
+    for (int k = 0; k < 3; k++) B.b[k] = bits[k];
+    hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_keys, sorted_vals,
+                       nentries, mean, B, origin_cell[0], origin_cell[1], origin_cell[2], res, cand);
     return (int)hipGetLastError();
 }
 
@@ -730,8 +742,22 @@ int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, con
  * with small components set to zero; R and t (the transform) stay exact. */
 template <typename T>
 struct PassParams {
-    T R[9], t[3], Ra[9], a4[3], a5[3];
+    double R[9], t[3];         /* the transform, float64 in every precision */
+    T Ra[9], a4[3], a5[3];
     T res, res2, d1, d2, nhd2; /* nhd2 = -d2 / 2 */
+    /* parameter k of the 32-value layout written by pass_params_f64 */
+    __device__ void set(int k, double v) {
+        if (k < 9) R[k] = v;
+        else if (k < 12) t[k - 9] = v;
+        else if (k < 21) Ra[k - 12] = (T)v;
+        else if (k < 24) a4[k - 21] = (T)v;
+        else if (k < 27) a5[k - 24] = (T)v;
+        else if (k == 27) res = (T)v;
+        else if (k == 28) res2 = (T)v;
+        else if (k == 29) d1 = (T)v;
+        else if (k == 30) d2 = (T)v;
+        else nhd2 = (T)v;
+    }
 };
 
 __device__ __attribute__((noinline)) void pass_params_f64(const double* x, const AlignParams& prm, double res,
@@ -767,8 +793,7 @@ template <typename T>
 __device__ inline void make_pass_params(const double* x, const AlignParams& prm, double res, PassParams<T>& pp) {
     double v[32];
     pass_params_f64(x, prm, res, v);
-    T* o = (T*)&pp;
-    for (int k = 0; k < 32; k++) o[k] = (T)v[k];
+    for (int k = 0; k < 32; k++) pp.set(k, v[k]);
 }
 
 __device__ inline float uniform_value(float v) {
@@ -785,10 +810,22 @@ __device__ inline double uniform_value(double v) {
 template <typename T>
 __device__ inline PassParams<T> load_uniform(const PassParams<T>& lds) {
     PassParams<T> u;
-    const T* src = (const T*)&lds;
-    T* dst = (T*)&u;
 #pragma unroll
-    for (int k = 0; k < 32; k++) dst[k] = uniform_value(src[k]);
+    for (int k = 0; k < 9; k++) u.R[k] = uniform_value(lds.R[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) u.t[k] = uniform_value(lds.t[k]);
+#pragma unroll
+    for (int k = 0; k < 9; k++) u.Ra[k] = uniform_value(lds.Ra[k]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        u.a4[k] = uniform_value(lds.a4[k]);
+        u.a5[k] = uniform_value(lds.a5[k]);
+    }
+    u.res = uniform_value(lds.res);
+    u.res2 = uniform_value(lds.res2);
+    u.d1 = uniform_value(lds.d1);
+    u.d2 = uniform_value(lds.d2);
+    u.nhd2 = uniform_value(lds.nhd2);
     return u;
 }
 
@@ -856,29 +893,27 @@ __device__ inline bool hash_find(const HashSlot* __restrict__ slots, int log2cap
 template <typename T, int SUM>
 __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, const PassParams<T>& u, Acc<T, SUM>& acc) {
 #pragma clang fp contract(fast)
-    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
-    const T xt0 = u.R[0] * x0 + u.R[1] * x1 + u.R[2] * x2 + u.t[0];
-    const T xt1 = u.R[3] * x0 + u.R[4] * x1 + u.R[5] * x2 + u.t[1];
-    const T xt2 = u.R[6] * x0 + u.R[7] * x1 + u.R[8] * x2 + u.t[2];
-    if (!(fabs(xt0) < (T)1e15) || !(fabs(xt1) < (T)1e15) || !(fabs(xt2) < (T)1e15)) return 0;
-    int64_t c0, c1, c2;
-    if constexpr (sizeof(T) == 4) {
-        c0 = (int64_t)floorf(xt0 * g.inv_resolution);
-        c1 = (int64_t)floorf(xt1 * g.inv_resolution);
-        c2 = (int64_t)floorf(xt2 * g.inv_resolution);
-    } else {
-        c0 = (int64_t)floor(xt0 / g.resolution_d);
-        c1 = (int64_t)floor(xt1 / g.resolution_d);
-        c2 = (int64_t)floor(xt2 / g.resolution_d);
-    }
-    c0 -= g.origin_cell[0];
-    c1 -= g.origin_cell[1];
-    c2 -= g.origin_cell[2];
+    /* transform and cell in float64 in every precision */
+    const double X0 = p.x, X1 = p.y, X2 = p.z;
+    const double xd0 = u.R[0] * X0 + u.R[1] * X1 + u.R[2] * X2 + u.t[0];
+    const double xd1 = u.R[3] * X0 + u.R[4] * X1 + u.R[5] * X2 + u.t[1];
+    const double xd2 = u.R[6] * X0 + u.R[7] * X1 + u.R[8] * X2 + u.t[2];
+    if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return 0;
+    const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
+    const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
+    const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
+    const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
     if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= (1ll << g.bits[0]) || c1 >= (1ll << g.bits[1]) || c2 >= (1ll << g.bits[2]))
         return 0;
     const uint64_t key = ((uint64_t)c2 << (g.bits[0] + g.bits[1])) | ((uint64_t)c1 << g.bits[0]) | (uint64_t)c0;
     uint32_t begin, count;
     if (!hash_find(g.slots, g.log2cap, key, begin, count)) return 0;
+    /* the point relative to the lower corner of its cell, where the candidate
+     * means are stored: float32 differences of values below 2 resolutions */
+    const float r0 = (float)(xd0 - (double)a0 * g.resolution_d);
+    const float r1 = (float)(xd1 - (double)a1 * g.resolution_d);
+    const float r2 = (float)(xd2 - (double)a2 * g.resolution_d);
+    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
     /* point Jacobian columns J3 = e_x x ya, J4 = a4 x ya, J5 = a5 x ya */
     const T y0 = u.Ra[0] * x0 + u.Ra[1] * x1 + u.Ra[2] * x2;
     const T y1 = u.Ra[3] * x0 + u.Ra[4] * x1 + u.Ra[5] * x2;
@@ -889,13 +924,12 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     /* Candidate filter first: branch-free over the (at most 27) list entries so
      * that their loads are in flight together; the float test carries a margin in
      * fp64 mode, where the exact test below decides. */
-    const float fx = (float)xt0, fy = (float)xt1, fz = (float)xt2;
-    const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.001f + 1e-3f;
+    const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
     uint32_t mask = 0;
 #pragma unroll 4
     for (uint32_t k = 0; k < count; k++) {
         const float4 cd = g.cand[begin + k];
-        const float e0 = fx - cd.x, e1 = fy - cd.y, e2 = fz - cd.z;
+        const float e0 = r0 - cd.x, e1 = r1 - cd.y, e2 = r2 - cd.z;
         mask |= (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
     }
     int pairs = 0;
@@ -906,14 +940,14 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
         const uint32_t v = __float_as_uint(cd.w);
         T d0, d1, d2;
         if constexpr (sizeof(T) == 4) {
-            d0 = xt0 - cd.x;
-            d1 = xt1 - cd.y;
-            d2 = xt2 - cd.z;
+            d0 = r0 - cd.x;
+            d1 = r1 - cd.y;
+            d2 = r2 - cd.z;
             if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
         } else {
-            d0 = xt0 - g.mean_d[3 * v];
-            d1 = xt1 - g.mean_d[3 * v + 1];
-            d2 = xt2 - g.mean_d[3 * v + 2];
+            d0 = xd0 - g.mean_d[3 * v];
+            d1 = xd1 - g.mean_d[3 * v + 1];
+            d2 = xd2 - g.mean_d[3 * v + 2];
             if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) continue;
         }
         T c00, c01, c02, c11, c12, c22;
@@ -1222,7 +1256,7 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
 }
 
 __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, AlignParams prm, ScanState* states,
-                              GroupSync* sync) {
+                              RoundSync* rs) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
     ScanState st;
@@ -1246,27 +1280,36 @@ __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, Ali
     pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
     st.phase = PH_INIT;
     states[s] = st;
-    sync[s].arrive = 0;
-    sync[s].gen = 0;
-    sync[s].abort = 0;
+    if (s == 0) {
+        rs->count = 0;
+        rs->abort = 0;
+    }
 }
 
-int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, GroupSync* sync,
+int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
                        hipStream_t stream) {
-    hipLaunchKernelGGL(k_init_states, dim3(grid_for(S, 64)), dim3(64), 0, stream, guesses, S, prm, states, sync);
+    hipLaunchKernelGGL(k_init_states, dim3(grid_for(S, 64)), dim3(64), 0, stream, guesses, S, prm, states, rs);
     return (int)hipGetLastError();
 }
 
 /* ------------------------------------------------------------------ persistent align kernel
- * One cooperative launch aligns every scan of a batch.  Scan s is processed by a
- * group of `group` consecutive workgroups that split its points evenly.  After
- * each pass every workgroup publishes its 28 partial sums; the workgroup that
- * arrives last reduces them in workgroup order (deterministic), runs the
- * controller on the scan's state and releases the others.  All waits are bounded. */
+ * One cooperative launch aligns every scan of a batch in rounds.  In a round each
+ * still-active scan is evaluated once at its current trial pose: the points of all
+ * active scans are cut into chunks of a fixed size, the co-resident workgroups
+ * take contiguous ranges of chunks, and every chunk leaves its 28 partial sums (and
+ * pair count) in its own slot.  After a grid barrier the scans are spread over the
+ * workgroups: each reduces its chunks' partials in chunk order (deterministic, and
+ * independent of the number of workgroups) and runs the controller, which chooses
+ * the next trial pose or finishes the scan.  A second grid barrier closes the
+ * round.  Scans that finish drop out, so the last slow scans get the whole GPU.
+ * All waits are bounded. */
 
-constexpr uint32_t SPIN_LIMIT = 1u << 24;
-/* waves per SIMD the register allocation must allow: the float64 variant needs
- * about 170 VGPRs without spilling in the point loop, the float32 ones about 120 */
+/* bound of one grid-barrier wait: 2 s of s_memrealtime (100 MHz) */
+constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;
+constexpr int MAX_BATCH = 2048; /* scans per launch (LDS tables) */
+/* waves per SIMD the register allocation must allow: float64 points or float64
+ * sums need up to 256 VGPRs to keep the point loop free of spills, float32 points
+ * with float32 (plain or compensated) sums fit in 168 */
 #ifndef ALIGN_MIN_WAVES_F64
 #define ALIGN_MIN_WAVES_F64 2
 #endif
@@ -1274,127 +1317,193 @@ constexpr uint32_t SPIN_LIMIT = 1u << 24;
 #define ALIGN_MIN_WAVES_F32 3
 #endif
 
+/* Grid-wide barrier of a cooperative launch.
+ *
+ * Every value that one workgroup hands to another (scan states, pass parameters,
+ * partial sums, done flags) is written and read with device-scope relaxed atomics,
+ * which gfx950 issues as sc1 stores and loads: they complete at, and read from,
+ * the device coherence point and never leave a stale copy in an XCD's L2.  The
+ * barrier therefore needs no cache maintenance: each wave waits until its own
+ * stores have completed (s_waitcnt vmcnt(0)), then one lane per workgroup bumps a
+ * monotonically growing arrival counter and polls it until all workgroups of this
+ * barrier have arrived.  A release/acquire pair would add an L2 write-back and, in
+ * every poll, an L2 invalidate that evicts the read-only voxel data the computing
+ * workgroups are using.  `target` is nwg * (barrier number + 1).  The wait is
+ * bounded in time; on expiry the workgroup raises the abort flag, which every
+ * other workgroup sees in its own wait.  Returns false after an abort. */
+__device__ inline bool grid_barrier(RoundSync* rs, uint32_t target, int* s_flag) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = 1;
+        __hip_atomic_fetch_add(&rs->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        uint32_t spins = 0;
+        while (__hip_atomic_load(&rs->count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(8);
+            if ((++spins & 63u) == 0) {
+                if (__hip_atomic_load(&rs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    ok = 0;
+                    break;
+                }
+                if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                    __hip_atomic_store(&rs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (__hip_atomic_load(&rs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        *s_flag = ok;
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+/* exclusive scan over n <= MAX_BATCH values held in LDS (in place); returns the total */
+template <int BLOCK>
+__device__ inline uint32_t lds_exclusive_scan(uint32_t* v, int n, uint32_t* lw) {
+    const int per = (n + BLOCK - 1) / BLOCK;
+    const int b = threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int i = b; i < b + per && i < n; i++) sum += v[i];
+    uint32_t ex;
+    uint32_t total = block_exclusive_scan(sum, &ex, lw);
+    uint32_t run = ex;
+    for (int i = b; i < b + per && i < n; i++) {
+        uint32_t t = v[i];
+        v[i] = run;
+        run += t;
+    }
+    __syncthreads();
+    return total;
+}
+
 template <typename T, int SUM, int BLOCK>
-__global__ __launch_bounds__(BLOCK, sizeof(T) == 8 ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(NdtGridDev g, const float4* __restrict__ pts,
-                                                 const ScanDesc* __restrict__ scans, const int32_t* __restrict__ wg_scan,
-                                                 AlignParams prm, ScanState* states, GroupSync* sync, double* partials,
-                                                 ScanOut* out) {
+__global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(
+    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t S, int32_t chunk,
+    AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out, double* prof_out) {
     __shared__ PassParams<T> pp;
     __shared__ double lds_red[BLOCK / WAVE][NRED];
     __shared__ double red[NRED];
     __shared__ uint64_t sstate[STATE_WORDS];
-    __shared__ int s_done, s_last, s_abort;
+    __shared__ uint32_t s_active[MAX_BATCH];
+    __shared__ uint32_t s_prefix[MAX_BATCH + 1];
+    __shared__ uint32_t lw[BLOCK / WAVE];
+    __shared__ int s_nactive, s_pp_scan, s_flag;
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
-    const int s = wg_scan[wg];
-    const ScanDesc sd = scans[s];
-    const int local = wg - sd.first_wg;
-    const int G = sd.group;
-    const int64_t pb = sd.offset + ((int64_t)sd.n * local) / G;
-    const int64_t pe = sd.offset + ((int64_t)sd.n * (local + 1)) / G;
-    ScanState* st = states + s;
-    GroupSync* gs = sync + s;
-    uint32_t my_gen = 0;
+    const int nwg = gridDim.x;
+    uint32_t target = 0;
     double prof[4] = {0, 0, 0, 0};
+    bool ok = true;
+    if (tid == 0) s_pp_scan = -1;
     for (;;) {
-        uint64_t t0 = __builtin_amdgcn_s_memrealtime();
-        if (tid == 0) s_done = ld_agent_i32(&st->done);
-        if (tid < 32) ((T*)&pp)[tid] = (T)ld_agent_f64(&st->pp[tid]);
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        /* active scans and their chunk counts, identical in every workgroup */
+        for (int i = tid; i < S; i += BLOCK) s_prefix[i] = ld_agent_i32(&states[i].done) ? 0u : 1u;
         __syncthreads();
-        if (s_done) break;
-        uint64_t t1 = __builtin_amdgcn_s_memrealtime();
-        Acc<T, SUM> acc;
-        acc.zero();
-        int pairs = 0;
-        {
-            const PassParams<T> u = load_uniform(pp);
-            for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+        const uint32_t nactive = lds_exclusive_scan<BLOCK>(s_prefix, S, lw);
+        for (int i = tid; i < S; i += BLOCK) {
+            const uint32_t a = s_prefix[i];
+            const uint32_t next = (i + 1 < S) ? s_prefix[i + 1] : nactive;
+            if (next != a) s_active[a] = (uint32_t)i;
         }
-        double v[NRED];
-#pragma unroll
-        for (int k = 0; k < NACC; k++) v[k] = acc.get(k);
-        v[NACC] = (double)pairs;
-        block_reduce_acc<BLOCK>(v, lds_red, red);
-        uint64_t t2 = __builtin_amdgcn_s_memrealtime();
-        uint64_t t3 = t2;
-        if (G > 1) {
-            if (tid < NRED) st_agent_f64(partials + (int64_t)wg * 32 + tid, red[tid]);
-            __syncthreads();
-            if (tid == 0) {
-                __threadfence();
-                uint32_t old = __hip_atomic_fetch_add(&gs->arrive, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-                s_last = (old == (uint32_t)(G - 1));
+        __syncthreads();
+        if (nactive == 0) break;
+        for (int a = tid; a < (int)nactive; a += BLOCK) s_prefix[a] = (uint32_t)scans[s_active[a]].nchunks;
+        __syncthreads();
+        const uint32_t nchunks = lds_exclusive_scan<BLOCK>(s_prefix, (int)nactive, lw);
+        if (tid == 0) s_prefix[nactive] = nchunks;
+        __syncthreads();
+        const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+        /* this workgroup's contiguous range of chunks */
+        const uint32_t c0 = (uint32_t)(((uint64_t)nchunks * wg) / nwg);
+        const uint32_t c1 = (uint32_t)(((uint64_t)nchunks * (wg + 1)) / nwg);
+        uint32_t a = 0;
+        for (uint32_t c = c0; c < c1; c++) {
+            /* active index of chunk c: last a with prefix[a] <= c */
+            uint32_t lo = a, hi = nactive;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if (s_prefix[mid] <= c) lo = mid;
+                else hi = mid;
             }
-            __syncthreads();
-        } else {
-            if (tid == 0) s_last = 1;
-            __syncthreads();
+            a = lo;
+            const int s = (int)s_active[a];
+            const ScanDesc sd = scans[s];
+            const int64_t k = c - s_prefix[a];
+            if (s != s_pp_scan) {
+                __syncthreads();
+                if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
+                if (tid == 0) s_pp_scan = s;
+                __syncthreads();
+            }
+            Acc<T, SUM> acc;
+            acc.zero();
+            int pairs = 0;
+            {
+                const PassParams<T> u = load_uniform(pp);
+                const int64_t pb = sd.offset + k * chunk;
+                const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * (int64_t)chunk);
+                for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+            }
+            double v[NRED];
+#pragma unroll
+            for (int q = 0; q < NACC; q++) v[q] = acc.get(q);
+            v[NACC] = (double)pairs;
+            block_reduce_acc<BLOCK>(v, lds_red, red);
+            if (tid < NRED) st_agent_f64(partials + (sd.chunk_base + k) * 32 + tid, red[tid]);
         }
-        if (s_last) {
-            if (tid < STATE_WORDS) sstate[tid] = ld_agent_u64((const uint64_t*)st + tid);
-            for (int k = tid + BLOCK; k < STATE_WORDS; k += BLOCK) sstate[k] = ld_agent_u64((const uint64_t*)st + k);
-            if (G > 1 && tid < NRED) {
+        const uint64_t t2 = __builtin_amdgcn_s_memrealtime();
+        target += (uint32_t)nwg;
+        ok = grid_barrier(rs, target, &s_flag);
+        if (!ok) break;
+        const uint64_t t3 = __builtin_amdgcn_s_memrealtime();
+        /* reductions and controllers, scans spread over the workgroups */
+        for (uint32_t ai = (uint32_t)wg; ai < nactive; ai += (uint32_t)nwg) {
+            const int s = (int)s_active[ai];
+            const ScanDesc sd = scans[s];
+            for (int q = tid; q < STATE_WORDS; q += BLOCK) sstate[q] = ld_agent_u64((const uint64_t*)(states + s) + q);
+            if (tid < NRED) {
                 double acc2 = 0.0;
-                for (int w = 0; w < G; w++) acc2 += ld_agent_f64(partials + (int64_t)(sd.first_wg + w) * 32 + tid);
+                for (int q = 0; q < sd.nchunks; q++) acc2 += ld_agent_f64(partials + (sd.chunk_base + q) * 32 + tid);
                 red[tid] = acc2;
             }
             __syncthreads();
-            t3 = __builtin_amdgcn_s_memrealtime();
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
             __syncthreads();
-            for (int k = tid; k < STATE_WORDS; k += BLOCK) st_agent_u64((uint64_t*)st + k, sstate[k]);
+            for (int q = tid; q < STATE_WORDS; q += BLOCK) st_agent_u64((uint64_t*)(states + s) + q, sstate[q]);
             __syncthreads();
-            if (tid == 0 && G > 1) {
-                __hip_atomic_store(&gs->arrive, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __threadfence();
-                __hip_atomic_fetch_add(&gs->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        } else {
-            if (tid == 0) {
-                uint32_t spins = 0;
-                int ab = 0;
-                while (__hip_atomic_load(&gs->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == my_gen) {
-                    __builtin_amdgcn_s_sleep(2);
-                    if (++spins > SPIN_LIMIT ||
-                        __hip_atomic_load(&gs->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                        ab = 1;
-                        break;
-                    }
-                }
-                s_abort = ab;
-                if (ab) {
-                    __hip_atomic_store(&gs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    out[s].status = NDT_ERR_TIMEOUT;
-                }
-            }
-            __syncthreads();
-            if (s_abort) break;
         }
-        my_gen++;
-        __syncthreads();
-        if (local == 0 && tid == 0) {
-            uint64_t t4 = __builtin_amdgcn_s_memrealtime();
+        if (tid == 0) s_pp_scan = -1;
+        const uint64_t t4 = __builtin_amdgcn_s_memrealtime();
+        target += (uint32_t)nwg;
+        ok = grid_barrier(rs, target, &s_flag);
+        if (!ok) break;
+        const uint64_t t5 = __builtin_amdgcn_s_memrealtime();
+        if (wg == 0 && tid == 0) {
             prof[0] += (double)(t1 - t0);
             prof[1] += (double)(t2 - t1);
-            prof[2] += (double)(t4 - t2) - (double)(s_last ? (t4 - t3) : 0);
-            prof[3] += (double)(s_last ? (t4 - t3) : 0);
+            prof[2] += (double)(t3 - t2) + (double)(t5 - t4);
+            prof[3] += (double)(t4 - t3);
         }
     }
-    if (local == 0 && tid == 0) {
+    if (wg == 0 && tid == 0) {
         /* s_memrealtime counts at 100 MHz */
-        out[s].t_prologue = prof[0] * 0.01;
-        out[s].t_compute = prof[1] * 0.01;
-        out[s].t_barrier = prof[2] * 0.01;
-        out[s].t_control = prof[3] * 0.01;
+        for (int q = 0; q < 4; q++) prof_out[q] = prof[q] * 0.01;
+        prof_out[4] = ok ? 0.0 : 1.0;
     }
 }
 
 template <typename T, int SUM, int BLOCK>
-static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, const int32_t* wg_scan,
-                          int32_t total_wg, AlignParams prm, ScanState* states, GroupSync* sync, double* partials,
-                          ScanOut* out, hipStream_t stream) {
-    void* args[] = {&g, &pts, &scans, &wg_scan, &prm, &states, &sync, &partials, &out};
-    return (int)hipLaunchCooperativeKernel((const void*)k_align<T, SUM, BLOCK>, dim3(total_wg), dim3(BLOCK), args, 0,
+static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk,
+                          int32_t nwg, AlignParams prm, ScanState* states, RoundSync* rs, double* partials,
+                          ScanOut* out, double* prof, hipStream_t stream) {
+    void* args[] = {&g, &pts, &scans, &S, &chunk, &prm, &states, &rs, &partials, &out, &prof};
+    return (int)hipLaunchCooperativeKernel((const void*)k_align<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
                                            stream);
 }
 
@@ -1431,11 +1540,12 @@ int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* b
 #undef NDT_MB
 }
 
-int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, const int32_t* wg_scan, int32_t total_wg,
-                 int block_size, AlignParams prm, ScanState* states, GroupSync* sync, double* partials, ScanOut* out,
-                 hipStream_t stream) {
-#define NDT_LA(T, S, B) \
-    launch_align_t<T, S, B>(grid, pts, scans, wg_scan, total_wg, prm, states, sync, partials, out, stream)
+int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk, int32_t nwg,
+                 int block_size, AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out,
+                 double* prof, hipStream_t stream) {
+    if (S > MAX_BATCH) return (int)hipErrorInvalidValue;
+#define NDT_LA(T, S_, B) \
+    launch_align_t<T, S_, B>(grid, pts, scans, S, chunk, nwg, prm, states, rs, partials, out, prof, stream)
     NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
 #undef NDT_LA
 }
@@ -1443,7 +1553,7 @@ int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, cons
 /* ------------------------------------------------------------------ single derivative pass (ndt) */
 
 template <typename T, int SUM>
-__global__ __launch_bounds__(256, sizeof(T) == 8 ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_deriv_ndt(NdtGridDev g, const float4* __restrict__ pts, int64_t n,
+__global__ __launch_bounds__(256, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_deriv_ndt(NdtGridDev g, const float4* __restrict__ pts, int64_t n,
                                                    const double* __restrict__ pose, AlignParams prm,
                                                    double* __restrict__ partials) {
     __shared__ PassParams<T> pp;

@@ -173,10 +173,10 @@ int ndt_derivatives(ndt_ctx* ctx, const ndt_grid* grid, const float* scan_xyzw, 
  * events on the context's stream, and the number of kernel launches. */
 int ndt_ctx_last_timing(ndt_ctx* ctx, double* align_kernel_ms, int32_t* launches, int64_t* pairs);
 
-/* Phase profile of the last ndt alignment on this context, in microseconds summed
- * over scans, as seen by the first workgroup of each scan: pass-parameter load,
- * point compute + workgroup reduction, waiting for the scan's other workgroups,
- * and the Newton / line-search controller. */
+/* Phase profile of the last ndt alignment on this context, in microseconds as seen
+ * by the first workgroup of the launch: building the list of active scans, point
+ * compute + workgroup reductions, grid barriers, and the per-scan reductions and
+ * Newton / line-search controllers. */
 int ndt_ctx_last_profile(ndt_ctx* ctx, double phases_us[4]);
 
 #ifdef __cplusplus
