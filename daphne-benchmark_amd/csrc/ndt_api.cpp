@@ -18,6 +18,7 @@
 
 #include "ndt_hip.h"
 #include "ndt_internal.h"
+#include "ndt_host.h"
 
 using namespace ndt;
 
@@ -25,98 +26,13 @@ using namespace ndt;
 
 static thread_local std::string g_last_error;
 
-static int fail(int code, const std::string& msg) {
+int ndt::fail(int code, const std::string& msg) {
     g_last_error = msg;
     return code;
 }
 
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t e_ = (hipError_t)(expr);                                                        \
-        if (e_ != hipSuccess)                                                                      \
-            return fail(e_ == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,                   \
-                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
-    } while (0)
-
-#define LAUNCH_TRY(expr)                                                                           \
-    do {                                                                                           \
-        int e_ = (expr);                                                                           \
-        if (e_ != 0)                                                                               \
-            return fail(NDT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString((hipError_t)e_)); \
-    } while (0)
-
 extern "C" const char* ndt_last_error(void) { return g_last_error.c_str(); }
 extern "C" const char* ndt_version(void) { return "ndt-mi355x 0.1 (gfx950)"; }
-
-/* ------------------------------------------------------------------ device buffers */
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    int ensure(size_t want) {
-        if (want <= bytes && p) return 0;
-        release();
-        size_t b = want ? want : 1;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) {
-            p = nullptr;
-            return fail(e == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,
-                        std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
-        }
-        bytes = b;
-        return 0;
-    }
-    template <class T> T* as() const { return (T*)p; }
-};
-
-#define BUF_TRY(buf, bytes)                \
-    do {                                   \
-        int e_ = (buf).ensure(bytes);      \
-        if (e_) return e_;                 \
-    } while (0)
-
-struct ndt_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int cus = 0;
-    char name[256] = {0};
-    /* workspaces */
-    DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2;
-    double last_ms = 0.0;
-    int32_t last_launches = 0;
-    int64_t last_pairs = 0;
-    double last_profile[4] = {0, 0, 0, 0};
-};
-
-struct ndt_grid {
-    ndt_ctx* ctx = nullptr;
-    ndt_params params;
-    ndt_grid_info info;
-    /* ndt */
-    NdtGridDev dev;
-    DevBuf slots, cand, icov_f, mean_c, icov_c;
-    /* all occupied voxels (diagnostics) */
-    DevBuf all_mean, all_icov, all_cov, all_count, all_valid, all_key;
-    /* compat */
-    CompatGridDev cdev;
-    int bits[3] = {0, 0, 0};
-    int64_t origin_cell[3] = {0, 0, 0};
-};
-
-struct ndt_scans {
-    ndt_ctx* ctx = nullptr;
-    int32_t S = 0;
-    std::vector<int64_t> counts, offsets;
-    int64_t total = 0;
-    DevBuf pts;
-};
 
 /* ------------------------------------------------------------------ params */
 
@@ -768,6 +684,23 @@ extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* s
     if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
     HIP_TRY(hipSetDevice(c->device));
     return align_packed(c, g, sc->pts.as<float4>(), sc->offsets.data(), sc->counts.data(), sc->S, guesses, p, results);
+}
+
+/* Aligns the scans idx[0..n) of a device-resident batch (ndt mode). */
+int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const int32_t* idx, int32_t n,
+                           const double* guesses, const ndt_params* p, ndt_result* results) {
+    if (!c || !g || !sc || !idx || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    std::vector<int64_t> off((size_t)n), cnt((size_t)n);
+    for (int32_t k = 0; k < n; k++) {
+        if (idx[k] < 0 || idx[k] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
+        off[(size_t)k] = sc->offsets[(size_t)idx[k]];
+        cnt[(size_t)k] = sc->counts[(size_t)idx[k]];
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results);
 }
 
 /* ------------------------------------------------------------------ single derivative pass */

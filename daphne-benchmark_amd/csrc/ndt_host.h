@@ -1,0 +1,120 @@
+/*
+ * ndt_host.h - host-side helpers shared by the C ABI translation units
+ * (ndt_api.cpp, ndt_learn.cpp): error recording, device buffers and the opaque
+ * handle structures.
+ */
+#ifndef NDT_HOST_H
+#define NDT_HOST_H
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "ndt_hip.h"
+#include "ndt_internal.h"
+
+namespace ndt {
+
+/* ------------------------------------------------------------------ errors */
+
+/* records the message of a failure for ndt_last_error() and returns code */
+int fail(int code, const std::string& msg);
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t e_ = (hipError_t)(expr);                                                        \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(e_ == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,                   \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+#define LAUNCH_TRY(expr)                                                                           \
+    do {                                                                                           \
+        int e_ = (expr);                                                                           \
+        if (e_ != 0)                                                                               \
+            return fail(NDT_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString((hipError_t)e_)); \
+    } while (0)
+
+/* ------------------------------------------------------------------ device buffers */
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int ensure(size_t want) {
+        if (want <= bytes && p) return 0;
+        release();
+        size_t b = want ? want : 1;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return fail(e == hipErrorOutOfMemory ? NDT_ERR_NOMEM : NDT_ERR_HIP,
+                        std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
+        }
+        bytes = b;
+        return 0;
+    }
+    template <class T> T* as() const { return (T*)p; }
+};
+
+#define BUF_TRY(buf, bytes)                \
+    do {                                   \
+        int e_ = (buf).ensure(bytes);      \
+        if (e_) return e_;                 \
+    } while (0)
+
+} // namespace ndt
+
+using ndt::DevBuf;
+
+namespace ndt {
+/* aligns the scans idx[0..n) of a device-resident batch from guesses[0..n) (ndt mode) */
+int align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const int32_t* idx, int32_t n,
+                      const double* guesses, const ndt_params* p, ndt_result* results);
+} // namespace ndt
+
+/* opaque handles of the C ABI */
+struct ndt_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0;
+    char name[256] = {0};
+    /* workspaces */
+    DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2;
+    double last_ms = 0.0;
+    int32_t last_launches = 0;
+    int64_t last_pairs = 0;
+    double last_profile[4] = {0, 0, 0, 0};
+};
+
+struct ndt_grid {
+    ndt_ctx* ctx = nullptr;
+    ndt_params params;
+    ndt_grid_info info;
+    /* ndt */
+    ndt::NdtGridDev dev;
+    DevBuf slots, cand, icov_f, mean_c, icov_c;
+    /* all occupied voxels (diagnostics) */
+    DevBuf all_mean, all_icov, all_cov, all_count, all_valid, all_key;
+    /* compat */
+    ndt::CompatGridDev cdev;
+    int bits[3] = {0, 0, 0};
+    int64_t origin_cell[3] = {0, 0, 0};
+};
+
+struct ndt_scans {
+    ndt_ctx* ctx = nullptr;
+    int32_t S = 0;
+    std::vector<int64_t> counts, offsets;
+    int64_t total = 0;
+    DevBuf pts;
+};
+
+#endif

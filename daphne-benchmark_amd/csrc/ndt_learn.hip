@@ -1,0 +1,487 @@
+/*
+ * ndt_learn.hip - gfx950 kernels of the learned warm start.
+ *
+ *  features   bird's-eye-view occupancy histograms in LDS: the scan in its sensor
+ *             frame, the map around a guess in the guess frame (map points
+ *             bucketed by 4 m tiles), log1p counts through a bf16 lookup table
+ *  model      bf16 GEMM C = A B^T on the matrix cores (v_mfma_f32_16x16x32_bf16,
+ *             128 x 128 tiles, 64-deep K steps double-buffered through LDS,
+ *             split-K with float32 partials), bias + ReLU epilogues that also
+ *             write the transposed activations the backward GEMMs read, the
+ *             6-wide output layer, its loss and backward, Adam
+ *
+ * All reductions run in a fixed order, so training is deterministic.  There is no
+ * counterpart in the reference: the guess it refines enters at
+ * src/CPU/ndt_mapping/kernel.cpp:1187-1206.
+ */
+#include <hip/hip_runtime.h>
+
+#include "ndt_learn_internal.h"
+
+namespace ndt {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+static inline int blocks_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+/* float32 -> bfloat16, round to nearest even (NaN stays NaN) */
+__device__ inline uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+/* ------------------------------------------------------------------ features */
+
+/* BEV cell of a point given in the grid's own frame, or -1 outside the grid
+ * (NaN coordinates fall outside). */
+__device__ inline int bev_index(double qx, double qy, double qz) {
+    const double fx = floor((qx - BEV_X0) * 2.0);
+    const double fy = floor((qy - BEV_Y0) * 2.0);
+    const double fz = floor((qz - BEV_Z0) / BEV_DZ);
+    if (!(fx >= 0.0 && fx < (double)BEV_NX && fy >= 0.0 && fy < (double)BEV_NY && fz >= 0.0 && fz < (double)BEV_NZ))
+        return -1;
+    return ((int)fz * BEV_NY + (int)fy) * BEV_NX + (int)fx;
+}
+
+constexpr int BEV_BLOCK = 512;
+constexpr int BEV_HALF = BEV_CELLS / 2; /* cells counted per pass (z planes 0-3, then 4-7) */
+
+__device__ inline void bev_clear(uint32_t* hist) {
+    for (int i = threadIdx.x; i < BEV_HALF; i += BEV_BLOCK) hist[i] = 0u;
+}
+__device__ inline void bev_emit(const uint32_t* hist, int half, const uint16_t* lut, uint16_t* out) {
+    for (int i = threadIdx.x; i < BEV_HALF; i += BEV_BLOCK) {
+        const uint32_t c = hist[i];
+        out[half * BEV_HALF + i] = lut[c > 65535u ? 65535u : c];
+    }
+}
+
+/* One workgroup per scan: histogram of its points (sensor frame), two passes of
+ * four z planes each so that exact 32-bit counters fit in 64 KiB of LDS. */
+__global__ __launch_bounds__(BEV_BLOCK) void k_bev_scan(const float4* __restrict__ pts,
+                                                        const int64_t* __restrict__ offsets,
+                                                        const int32_t* __restrict__ counts,
+                                                        const uint16_t* __restrict__ lut, uint16_t* __restrict__ out) {
+    __shared__ uint32_t hist[BEV_HALF];
+    const int s = blockIdx.x;
+    const int64_t off = offsets[s];
+    const int n = counts[s];
+    uint16_t* o = out + (int64_t)s * BEV_CELLS;
+    for (int half = 0; half < 2; half++) {
+        bev_clear(hist);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += BEV_BLOCK) {
+            const float4 p = pts[off + i];
+            const int c = bev_index((double)p.x, (double)p.y, (double)p.z) - half * BEV_HALF;
+            if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
+        }
+        __syncthreads();
+        bev_emit(hist, half, lut, o);
+        __syncthreads();
+    }
+}
+
+int launch_bev_scan(const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S, const uint16_t* lut,
+                    uint16_t* out, hipStream_t stream) {
+    if (S <= 0) return 0;
+    hipLaunchKernelGGL(k_bev_scan, dim3(S), dim3(BEV_BLOCK), 0, stream, pts, offsets, counts, lut, out);
+    return (int)hipGetLastError();
+}
+
+/* One workgroup per guess: histogram of the map points around the guess, in the
+ * guess frame q = R^T (p - t).  Only the tiles under the grid's footprint are
+ * visited: the footprint's corners at both z limits, mapped to the map frame,
+ * bound the tile rows and columns. */
+__global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const double* __restrict__ guesses,
+                                                        const uint16_t* __restrict__ lut, uint16_t* __restrict__ X,
+                                                        int64_t ldx, int64_t col_off) {
+    __shared__ uint32_t hist[BEV_HALF];
+    __shared__ int s_rng[4];
+    const int b = blockIdx.x;
+    const double* G = guesses + 16 * (int64_t)b;
+    const double r00 = G[0], r01 = G[1], r02 = G[2], t0 = G[3];
+    const double r10 = G[4], r11 = G[5], r12 = G[6], t1 = G[7];
+    const double r20 = G[8], r21 = G[9], r22 = G[10], t2 = G[11];
+    if (threadIdx.x == 0) {
+        double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+        for (int k = 0; k < 8; k++) {
+            const double qx = (k & 1) ? -BEV_X0 : BEV_X0;
+            const double qy = (k & 2) ? -BEV_Y0 : BEV_Y0;
+            const double qz = (k & 4) ? BEV_Z0 + BEV_NZ * BEV_DZ : BEV_Z0;
+            const double px = r00 * qx + r01 * qy + r02 * qz + t0;
+            const double py = r10 * qx + r11 * qy + r12 * qz + t1;
+            xmin = fmin(xmin, px);
+            xmax = fmax(xmax, px);
+            ymin = fmin(ymin, py);
+            ymax = fmax(ymax, py);
+        }
+        /* tile range, clamped to the tile grid; empty when the footprint misses it */
+        double fx0 = floor((xmin - 0.5 - tg.x0) / MAP_TILE), fx1 = floor((xmax + 0.5 - tg.x0) / MAP_TILE);
+        double fy0 = floor((ymin - 0.5 - tg.y0) / MAP_TILE), fy1 = floor((ymax + 0.5 - tg.y0) / MAP_TILE);
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        if (fx1 >= 0.0 && fx0 < (double)tg.ntx && fy1 >= 0.0 && fy0 < (double)tg.nty) {
+            tx0 = (int)fmax(fx0, 0.0);
+            tx1 = (int)fmin(fx1, (double)(tg.ntx - 1));
+            ty0 = (int)fmax(fy0, 0.0);
+            ty1 = (int)fmin(fy1, (double)(tg.nty - 1));
+        }
+        s_rng[0] = tx0;
+        s_rng[1] = tx1;
+        s_rng[2] = ty0;
+        s_rng[3] = ty1;
+    }
+    __syncthreads();
+    const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
+    uint16_t* o = X + (int64_t)b * ldx + col_off;
+    for (int half = 0; half < 2; half++) {
+        bev_clear(hist);
+        __syncthreads();
+        for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
+            const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
+            const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
+            for (uint32_t i = pb + threadIdx.x; i < pe; i += BEV_BLOCK) {
+                const float4 p = tg.pts[i];
+                const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+                const double qx = r00 * dx + r10 * dy + r20 * dz;
+                const double qy = r01 * dx + r11 * dy + r21 * dz;
+                const double qz = r02 * dx + r12 * dy + r22 * dz;
+                const int c = bev_index(qx, qy, qz) - half * BEV_HALF;
+                if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
+            }
+        }
+        __syncthreads();
+        bev_emit(hist, half, lut, o);
+        __syncthreads();
+    }
+}
+
+int launch_bev_crop(TileGridDev tg, const double* guesses, int32_t B, const uint16_t* lut, uint16_t* X, int64_t ldx,
+                    int64_t col_off, hipStream_t stream) {
+    if (B <= 0) return 0;
+    hipLaunchKernelGGL(k_bev_crop, dim3(B), dim3(BEV_BLOCK), 0, stream, tg, guesses, lut, X, ldx, col_off);
+    return (int)hipGetLastError();
+}
+
+/* dst row b = src row idx[b] (ncols u16 values, ncols a multiple of 8) */
+__global__ void k_gather_rows(const uint16_t* __restrict__ src, const int32_t* __restrict__ idx, int64_t ncols,
+                              uint16_t* __restrict__ dst, int64_t ldd) {
+    const int b = blockIdx.y;
+    const int64_t r = idx[b];
+    const uint4* s = (const uint4*)(src + r * ncols);
+    uint4* d = (uint4*)(dst + (int64_t)b * ldd);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < ncols / 8; i += (int64_t)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+int launch_gather_rows(const uint16_t* src, const int32_t* idx, int32_t B, int64_t ncols, uint16_t* dst, int64_t ldd,
+                       hipStream_t stream) {
+    if (B <= 0) return 0;
+    hipLaunchKernelGGL(k_gather_rows, dim3(8, B), dim3(256), 0, stream, src, idx, ncols, dst, ldd);
+    return (int)hipGetLastError();
+}
+
+/* dst[c][r] = src[r][c]; rows and cols multiples of 64 */
+__global__ __launch_bounds__(256) void k_transpose_u16(const uint16_t* __restrict__ src, int64_t rows, int64_t cols,
+                                                       uint16_t* __restrict__ dst) {
+    __shared__ uint16_t t[64][66];
+    const int64_t r0 = (int64_t)blockIdx.y * 64, c0 = (int64_t)blockIdx.x * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int k = ty; k < 64; k += 4) t[k][tx] = src[(r0 + k) * cols + c0 + tx];
+    __syncthreads();
+    for (int k = ty; k < 64; k += 4) dst[(c0 + k) * rows + r0 + tx] = t[tx][k];
+}
+
+int launch_transpose_u16(const uint16_t* src, int64_t rows, int64_t cols, uint16_t* dst, hipStream_t stream) {
+    if (rows % 64 || cols % 64) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_transpose_u16, dim3((unsigned)(cols / 64), (unsigned)(rows / 64)), dim3(256), 0, stream, src,
+                       rows, cols, dst);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ GEMM
+ * C_z[M x N] = A[M x K_z] * B[N x K_z]^T with A and B bf16, row-major in K (lda,
+ * ldb in elements), float32 accumulation; split z covers K range
+ * [z K/splits, (z+1) K/splits) and writes its own M x N float32 slice of C.
+ *
+ * 256 threads = 4 waves in a 2 x 2 arrangement, each wave owns a 64 x 64 block
+ * of C as 4 x 4 tiles of v_mfma_f32_16x16x32_bf16.  For that instruction lane l
+ * supplies row (l mod 16) of A and column (l mod 16) of B over the 8 K values
+ * 8 (l / 16) .. 8 (l / 16) + 7 of the 32-deep step, and receives
+ * D[4 (l / 16) + r][l mod 16], r = 0..3.  A and B take their K values from the
+ * same positions, so the product is independent of how the hardware orders the
+ * K values inside one lane's 8.  K steps of 64 are staged through two LDS
+ * buffers: the next step's global loads are in flight while the current one is
+ * multiplied.  Rows are padded by 8 elements (16 bytes). */
+constexpr int GPAD = 8;
+
+__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A, int64_t lda,
+                                                 const uint16_t* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                 int64_t ldc, int64_t split_stride, int kps) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][GEMM_BM][GEMM_BK + GPAD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][GEMM_BN][GEMM_BK + GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int64_t m0 = (int64_t)blockIdx.y * GEMM_BM, n0 = (int64_t)blockIdx.x * GEMM_BN;
+    const int64_t k0 = (int64_t)blockIdx.z * kps;
+    const int nk = kps / GEMM_BK;
+    const uint16_t* Ab = A + m0 * lda + k0;
+    const uint16_t* Bb = B + n0 * ldb + k0;
+    uint4 ra[4], rb[4];
+    /* tile chunk c = tid + 256 q (q = 0..3): row c / 8, columns 8 (c mod 8) .. +7 */
+    const int lr = tid >> 3, lc = (tid & 7) * 8;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        ra[q] = *(const uint4*)(Ab + (int64_t)(lr + 32 * q) * lda + lc);
+        rb[q] = *(const uint4*)(Bb + (int64_t)(lr + 32 * q) * ldb + lc);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        *(uint4*)&As[0][lr + 32 * q][lc] = ra[q];
+        *(uint4*)&Bs[0][lr + 32 * q][lc] = rb[q];
+    }
+    __syncthreads();
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        /* next K step (the last iteration reloads its own step: in bounds, unused) */
+        const int64_t ko = (int64_t)(kt + 1 < nk ? kt + 1 : kt) * GEMM_BK;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            ra[q] = *(const uint4*)(Ab + (int64_t)(lr + 32 * q) * lda + ko + lc);
+            rb[q] = *(const uint4*)(Bb + (int64_t)(lr + 32 * q) * ldb + ko + lc);
+        }
+#pragma unroll
+        for (int kk = 0; kk < GEMM_BK; kk += 32) {
+            bf16x8 a[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = *(const bf16x8*)&As[buf][wm * 64 + i * 16 + fr][kk + fk];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = *(const bf16x8*)&Bs[buf][wn * 64 + j * 16 + fr][kk + fk];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        /* the other buffer was last read in the previous iteration, before its barrier */
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            *(uint4*)&As[buf ^ 1][lr + 32 * q][lc] = ra[q];
+            *(uint4*)&Bs[buf ^ 1][lr + 32 * q][lc] = rb[q];
+        }
+        __syncthreads();
+    }
+    float* Cz = C + (int64_t)blockIdx.z * split_stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+                const int64_t col = n0 + wn * 64 + j * 16 + fr;
+                Cz[row * ldc + col] = acc[i][j][r];
+            }
+}
+
+int launch_gemm_nt(const uint16_t* A, int64_t lda, const uint16_t* B, int64_t ldb, float* C, int64_t ldc, int M, int N,
+                   int K, int splits, hipStream_t stream) {
+    if (M <= 0 || N <= 0 || K <= 0 || splits <= 0 || M % GEMM_BM || N % GEMM_BN || K % (GEMM_BK * splits) ||
+        lda % 8 || ldb % 8 || lda < K || ldb < K || ldc < N)
+        return (int)hipErrorInvalidValue;
+    const int kps = K / splits;
+    hipLaunchKernelGGL(k_gemm_nt, dim3(N / GEMM_BN, M / GEMM_BM, splits), dim3(256), 0, stream, A, lda, B, ldb, C, ldc,
+                       (int64_t)M * ldc, kps);
+    return (int)hipGetLastError();
+}
+
+/* out[m][n] = act(sum_z P_z[m][n] + bias[n]) in bf16, and optionally its transpose */
+__global__ void k_bias_act(const float* __restrict__ P, int splits, int M, int N, const float* __restrict__ bias,
+                           int relu, uint16_t* __restrict__ out, uint16_t* __restrict__ outT) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)M * N) return;
+    const int m = (int)(i / N), n = (int)(i % N);
+    float v = 0.f;
+    for (int z = 0; z < splits; z++) v += P[(int64_t)z * M * N + i];
+    v += bias[n];
+    if (relu && !(v > 0.f)) v = 0.f;
+    const uint16_t h = f2bf(v);
+    out[i] = h;
+    if (outT) outT[(int64_t)n * M + m] = h;
+}
+
+int launch_bias_act(const float* P, int splits, int M, int N, const float* bias, int relu, uint16_t* out,
+                    uint16_t* outT, hipStream_t stream) {
+    const int64_t n = (int64_t)M * N;
+    hipLaunchKernelGGL(k_bias_act, dim3(blocks_for(n, 256)), dim3(256), 0, stream, P, splits, M, N, bias, relu, out,
+                       outT);
+    return (int)hipGetLastError();
+}
+
+/* Output layer, one wave per sample: Y = H2 W3^T + b3.  With a target, also the
+ * weighted squared error of the sample and dL/dY for the batch mean
+ * L = (1/B) sum_b sum_o w_o (Y_bo - T_bo)^2. */
+__global__ __launch_bounds__(256) void k_head_fwd(const uint16_t* __restrict__ H2, const uint16_t* __restrict__ W3,
+                                                  const float* __restrict__ b3, int B, int F2, float* __restrict__ Y,
+                                                  const float* __restrict__ target, const float* __restrict__ wts,
+                                                  float* __restrict__ dY, float* __restrict__ loss_b) {
+    const int b = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (b >= B) return;
+    const uint16_t* h = H2 + (int64_t)b * F2;
+    float y[NOUT];
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) {
+        float s = 0.f;
+        for (int k = lane; k < F2; k += 64) s += bf2f(h[k]) * bf2f(W3[(int64_t)o * F2 + k]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        y[o] = s + b3[o];
+    }
+    if (lane == 0) {
+        float l = 0.f;
+        for (int o = 0; o < NOUT; o++) {
+            Y[(int64_t)b * NOUT + o] = y[o];
+            if (target) {
+                const float d = y[o] - target[(int64_t)b * NOUT + o];
+                l += wts[o] * d * d;
+                dY[(int64_t)b * NOUT + o] = 2.f * wts[o] * d / (float)B;
+            }
+        }
+        if (target) loss_b[b] = l;
+    }
+}
+
+int launch_head_fwd(const uint16_t* H2, const uint16_t* W3, const float* b3, int B, int F2, float* Y,
+                    const float* target, const float* weights, float* dY, float* loss_b, hipStream_t stream) {
+    hipLaunchKernelGGL(k_head_fwd, dim3(blocks_for(B, 4)), dim3(256), 0, stream, H2, W3, b3, B, F2, Y, target, weights,
+                       dY, loss_b);
+    return (int)hipGetLastError();
+}
+
+/* Backward of the output layer and the second ReLU, one thread per hidden unit n
+ * of layer 2, samples in order: dW3[o][n] = sum_b dY[b][o] H2[b][n],
+ * dZ2[b][n] = [H2[b][n] > 0] sum_o dY[b][o] W3[o][n] (bf16, and transposed),
+ * db2[n] = sum_b dZ2[b][n]; thread 0 also forms db3[o] = sum_b dY[b][o]. */
+__global__ void k_head_bwd(const float* __restrict__ dY, const uint16_t* __restrict__ H2,
+                           const uint16_t* __restrict__ W3, int B, int F2, float* __restrict__ dW3,
+                           float* __restrict__ db3, uint16_t* __restrict__ dZ2, uint16_t* __restrict__ dZ2T,
+                           float* __restrict__ db2) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= F2) return;
+    float w3[NOUT], gw[NOUT];
+    for (int o = 0; o < NOUT; o++) {
+        w3[o] = bf2f(W3[(int64_t)o * F2 + n]);
+        gw[o] = 0.f;
+    }
+    float gb = 0.f;
+    for (int b = 0; b < B; b++) {
+        const float h = bf2f(H2[(int64_t)b * F2 + n]);
+        float dz = 0.f;
+        for (int o = 0; o < NOUT; o++) {
+            const float d = dY[(int64_t)b * NOUT + o];
+            dz += d * w3[o];
+            gw[o] += d * h;
+        }
+        if (!(h > 0.f)) dz = 0.f;
+        const uint16_t zb = f2bf(dz);
+        dZ2[(int64_t)b * F2 + n] = zb;
+        dZ2T[(int64_t)n * B + b] = zb;
+        gb += dz;
+    }
+    for (int o = 0; o < NOUT; o++) dW3[(int64_t)o * F2 + n] = gw[o];
+    db2[n] = gb;
+    if (n == 0) {
+        for (int o = 0; o < NOUT; o++) {
+            float s = 0.f;
+            for (int b = 0; b < B; b++) s += dY[(int64_t)b * NOUT + o];
+            db3[o] = s;
+        }
+    }
+}
+
+int launch_head_bwd(const float* dY, const uint16_t* H2, const uint16_t* W3, int B, int F2, float* dW3, float* db3,
+                    uint16_t* dZ2, uint16_t* dZ2T, float* db2, hipStream_t stream) {
+    hipLaunchKernelGGL(k_head_bwd, dim3(blocks_for(F2, 64)), dim3(64), 0, stream, dY, H2, W3, B, F2, dW3, db3, dZ2, dZ2T,
+                       db2);
+    return (int)hipGetLastError();
+}
+
+/* Backward of a ReLU layer, one thread per unit n, samples in order:
+ * dZ[b][n] = [H[b][n] > 0] dH[b][n], written transposed in bf16; db[n] = sum_b dZ. */
+__global__ void k_relu_bwd(const float* __restrict__ dH, const uint16_t* __restrict__ H, int B, int F,
+                           uint16_t* __restrict__ dZT, float* __restrict__ db) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= F) return;
+    float gb = 0.f;
+    for (int b = 0; b < B; b++) {
+        const float h = bf2f(H[(int64_t)b * F + n]);
+        const float dz = (h > 0.f) ? dH[(int64_t)b * F + n] : 0.f;
+        dZT[(int64_t)n * B + b] = f2bf(dz);
+        gb += dz;
+    }
+    db[n] = gb;
+}
+
+int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, hipStream_t stream) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(F, 64)), dim3(64), 0, stream, dH, H, B, F, dZT, db);
+    return (int)hipGetLastError();
+}
+
+/* Adam (Kingma & Ba 2015) on float32 master weights; bc1 = 1 - beta1^t and
+ * bc2 = 1 - beta2^t; wb (optional) receives the bf16 copy used by the GEMMs. */
+__global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
+                       int64_t n, float lr, float beta1, float beta2, float eps, float bc1, float bc2,
+                       uint16_t* __restrict__ wb) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = beta1 * m[i] + (1.f - beta1) * gi;
+    const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+int launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, float beta1, float beta2, float eps,
+                float bc1, float bc2, uint16_t* wb, hipStream_t stream) {
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, stream, w, m, v, g, n, lr, beta1, beta2, eps, bc1,
+                       bc2, wb);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_to_bf16(const float* __restrict__ x, int64_t n, uint16_t* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f2bf(x[i]);
+}
+
+int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n, 256)), dim3(256), 0, stream, x, n, out);
+    return (int)hipGetLastError();
+}
+
+/* out[0] += x[0] + ... + x[n-1], in index order (one thread) */
+__global__ void k_sum_ordered(const float* __restrict__ x, int n, float* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    float s = 0.f;
+    for (int i = 0; i < n; i++) s += x[i];
+    out[0] += s;
+}
+
+int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(64), 0, stream, x, n, out);
+    return (int)hipGetLastError();
+}
+
+} // namespace ndt

@@ -197,6 +197,37 @@ def make_workload(name: str, seed: int = 1, scans: int | None = None, map_points
                     dict(seed=seed, **w))
 
 
+def make_scans(scene: SceneCfg, map_xyzw: np.ndarray, sizes, seeds, perturb_xyz=1.0, perturb_yaw_deg=3.0,
+               perturb_rollpitch_deg=0.5, threads: int | None = None):
+    """Scans of the given sizes and seeds, drawn on host threads (the generator
+    call releases the GIL).  Returns (scans, gt (S,4,4), guesses (S,4,4))."""
+    from concurrent.futures import ThreadPoolExecutor
+    threads = threads or min(16, os.cpu_count() or 1)
+
+    def one(k):
+        cfg = scan_cfg(int(sizes[k]), perturb_xyz, perturb_yaw_deg, perturb_rollpitch_deg)
+        return make_scan(scene, map_xyzw, cfg, int(seeds[k]))
+
+    with ThreadPoolExecutor(threads) as ex:
+        out = list(ex.map(one, range(len(sizes))))
+    return [o[0] for o in out], np.stack([o[1] for o in out]), np.stack([o[2] for o in out])
+
+
+def perturb_poses(gt, rng, xy=3.0, z=0.3, yaw_deg=10.0, rollpitch_deg=0.5):
+    """Guesses around ground-truth poses: world translation U[-xy, xy] in x and y,
+    U[-z, z] in z, yaw U[-yaw, yaw] added to the pose's yaw, roll and pitch
+    U[-rollpitch, rollpitch] (the generator's convention, R = Rx Ry Rz)."""
+    gt = np.asarray(gt, np.float64).reshape(-1, 4, 4)
+    out = np.empty_like(gt)
+    d = np.pi / 180.0
+    for i, G in enumerate(gt):
+        yaw = np.arctan2(-G[0, 1], G[0, 0])
+        e = rng.uniform(-1, 1, 6)
+        out[i] = pose(G[0, 3] + xy * e[0], G[1, 3] + xy * e[1], G[2, 3] + z * e[2], rollpitch_deg * d * e[3],
+                      rollpitch_deg * d * e[4], yaw + yaw_deg * d * e[5])
+    return out
+
+
 # ---------------------------------------------------------------------------
 # DAPHNE data files (src/CPU/ndt_mapping/kernel.cpp:143-217, :178-190)
 # ---------------------------------------------------------------------------
@@ -337,6 +368,11 @@ class NdtGridInfo(C.Structure):
                 ("max_corner", C.c_float * 3), ("build_ms", C.c_double)]
 
 
+class TrainParams(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("weights", C.c_float * 6)]
+
+
 class NdtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"ndt error {code}: {msg}")
@@ -374,6 +410,28 @@ def hip_lib():
         L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.ndt_ctx_last_profile.argtypes = [P, P]
+        # learned warm start (ndt_learn.h)
+        L.ndt_bevmap_build.argtypes = [P, P, C.c_int64, P, C.POINTER(P)]
+        L.ndt_bevmap_destroy.argtypes = [P]
+        L.ndt_features.argtypes = [P, P, P, P, P, C.c_int32, P]
+        L.ndt_model_create.argtypes = [P, C.c_int32, C.c_int32, C.c_int32, C.c_uint64, C.POINTER(P)]
+        L.ndt_model_destroy.argtypes = [P]
+        L.ndt_model_dims.argtypes = [P, P]
+        L.ndt_model_save.argtypes = [P, C.c_char_p]
+        L.ndt_model_load.argtypes = [P, C.c_char_p, C.POINTER(P)]
+        L.ndt_model_get.argtypes = [P, C.c_int32, P, C.c_int64]
+        L.ndt_model_set.argtypes = [P, C.c_int32, P, C.c_int64]
+        L.ndt_model_forward.argtypes = [P, P, P, C.c_int32, P]
+        L.ndt_model_backward.argtypes = [P, P, P, P, P, C.c_int32, C.POINTER(C.c_float)]
+        L.ndt_model_grad.argtypes = [P, C.c_int32, P, C.c_int64]
+        L.ndt_model_adam_step.argtypes = [P, P, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.ndt_train_params_default.argtypes = [C.POINTER(TrainParams)]
+        L.ndt_train_params_default.restype = None
+        L.ndt_dataset_create.argtypes = [P, P, P, P, P, P, C.c_int64, C.POINTER(P)]
+        L.ndt_dataset_destroy.argtypes = [P]
+        L.ndt_model_train.argtypes = [P, P, P, C.POINTER(TrainParams), P]
+        L.ndt_model_predict.argtypes = [P, P, P, P, P, P]
+        L.ndt_align_scans_learned.argtypes = [P, P, P, P, P, P, C.POINTER(NdtParams), P, P]
         _hip_lib = L
     return _hip_lib
 
@@ -383,6 +441,11 @@ EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_
                     "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
                     "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing",
                     "ndt_ctx_last_profile")
+LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_model_create", "ndt_model_destroy",
+                 "ndt_model_dims", "ndt_model_save", "ndt_model_load", "ndt_model_get", "ndt_model_set",
+                 "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
+                 "ndt_train_params_default", "ndt_dataset_create", "ndt_dataset_destroy", "ndt_model_train",
+                 "ndt_model_predict", "ndt_align_scans_learned")
 
 
 def _check(rc):
@@ -563,3 +626,154 @@ def partial_points_callback(ctx: Context, input_cloud, init_guess, target_cloud)
     """The reference's per-testcase callback (kernel.cpp:1405-1414), on the GPU."""
     r = ctx.compat_callback(input_cloud, init_guess, target_cloud)
     return CallbackResult(r["converged"], r["final_f32"], 0.0)
+
+
+# ---------------------------------------------------------------- learned warm start
+
+BEV_CELLS = 32768
+FEATURES = 65536
+W1, B1, W2, B2, W3, B3 = range(6)
+CORR_SCALE = np.array([3.0, 3.0, 1.0, 0.05, 0.05, 0.2])
+
+
+def train_params(**kw) -> TrainParams:
+    tp = TrainParams()
+    hip_lib().ndt_train_params_default(C.byref(tp))
+    for k, v in kw.items():
+        if k == "weights":
+            for i in range(6):
+                tp.weights[i] = v[i]
+        else:
+            setattr(tp, k, v)
+    return tp
+
+
+class _Handle:
+    _destroy = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(hip_lib(), self._destroy)(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BevMap(_Handle):
+    """Map points bucketed in 4 m tiles for the map grids (origin: world offset)."""
+    _destroy = "ndt_bevmap_destroy"
+
+    def __init__(self, ctx: Context, map_xyzw, origin=(0.0, 0.0, 0.0)):
+        self.ctx = ctx
+        m = np.ascontiguousarray(map_xyzw, dtype=np.float32)
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        self.h = C.c_void_p()
+        _check(hip_lib().ndt_bevmap_build(ctx.h, m.ctypes.data, m.shape[0], o.ctypes.data, C.byref(self.h)))
+
+    def features(self, scans: "Scans", scan_index, guesses) -> np.ndarray:
+        """(B, 65536) uint16 bf16 bit patterns: [scan grid | map grid at the guess]."""
+        idx = np.ascontiguousarray(scan_index, dtype=np.int32)
+        B = idx.shape[0]
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(B, 16)
+        out = np.zeros((B, FEATURES), np.uint16)
+        _check(hip_lib().ndt_features(self.ctx.h, self.h, scans.h, idx.ctypes.data, g.ctypes.data, B,
+                                      out.ctypes.data))
+        return out
+
+
+class Model(_Handle):
+    """bf16 MLP f0 -> f1 -> f2 -> 6 on the GPU (weights float32 masters)."""
+    _destroy = "ndt_model_destroy"
+
+    def __init__(self, ctx: Context, f0=FEATURES, f1=1024, f2=256, seed=1, path=None):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        if path is not None:
+            _check(hip_lib().ndt_model_load(ctx.h, os.fsencode(path), C.byref(self.h)))
+        else:
+            _check(hip_lib().ndt_model_create(ctx.h, f0, f1, f2, seed, C.byref(self.h)))
+        d = np.zeros(3, np.int32)
+        _check(hip_lib().ndt_model_dims(self.h, d.ctypes.data))
+        self.dims = tuple(int(v) for v in d)
+
+    def shape(self, which):
+        f0, f1, f2 = self.dims
+        return [(f1, f0), (f1,), (f2, f1), (f2,), (6, f2), (6,)][which]
+
+    def get(self, which) -> np.ndarray:
+        out = np.zeros(self.shape(which), np.float32)
+        _check(hip_lib().ndt_model_get(self.h, which, out.ctypes.data, out.size))
+        return out
+
+    def set(self, which, value):
+        v = np.ascontiguousarray(value, dtype=np.float32).reshape(self.shape(which))
+        _check(hip_lib().ndt_model_set(self.h, which, v.ctypes.data, v.size))
+
+    def grad(self, which) -> np.ndarray:
+        out = np.zeros(self.shape(which), np.float32)
+        _check(hip_lib().ndt_model_grad(self.h, which, out.ctypes.data, out.size))
+        return out
+
+    def save(self, path):
+        _check(hip_lib().ndt_model_save(self.h, os.fsencode(path)))
+
+    def forward(self, x_bf16) -> np.ndarray:
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        B = x.shape[0]
+        y = np.zeros((B, 6), np.float32)
+        _check(hip_lib().ndt_model_forward(self.ctx.h, self.h, x.ctypes.data, B, y.ctypes.data))
+        return y
+
+    def backward(self, x_bf16, target, weights=None) -> float:
+        x = np.ascontiguousarray(x_bf16, dtype=np.uint16)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        loss = C.c_float()
+        _check(hip_lib().ndt_model_backward(self.ctx.h, self.h, x.ctypes.data, t.ctypes.data,
+                                            None if w is None else w.ctypes.data, x.shape[0], C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        _check(hip_lib().ndt_model_adam_step(self.ctx.h, self.h, lr, beta1, beta2, eps))
+
+    def train(self, dataset: "Dataset", tp: TrainParams | None = None) -> np.ndarray:
+        tp = tp or train_params()
+        losses = np.zeros(tp.epochs, np.float32)
+        _check(hip_lib().ndt_model_train(self.ctx.h, self.h, dataset.h, C.byref(tp), losses.ctypes.data))
+        return losses
+
+    def predict(self, bevmap: BevMap, scans: "Scans", guesses) -> np.ndarray:
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(scans.S, 16)
+        out = np.zeros_like(g)
+        _check(hip_lib().ndt_model_predict(self.ctx.h, self.h, bevmap.h, scans.h, g.ctypes.data, out.ctypes.data))
+        return out.reshape(scans.S, 4, 4)
+
+    def align(self, grid: Grid, bevmap: BevMap, scans: "Scans", guesses, prm: NdtParams | None = None):
+        """Batched NDT from learned guesses with fallback; returns (results, used_learned)."""
+        prm = prm or grid.params
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(scans.S, 16)
+        out = (NdtResult * scans.S)()
+        used = np.zeros(scans.S, np.int32)
+        _check(hip_lib().ndt_align_scans_learned(self.ctx.h, grid.h, bevmap.h, self.h, scans.h, g.ctypes.data,
+                                                 C.byref(prm), out, used.ctypes.data))
+        return [_result(out[i]) for i in range(scans.S)], used
+
+
+class Dataset(_Handle):
+    """Training triples (scan, guess, target pose) resident on the device."""
+    _destroy = "ndt_dataset_destroy"
+
+    def __init__(self, ctx: Context, bevmap: BevMap, scans: "Scans", scan_index, guesses, targets):
+        self.ctx = ctx
+        idx = np.ascontiguousarray(scan_index, dtype=np.int32)
+        n = idx.shape[0]
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(n, 16)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(n, 16)
+        self.n = n
+        self.h = C.c_void_p()
+        _check(hip_lib().ndt_dataset_create(ctx.h, bevmap.h, scans.h, idx.ctypes.data, g.ctypes.data, t.ctypes.data,
+                                            n, C.byref(self.h)))

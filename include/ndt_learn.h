@@ -1,0 +1,119 @@
+/*
+ * ndt_learn.h - C ABI of the learned warm start of the NDT engine (libndthip.so).
+ *
+ * A small learned pose proposer refines the initial guess before NDT runs.  The
+ * reference has no counterpart: it aligns from the guess it is given
+ * (src/CPU/ndt_mapping/kernel.cpp:1187-1206, where init_guess becomes the start
+ * pose of computeTransformation).  NDT stays the judge: a learned start whose
+ * final NDT score is worse than the score at the given guess, or that does not
+ * converge, is replaced by plain NDT from the given guess.
+ *
+ * Features: two bird's-eye-view occupancy grids of 64 x 64 x 8 cells
+ * (0.5 m x 0.5 m x 1.5 m; x, y in [-16, 16) m, z in [-2.5, 9.5) m): the scan in its
+ * sensor frame and the map points around the guess, expressed in the guess frame.
+ * Cells hold log1p(count) in bfloat16 (counts saturate at 65535); feature index =
+ * grid * 32768 + (iz * 64 + iy) * 64 + ix, grid 0 = scan, grid 1 = map.
+ *
+ * Model: bf16 MLP f0 -> f1 -> f2 -> 6 with ReLU after both hidden layers and
+ * float32 accumulation (default 65536 -> 1024 -> 256 -> 6), trained with Adam on
+ * the GPU.  It predicts the pose correction C = guess^-1 * target as
+ * (x, y, z, roll, pitch, yaw) (R = Rx Ry Rz), each divided by a fixed scale
+ * (3 m, 3 m, 1 m, 0.05 rad, 0.05 rad, 0.2 rad); the refined guess is guess * C.
+ *
+ * Poses are row-major 4x4 float64 in world coordinates (map-local + the origin
+ * given to ndt_bevmap_build), as for ndt_align_scans.
+ */
+#ifndef NDT_LEARN_H
+#define NDT_LEARN_H
+
+#include <stdint.h>
+
+#include "ndt_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ndt_bevmap ndt_bevmap;   /* map points bucketed for BEV crops */
+typedef struct ndt_model ndt_model;     /* MLP weights, bf16 copies, Adam state */
+typedef struct ndt_dataset ndt_dataset; /* training triples resident on the device */
+
+enum { NDT_BEV_CELLS = 32768, NDT_FEATURES = 65536, NDT_MODEL_OUT = 6 };
+
+/* Parameter tensors of a model, float32, row-major:
+ * W1 [f1 x f0], b1 [f1], W2 [f2 x f1], b2 [f2], W3 [6 x f2], b3 [6]. */
+enum { NDT_W1 = 0, NDT_B1 = 1, NDT_W2 = 2, NDT_B2 = 3, NDT_W3 = 4, NDT_B3 = 5 };
+
+typedef struct ndt_train_params {
+    int32_t epochs;        /* passes over the dataset */
+    int32_t batch;         /* samples per step, a multiple of 128 */
+    float lr;              /* Adam step size */
+    float beta1, beta2, eps;
+    uint64_t seed;         /* sample order */
+    float weights[6];      /* loss weight per output component */
+} ndt_train_params;
+
+/* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m
+ * tiles for the map grids; origin = world position of the map-local frame. */
+int ndt_bevmap_build(ndt_ctx* ctx, const float* map_xyzw, int64_t m, const double origin[3], ndt_bevmap** out);
+int ndt_bevmap_destroy(ndt_bevmap* bm);
+
+/* Features of B (scan, guess) pairs into out (B x 65536 bf16 bit patterns, host):
+ * row b = [BEV of scans[scan_index[b]] | map BEV around guesses[b]]. */
+int ndt_features(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
+                 const double* guesses, int32_t B, uint16_t* out);
+
+/* Creates a model with He-normal hidden weights (seeded), small output weights
+ * and zero biases.  f0, f1, f2 must be multiples of 128; features need f0 = 65536. */
+int ndt_model_create(ndt_ctx* ctx, int32_t f0, int32_t f1, int32_t f2, uint64_t seed, ndt_model** out);
+int ndt_model_destroy(ndt_model* model);
+int ndt_model_dims(const ndt_model* model, int32_t dims[3]);
+/* Float32 weights to / from a file (Adam state is not stored). */
+int ndt_model_save(const ndt_model* model, const char* path);
+int ndt_model_load(ndt_ctx* ctx, const char* path, ndt_model** out);
+/* Copies parameter tensor `which` (NDT_W1 ... NDT_B3) to / from host float32. */
+int ndt_model_get(const ndt_model* model, int32_t which, float* out, int64_t n);
+int ndt_model_set(ndt_model* model, int32_t which, const float* in, int64_t n);
+
+/* Forward pass on host features (B x f0 bf16 bit patterns, B a multiple of 128):
+ * y (B x 6) receives the scaled outputs. */
+int ndt_model_forward(ndt_ctx* ctx, ndt_model* model, const uint16_t* x, int32_t B, float* y);
+/* Forward and backward of the loss L = (1/B) sum_b sum_o w_o (y_bo - t_bo)^2 on host
+ * features; stores the gradients (read them with ndt_model_grad) and the loss. */
+int ndt_model_backward(ndt_ctx* ctx, ndt_model* model, const uint16_t* x, const float* target,
+                       const float* weights, int32_t B, float* loss);
+int ndt_model_grad(const ndt_model* model, int32_t which, float* out, int64_t n);
+/* One Adam step with the stored gradients. */
+int ndt_model_adam_step(ndt_ctx* ctx, ndt_model* model, float lr, float beta1, float beta2, float eps);
+
+void ndt_train_params_default(ndt_train_params* tp);
+
+/* Training triples: scan scan_index[i] of `scans`, the guess guesses[i] and the
+ * target pose targets[i] (an NDT-converged pose), n samples.  Scan grids are
+ * computed once and kept on the device with the guesses and scaled corrections. */
+int ndt_dataset_create(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
+                       const double* guesses, const double* targets, int64_t n, ndt_dataset** out);
+int ndt_dataset_destroy(ndt_dataset* ds);
+
+/* Trains on the dataset; epoch_loss (optional, `epochs` values) receives the mean
+ * loss of each epoch.  Drops the last partial batch of an epoch. */
+int ndt_model_train(ndt_ctx* ctx, ndt_model* model, const ndt_dataset* ds, const ndt_train_params* tp,
+                    float* epoch_loss);
+
+/* Refined guesses of S scans: out_guesses[s] = guesses[s] * C(model output). */
+int ndt_model_predict(ndt_ctx* ctx, ndt_model* model, const ndt_bevmap* bm, const ndt_scans* scans,
+                      const double* guesses, double* out_guesses);
+
+/* Batched NDT from learned guesses with the fallback described above.
+ * used_learned[s] (optional) = 1 where the learned start was kept.  Timing of
+ * the whole call, device milliseconds of feature + model + NDT kernels, is
+ * available through ndt_ctx_last_timing. */
+int ndt_align_scans_learned(ndt_ctx* ctx, const ndt_grid* grid, const ndt_bevmap* bm, ndt_model* model,
+                            const ndt_scans* scans, const double* guesses, const ndt_params* params,
+                            ndt_result* results, int32_t* used_learned);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

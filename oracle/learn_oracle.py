@@ -1,0 +1,183 @@
+"""CPU oracle of the learned warm start (numpy).  TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py may import this module, as the
+checker of the GPU engine; the product never calls it.
+
+The reference has no learned component: its NDT starts from the guess it is given
+(src/CPU/ndt_mapping/kernel.cpp:1187-1206).  This module restates the engine's own
+definitions (include/ndt_learn.h):
+
+  bev_index / scan_grid / map_grid    the two 64 x 64 x 8 occupancy grids, log1p
+                                      counts in bf16 (bit patterns)
+  correction_params / apply_correction   the pose correction the model predicts
+  mlp_forward / mlp_backward          the bf16 MLP and its gradients, float64
+                                      arithmetic with the engine's bf16 roundings
+                                      (or none, for finite-difference checks)
+  adam_step                           Adam (Kingma & Ba 2015)
+
+Pins (tests/test_learn_oracle.py): bf16 rounding against torch's bfloat16
+conversion; grids against a per-axis linear search on small inputs and against
+the invariance scan grid(G^-1 map) = map grid(G); gradients against central
+finite differences in float64; the correction against its round trip.  No
+function is left unpinned.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+NX, NY, NZ = 64, 64, 8
+CELLS = NX * NY * NZ
+X0, Y0, Z0, DZ = -16.0, -16.0, -2.5, 1.5
+CORR_SCALE = np.array([3.0, 3.0, 1.0, 0.05, 0.05, 0.2])
+
+
+# ---------------------------------------------------------------- bf16
+
+def bf16_bits(x) -> np.ndarray:
+    """float32 -> bf16 bit patterns, round to nearest even, NaN kept NaN."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    nan = (u & 0x7FFFFFFF) > 0x7F800000
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF
+    r = np.where(nan, ((u >> 16) | 0x40) & 0xFFFF, r)
+    return r.astype(np.uint16)
+
+
+def bf16_value(bits) -> np.ndarray:
+    return (np.asarray(bits, dtype=np.uint16).astype(np.uint32) << 16).view(np.float32)
+
+
+def bf16_round(x) -> np.ndarray:
+    return bf16_value(bf16_bits(x))
+
+
+def log1p_lut() -> np.ndarray:
+    c = np.arange(65536, dtype=np.float64)
+    return bf16_bits(np.log1p(c).astype(np.float32))
+
+
+# ---------------------------------------------------------------- grids
+
+def bev_index(qx, qy, qz) -> np.ndarray:
+    """Cell of each point in the grid frame (float64 arithmetic), -1 outside."""
+    fx = np.floor((np.asarray(qx, np.float64) - X0) * 2.0)
+    fy = np.floor((np.asarray(qy, np.float64) - Y0) * 2.0)
+    fz = np.floor((np.asarray(qz, np.float64) - Z0) / DZ)
+    with np.errstate(invalid="ignore"):
+        ok = (fx >= 0) & (fx < NX) & (fy >= 0) & (fy < NY) & (fz >= 0) & (fz < NZ)
+    idx = np.full(fx.shape, -1, np.int64)
+    idx[ok] = ((fz[ok].astype(np.int64) * NY + fy[ok].astype(np.int64)) * NX + fx[ok].astype(np.int64))
+    return idx
+
+
+def _grid_from_index(idx, lut) -> np.ndarray:
+    counts = np.bincount(idx[idx >= 0], minlength=CELLS)
+    return lut[np.minimum(counts, 65535)]
+
+
+def scan_grid(scan_xyzw, lut=None) -> np.ndarray:
+    """Grid of a scan in its sensor frame: (32768,) bf16 bits."""
+    lut = log1p_lut() if lut is None else lut
+    s = np.asarray(scan_xyzw, np.float32)
+    return _grid_from_index(bev_index(s[:, 0], s[:, 1], s[:, 2]), lut)
+
+
+def map_grid(map_xyzw, guess_local, lut=None) -> np.ndarray:
+    """Grid of the map points around a guess (map-local 4x4), in the guess frame
+    q = R^T (p - t), evaluated term by term as the kernel does."""
+    lut = log1p_lut() if lut is None else lut
+    m = np.asarray(map_xyzw, np.float32)
+    G = np.asarray(guess_local, np.float64).reshape(4, 4)
+    dx = m[:, 0].astype(np.float64) - G[0, 3]
+    dy = m[:, 1].astype(np.float64) - G[1, 3]
+    dz = m[:, 2].astype(np.float64) - G[2, 3]
+    qx = G[0, 0] * dx + G[1, 0] * dy + G[2, 0] * dz
+    qy = G[0, 1] * dx + G[1, 1] * dy + G[2, 1] * dz
+    qz = G[0, 2] * dx + G[1, 2] * dy + G[2, 2] * dz
+    return _grid_from_index(bev_index(qx, qy, qz), lut)
+
+
+# ---------------------------------------------------------------- pose correction
+
+def pose_matrix(p) -> np.ndarray:
+    """R = Rx(roll) Ry(pitch) Rz(yaw), t = p[0:3]."""
+    cx, sx = np.cos(p[3]), np.sin(p[3])
+    cy, sy = np.cos(p[4]), np.sin(p[4])
+    cz, sz = np.cos(p[5]), np.sin(p[5])
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    M = np.eye(4)
+    M[:3, :3] = Rx @ Ry @ Rz
+    M[:3, 3] = p[:3]
+    return M
+
+
+def correction_params(G, T) -> np.ndarray:
+    """(x, y, z, roll, pitch, yaw) of C = G^-1 T."""
+    G = np.asarray(G, np.float64).reshape(4, 4)
+    T = np.asarray(T, np.float64).reshape(4, 4)
+    C = np.linalg.inv(G) @ T
+    R = C[:3, :3]
+    roll = np.arctan2(-R[1, 2], R[2, 2])
+    pitch = np.arcsin(np.clip(R[0, 2], -1.0, 1.0))
+    yaw = np.arctan2(-R[0, 1], R[0, 0])
+    return np.array([C[0, 3], C[1, 3], C[2, 3], roll, pitch, yaw])
+
+
+def apply_correction(G, p6) -> np.ndarray:
+    return np.asarray(G, np.float64).reshape(4, 4) @ pose_matrix(np.asarray(p6, np.float64))
+
+
+# ---------------------------------------------------------------- MLP
+
+def _rnd(x, on):
+    return bf16_round(np.asarray(x, np.float32)).astype(np.float64) if on else np.asarray(x, np.float64)
+
+
+def mlp_forward(x, P, bf16=True) -> dict:
+    """x: (B, f0) float64 (or bf16 values); P: dict W1 b1 W2 b2 W3 b3 (float).
+    With bf16, weight matrices and hidden activations are rounded as the engine
+    rounds them; accumulation is exact (float64)."""
+    X = np.asarray(x, np.float64)
+    W1, W2, W3 = (_rnd(P[k], bf16) for k in ("W1", "W2", "W3"))
+    Z1 = X @ W1.T + np.asarray(P["b1"], np.float64)
+    H1 = _rnd(np.maximum(Z1, 0.0), bf16)
+    Z2 = H1 @ W2.T + np.asarray(P["b2"], np.float64)
+    H2 = _rnd(np.maximum(Z2, 0.0), bf16)
+    Y = H2 @ W3.T + np.asarray(P["b3"], np.float64)
+    return dict(X=X, W1=W1, W2=W2, W3=W3, Z1=Z1, H1=H1, Z2=Z2, H2=H2, Y=Y)
+
+
+def mlp_loss(Y, T, w=None) -> float:
+    w = np.ones(Y.shape[1]) if w is None else np.asarray(w, np.float64)
+    return float(np.sum(w * (Y - T) ** 2) / Y.shape[0])
+
+
+def mlp_backward(x, P, T, w=None, bf16=True) -> tuple:
+    """Loss L = (1/B) sum_b sum_o w_o (Y - T)^2 and its gradients.  With bf16, the
+    backpropagated signals dZ2 and dZ1 are rounded before the products that use
+    them, as the engine stores them in bf16; the bias gradients use the unrounded
+    values."""
+    f = mlp_forward(x, P, bf16)
+    B = f["X"].shape[0]
+    T = np.asarray(T, np.float64)
+    w = np.ones(f["Y"].shape[1]) if w is None else np.asarray(w, np.float64)
+    dY = 2.0 * w * (f["Y"] - T) / B
+    g = {"W3": dY.T @ f["H2"], "b3": dY.sum(0)}
+    dZ2 = (dY @ f["W3"]) * (f["H2"] > 0)
+    g["b2"] = dZ2.sum(0)
+    dZ2r = _rnd(dZ2, bf16)
+    g["W2"] = dZ2r.T @ f["H1"]
+    dZ1 = (dZ2r @ f["W2"]) * (f["H1"] > 0)
+    g["b1"] = dZ1.sum(0)
+    g["W1"] = _rnd(dZ1, bf16).T @ f["X"]
+    return mlp_loss(f["Y"], T, w), g, f
+
+
+def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+    """One Adam update at step t (1-based); returns new (w, m, v)."""
+    m = beta1 * m + (1 - beta1) * g
+    v = beta2 * v + (1 - beta2) * g * g
+    mh = m / (1 - beta1 ** t)
+    vh = v / (1 - beta2 ** t)
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v

@@ -29,6 +29,14 @@ def test_hip_library_exports_header(A):
     assert set(A.EXPORTED_SYMBOLS) <= names
 
 
+def test_learn_entry_points_exported(A):
+    names = declared("ndt_learn.h")
+    assert len(names) >= 18
+    missing = names - exported(A.HIP_LIB_PATH)
+    assert not missing, missing
+    assert set(A.LEARN_SYMBOLS) == names
+
+
 def test_scene_library_exports_header(A):
     A.scene_lib()
     names = declared("ndt_scene.h")
