@@ -71,7 +71,9 @@ static int check_params(const ndt_params* p) {
     if (p->point_precision < 0 || p->point_precision > 1) return fail(NDT_ERR_INVALID, "bad point_precision");
     if (p->block_size != 0 && p->block_size != 128 && p->block_size != 256 && p->block_size != 512)
         return fail(NDT_ERR_INVALID, "block_size must be 0, 128, 256 or 512");
-    if (p->points_per_wg < 0) return fail(NDT_ERR_INVALID, "points_per_wg must be >= 0");
+    if (p->points_per_wg < 0 || p->points_per_wg > (1 << 24))
+        return fail(NDT_ERR_INVALID, "points_per_wg must be in 0 .. 2^24");
+    if (p->scheduler < NDT_SCHED_DEFAULT || p->scheduler > NDT_SCHED_QUEUE) return fail(NDT_ERR_INVALID, "bad scheduler");
     return 0;
 }
 
@@ -239,6 +241,24 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
                                       g->all_mean.as<double>(), g->all_icov.as<double>(), g->all_cov.as<double>(),
                                       g->all_count.as<int32_t>(), g->all_valid.as<uint32_t>(), g->all_key.as<uint64_t>(),
                                       c->stream));
+    /* map points sorted by voxel and a hash of the occupied voxels, for nearest-point
+     * queries (fitness score) */
+    BUF_TRY(g->fit_pts, sizeof(float4) * (size_t)std::max<int64_t>(nvalid_pts, 1));
+    LAUNCH_TRY(launch_gather_points(c->pts.as<float4>(), vals.as<uint32_t>(), nvalid_pts, g->fit_pts.as<float4>(),
+                                    c->stream));
+    const int flog2 = std::max(4, ceil_log2(2 * nvox));
+    BUF_TRY(g->fit_slots, sizeof(HashSlot) << flog2);
+    LAUNCH_TRY(launch_hash_clear(g->fit_slots.as<HashSlot>(), 1ll << flog2, c->stream));
+    LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nvox, nvalid_pts, 0,
+                                  g->fit_slots.as<HashSlot>(), flog2, c->stream));
+    g->fdev.slots = g->fit_slots.as<HashSlot>();
+    g->fdev.pts = g->fit_pts.as<float4>();
+    g->fdev.log2cap = flog2;
+    for (int k = 0; k < 3; k++) {
+        g->fdev.bits[k] = g->bits[k];
+        g->fdev.origin_cell[k] = g->origin_cell[k];
+    }
+    g->fdev.resolution = res;
     /* compact the valid voxels */
     DevBuf vrank, sc3;
     BUF_TRY(vrank, sizeof(uint32_t) * (nvox + 1));
@@ -477,31 +497,77 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     return a;
 }
 
-/* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host:
- * one cooperative launch per group of up to MAX_BATCH_SCANS scans.  The chunk size
- * (points per partial sum) is a constant unless the caller sets points_per_wg, so
- * a scan's sums, and therefore its result, do not depend on the other scans of the
- * batch nor on how many workgroups the device runs. */
+/* Fills results from the device outputs of S scans; aborted marks scans that
+ * never finished as timed out. */
+static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results) {
+    std::vector<ScanOut> out(S);
+    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    int64_t pairs = 0;
+    for (int32_t s = 0; s < S; s++) {
+        ndt_result& r = results[s];
+        std::memset(&r, 0, sizeof(r));
+        for (int k = 0; k < 16; k++) {
+            r.final_transformation_f64[k] = out[s].final_m[k];
+            r.final_transformation[k] = (float)out[s].final_m[k];
+        }
+        r.score = out[s].score;
+        r.trans_probability = counts[s] > 0 ? out[s].score / (double)counts[s] : 0.0;
+        r.iterations = out[s].iterations;
+        r.converged = out[s].converged;
+        r.passes = out[s].passes;
+        r.status = out[s].status;
+        /* a scan that never finished has no final pose */
+        if (aborted && out[s].final_m[15] == 0.0) r.status = NDT_ERR_TIMEOUT;
+        pairs += (int64_t)out[s].pairs;
+    }
+    c->last_pairs = pairs;
+    if (aborted) return fail(NDT_ERR_TIMEOUT, "a device-side wait exceeded its bound");
+    return 0;
+}
+
+/* Work-queue items needed by a launch: every scan's chunks for each of its
+ * possible passes. */
+static uint64_t queue_capacity(int64_t total_chunks, int32_t max_iterations) {
+    return (uint64_t)total_chunks * (uint64_t)max_passes(max_iterations);
+}
+constexpr uint64_t MAX_QUEUE_ITEMS = 1ull << 28;
+
+/* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host.
+ * The chunk size (points per partial sum) is a constant unless the caller sets
+ * points_per_wg, and partial sums are reduced in a fixed order, so a scan's result
+ * does not depend on the other scans of the batch, on the number of workgroups
+ * the device runs, nor on the scheduler.  The work-queue scheduler runs the whole
+ * batch in one launch; the round scheduler runs groups of up to MAX_BATCH_SCANS
+ * scans and is also used when the queue would exceed MAX_QUEUE_ITEMS. */
 static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
                         const int64_t* counts, int32_t S, const double* guesses, const ndt_params* p,
                         ndt_result* results) {
     if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
+    if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
     const int block = p->block_size ? p->block_size : 256;
     AlignParams prm = make_align_params(p, S);
-    int bpc = 0;
-    LAUNCH_TRY(align_kernel_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
-    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
-    const int32_t nwg = bpc * c->cus;
     const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : ALIGN_CHUNK_POINTS;
     std::vector<ScanDesc> desc(S);
-    int64_t slots = 0;
+    int64_t slots = 0, max_chunks = 0;
     for (int32_t s = 0; s < S; s++) {
         desc[s].offset = offsets[s];
         desc[s].n = (int32_t)counts[s];
         desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
         desc[s].chunk_base = slots;
         slots += desc[s].nchunks;
+        max_chunks = std::max<int64_t>(max_chunks, desc[s].nchunks);
     }
+    const uint64_t cap = queue_capacity(slots, p->max_iterations);
+    const bool queue = p->scheduler != NDT_SCHED_ROUNDS && cap <= MAX_QUEUE_ITEMS && S < (1 << ITEM_SCAN_BITS) &&
+                       max_chunks < (1 << ITEM_CHUNK_BITS);
+    if (p->scheduler == NDT_SCHED_QUEUE && !queue)
+        return fail(NDT_ERR_UNSUPPORTED, "batch too large for the work-queue scheduler");
+    int bpc = 0;
+    if (queue) LAUNCH_TRY(align_wq_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
+    else LAUNCH_TRY(align_kernel_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
+    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
+    const int32_t nwg = bpc * c->cus;
     BUF_TRY(c->guesses, sizeof(double) * 16 * S);
     BUF_TRY(c->states, sizeof(ScanState) * S);
     BUF_TRY(c->sync, sizeof(RoundSync) * ((S + MAX_BATCH_SCANS - 1) / MAX_BATCH_SCANS));
@@ -516,56 +582,54 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     double prof_sum[4] = {0, 0, 0, 0};
     int aborted = 0;
     float ms_total = 0;
-    for (int32_t s0 = 0; s0 < S; s0 += MAX_BATCH_SCANS) {
-        const int32_t n = std::min<int32_t>(MAX_BATCH_SCANS, S - s0);
-        RoundSync* rs = c->sync.as<RoundSync>() + s0 / MAX_BATCH_SCANS;
-        LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0, n, prm, c->states.as<ScanState>() + s0,
-                                      rs, c->stream));
+    if (queue) {
+        BUF_TRY(c->queue, sizeof(WorkQueue));
+        BUF_TRY(c->items, sizeof(uint64_t) * cap);
+        BUF_TRY(c->done, sizeof(uint32_t) * S);
+        HIP_TRY(hipMemsetAsync(c->items.p, 0, sizeof(uint64_t) * cap, c->stream));
+        LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), S, prm, c->states.as<ScanState>(), c->sync.as<RoundSync>(),
+                                      c->stream));
+        LAUNCH_TRY(launch_init_queue(c->desc.as<ScanDesc>(), S, (uint64_t)slots, c->queue.as<WorkQueue>(),
+                                     c->items.as<uint64_t>(), c->done.as<uint32_t>(), c->stream));
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, (int32_t)chunk, nwg, block, prm,
-                                c->states.as<ScanState>() + s0, rs, c->partials.as<double>(), c->out.as<ScanOut>() + s0,
-                                c->small.as<double>(), c->stream));
+        LAUNCH_TRY(launch_align_wq(g->dev, pts, c->desc.as<ScanDesc>(), (int32_t)chunk, nwg, block, prm,
+                                   c->states.as<ScanState>(), c->queue.as<WorkQueue>(), c->items.as<uint64_t>(), cap,
+                                   c->done.as<uint32_t>(), c->partials.as<double>(), c->out.as<ScanOut>(),
+                                   c->small.as<double>(), c->stream));
         HIP_TRY(hipEventRecord(c->ev1, c->stream));
         double prof[5];
         HIP_TRY(hipMemcpyAsync(prof, c->small.p, sizeof(prof), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
-        float ms = 0;
-        HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-        ms_total += ms;
-        for (int k = 0; k < 4; k++) prof_sum[k] += prof[k];
-        if (prof[4] != 0.0) aborted = 1;
-        launches++;
+        HIP_TRY(hipEventElapsedTime(&ms_total, c->ev0, c->ev1));
+        for (int k = 0; k < 4; k++) prof_sum[k] = prof[k];
+        aborted = prof[4] != 0.0;
+        launches = 1;
+    } else {
+        for (int32_t s0 = 0; s0 < S; s0 += MAX_BATCH_SCANS) {
+            const int32_t n = std::min<int32_t>(MAX_BATCH_SCANS, S - s0);
+            RoundSync* rs = c->sync.as<RoundSync>() + s0 / MAX_BATCH_SCANS;
+            LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0, n, prm,
+                                          c->states.as<ScanState>() + s0, rs, c->stream));
+            HIP_TRY(hipEventRecord(c->ev0, c->stream));
+            LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, (int32_t)chunk, nwg, block, prm,
+                                    c->states.as<ScanState>() + s0, rs, c->partials.as<double>(),
+                                    c->out.as<ScanOut>() + s0, c->small.as<double>(), c->stream));
+            HIP_TRY(hipEventRecord(c->ev1, c->stream));
+            double prof[5];
+            HIP_TRY(hipMemcpyAsync(prof, c->small.p, sizeof(prof), hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+            ms_total += ms;
+            for (int k = 0; k < 4; k++) prof_sum[k] += prof[k];
+            if (prof[4] != 0.0) aborted = 1;
+            launches++;
+        }
     }
-    std::vector<ScanOut> out(S);
-    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
     c->last_ms = ms_total;
     c->last_launches = launches;
-    int64_t pairs = 0;
-    for (int32_t s = 0; s < S; s++) {
-        ndt_result& r = results[s];
-        std::memset(&r, 0, sizeof(r));
-        for (int k = 0; k < 16; k++) {
-            r.final_transformation_f64[k] = out[s].final_m[k];
-            r.final_transformation[k] = (float)out[s].final_m[k];
-        }
-        r.score = out[s].score;
-        r.trans_probability = counts[s] > 0 ? out[s].score / (double)counts[s] : 0.0;
-        r.iterations = out[s].iterations;
-        r.converged = out[s].converged;
-        r.passes = out[s].passes;
-        r.status = aborted && out[s].passes == 0 ? NDT_ERR_TIMEOUT : out[s].status;
-        if (aborted && r.status == 0 && r.iterations == 0 && r.passes == 0) r.status = NDT_ERR_TIMEOUT;
-        pairs += (int64_t)out[s].pairs;
-    }
-    c->last_pairs = pairs;
     for (int k = 0; k < 4; k++) c->last_profile[k] = prof_sum[k];
-    if (aborted) {
-        for (int32_t s = 0; s < S; s++)
-            if (results[s].passes == 0 || out[s].final_m[15] == 0.0) results[s].status = NDT_ERR_TIMEOUT;
-        return fail(NDT_ERR_TIMEOUT, "a device-side wait exceeded its bound");
-    }
-    return 0;
+    return collect_results(c, counts, S, aborted, results);
 }
 
 /* Spreads the bits of a 10-bit value to every third bit (Morton interleave). */
@@ -701,6 +765,61 @@ int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
     }
     HIP_TRY(hipSetDevice(c->device));
     return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results);
+}
+
+/* ------------------------------------------------------------------ fitness score */
+
+extern "C" int ndt_fitness_score(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* poses,
+                                 double max_range, double* fitness, int64_t* counts) {
+    if (!c || !g || !sc || !poses || !fitness) return fail(NDT_ERR_INVALID, "null argument");
+    if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "fitness needs an ndt-mode grid");
+    if (max_range != max_range) return fail(NDT_ERR_INVALID, "max_range is NaN");
+    HIP_TRY(hipSetDevice(c->device));
+    const int32_t S = sc->S;
+    const double res = (double)g->params.resolution;
+    int shells = FIT_MAX_SHELLS;
+    double max_d2 = INFINITY;
+    if (max_range > 0 && std::isfinite(max_range)) {
+        shells = (int)std::min<double>(FIT_MAX_SHELLS, std::ceil(max_range / res) + 1.0);
+        max_d2 = max_range * max_range;
+    }
+    std::vector<double> local(poses, poses + 16 * (size_t)S);
+    for (int32_t s = 0; s < S; s++) {
+        local[16 * (size_t)s + 3] -= g->params.origin[0];
+        local[16 * (size_t)s + 7] -= g->params.origin[1];
+        local[16 * (size_t)s + 11] -= g->params.origin[2];
+    }
+    std::vector<int32_t> cnt((size_t)S);
+    int64_t maxn = 1;
+    for (int32_t s = 0; s < S; s++) {
+        cnt[(size_t)s] = (int32_t)sc->counts[(size_t)s];
+        maxn = std::max<int64_t>(maxn, sc->counts[(size_t)s]);
+    }
+    const int nblk = (int)std::min<int64_t>(64, (maxn + 255) / 256);
+    DevBuf d_off, d_cnt, d_pose, d_part;
+    BUF_TRY(d_off, 8 * (size_t)S);
+    BUF_TRY(d_cnt, 4 * (size_t)S);
+    BUF_TRY(d_pose, 8 * 16 * (size_t)S);
+    BUF_TRY(d_part, 16 * (size_t)S * nblk);
+    HIP_TRY(hipMemcpyAsync(d_off.p, sc->offsets.data(), 8 * (size_t)S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_cnt.p, cnt.data(), 4 * (size_t)S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(d_pose.p, local.data(), 8 * local.size(), hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_fitness(g->fdev, sc->pts.as<float4>(), d_off.as<int64_t>(), d_cnt.as<int32_t>(), S,
+                              d_pose.as<double>(), max_d2, shells, nblk, d_part.as<double>(), c->stream));
+    std::vector<double> part(2 * (size_t)S * nblk);
+    HIP_TRY(hipMemcpyAsync(part.data(), d_part.p, 8 * part.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int32_t s = 0; s < S; s++) {
+        double sum = 0.0, n = 0.0;
+        for (int b = 0; b < nblk; b++) {
+            sum += part[((size_t)s * nblk + b) * 2];
+            n += part[((size_t)s * nblk + b) * 2 + 1];
+        }
+        /* PCL returns the largest double when no point has a correspondence */
+        fitness[s] = n > 0 ? sum / n : 1.7976931348623157e308;
+        if (counts) counts[s] = (int64_t)n;
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------ single derivative pass */

@@ -87,7 +87,7 @@ struct ndt_ctx {
     int cus = 0;
     char name[256] = {0};
     /* workspaces */
-    DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2;
+    DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2, queue, items, done;
     double last_ms = 0.0;
     int32_t last_launches = 0;
     int64_t last_pairs = 0;
@@ -101,6 +101,9 @@ struct ndt_grid {
     /* ndt */
     ndt::NdtGridDev dev;
     DevBuf slots, cand, icov_f, mean_c, icov_c;
+    /* map points sorted by voxel + occupied-voxel hash (fitness score) */
+    ndt::FitGridDev fdev;
+    DevBuf fit_pts, fit_slots;
     /* all occupied voxels (diagnostics) */
     DevBuf all_mean, all_icov, all_cov, all_count, all_valid, all_key;
     /* compat */

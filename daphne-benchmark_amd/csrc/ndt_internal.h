@@ -236,6 +236,18 @@ struct NdtGridDev {
     double resolution_d;
 };
 
+/* Device view of the map points of an ndt grid, sorted by voxel, for nearest-point
+ * queries (fitness score). */
+struct FitGridDev {
+    const HashSlot* slots;   /* occupied voxel -> range of sorted points */
+    const float4* pts;       /* map points sorted by voxel key */
+    int log2cap;
+    int bits[3];
+    int64_t origin_cell[3];
+    double resolution;
+};
+constexpr int FIT_MAX_SHELLS = 16; /* nearest-point search radius cap, in cells */
+
 /* Device view of a compat-mode grid (the reference's dense indexing, hashed). */
 struct CompatGridDev {
     const HashSlot* slots;   /* dense cell index -> voxel id (begin) */
@@ -299,6 +311,28 @@ struct RoundSync {
     uint32_t pad[62];
 };
 
+/* Work queue of one asynchronous align launch: items are (pass, scan, chunk)
+ * triples, written once each at increasing positions; head = next position to
+ * claim, tail = next position to fill, remaining = scans not finished. */
+struct WorkQueue {
+    uint32_t head;
+    uint32_t tail;
+    uint32_t remaining;
+    uint32_t abort;
+    uint32_t pad[60];
+};
+constexpr uint64_t ITEM_VALID = 1ull << 63;
+constexpr int ITEM_CHUNK_BITS = 20;
+constexpr int ITEM_SCAN_BITS = 20;
+constexpr int ITEM_PASS_BITS = 23;
+NDT_HD uint64_t make_item(uint32_t pass, uint32_t scan, uint32_t chunk) {
+    return ITEM_VALID | ((uint64_t)pass << (ITEM_CHUNK_BITS + ITEM_SCAN_BITS)) | ((uint64_t)scan << ITEM_CHUNK_BITS) |
+           (uint64_t)chunk;
+}
+/* upper bound of the derivative passes of one scan: the first pass, then per
+ * Newton iteration at most 1 + 10 line-search evaluations */
+NDT_HD int64_t max_passes(int32_t max_iterations) { return 1 + 11 * (int64_t)max_iterations; }
+
 /* per-scan output written by the device */
 struct ScanOut {
     double final_m[16];
@@ -353,6 +387,15 @@ int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, con
                               double* mean, double* icov, int32_t* count, uint64_t* vkey, hipStream_t stream);
 
 /* ndt alignment: init states from guesses, then the cooperative persistent kernel */
+int launch_gather_points(const float4* pts, const uint32_t* idx, int64_t n, float4* out, hipStream_t stream);
+int launch_fitness(FitGridDev fg, const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S,
+                   const double* poses, double max_d2, int shells, int nblk, double* partials, hipStream_t stream);
+int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, WorkQueue* q, uint64_t* items,
+                      uint32_t* done_cnt, hipStream_t stream);
+int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg, int block_size,
+                    AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
+                    double* partials, ScanOut* out, double* prof, hipStream_t stream);
+int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
 int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
                        hipStream_t stream);
 int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);

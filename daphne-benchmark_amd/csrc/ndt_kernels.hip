@@ -1381,6 +1381,28 @@ __device__ inline uint32_t lds_exclusive_scan(uint32_t* v, int n, uint32_t* lw) 
     return total;
 }
 
+/* Sum of a scan's chunk slots in a fixed order, the same in every kernel and for
+ * every block size: four interleaved groups (group q sums chunks q, q + 4, ...)
+ * added in group order.  Threads 0..127 take part; red[0..NRED) receives the sums. */
+constexpr int SLOT_GROUPS = 4;
+__device__ inline void reduce_slots(const double* partials, int64_t base, int nchunks, double (*gred)[32],
+                                    double* red) {
+    const int grp = threadIdx.x >> 5, j = threadIdx.x & 31;
+    if (threadIdx.x < SLOT_GROUPS * 32) {
+        double a = 0.0;
+        if (j < NRED)
+            for (int c = grp; c < nchunks; c += SLOT_GROUPS) a += ld_agent_f64(partials + (base + c) * 32 + j);
+        gred[grp][j] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED) {
+        double r = gred[0][threadIdx.x];
+        for (int q = 1; q < SLOT_GROUPS; q++) r += gred[q][threadIdx.x];
+        red[threadIdx.x] = r;
+    }
+    __syncthreads();
+}
+
 template <typename T, int SUM, int BLOCK>
 __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(
     NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t S, int32_t chunk,
@@ -1389,6 +1411,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double lds_red[BLOCK / WAVE][NRED];
     __shared__ double red[NRED];
     __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ double gred[SLOT_GROUPS][32];
     __shared__ uint32_t s_active[MAX_BATCH];
     __shared__ uint32_t s_prefix[MAX_BATCH + 1];
     __shared__ uint32_t lw[BLOCK / WAVE];
@@ -1467,12 +1490,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const int s = (int)s_active[ai];
             const ScanDesc sd = scans[s];
             for (int q = tid; q < STATE_WORDS; q += BLOCK) sstate[q] = ld_agent_u64((const uint64_t*)(states + s) + q);
-            if (tid < NRED) {
-                double acc2 = 0.0;
-                for (int q = 0; q < sd.nchunks; q++) acc2 += ld_agent_f64(partials + (sd.chunk_base + q) * 32 + tid);
-                red[tid] = acc2;
-            }
-            __syncthreads();
+            reduce_slots(partials, sd.chunk_base, sd.nchunks, gred, red);
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
             __syncthreads();
             for (int q = tid; q < STATE_WORDS; q += BLOCK) st_agent_u64((uint64_t*)(states + s) + q, sstate[q]);
@@ -1548,6 +1566,306 @@ int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int3
     launch_align_t<T, S_, B>(grid, pts, scans, S, chunk, nwg, prm, states, rs, partials, out, prof, stream)
     NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
 #undef NDT_LA
+}
+
+/* ------------------------------------------------------------------ asynchronous align kernel
+ * One launch aligns every scan of a batch without grid-wide synchronisation.  A
+ * scan's pass is cut into chunks; each chunk is an item of a work queue.  Every
+ * workgroup repeatedly claims the next queue position, waits until its item is
+ * written, evaluates the chunk at the scan's current trial pose and leaves the 28
+ * sums (and pair count) in the chunk's slot.  The workgroup that completes a
+ * scan's last chunk (a per-scan counter) reduces the slots in chunk order, runs
+ * the controller and appends the chunks of the scan's next pass to the queue, or
+ * retires the scan.  Scans therefore advance independently: nothing waits for
+ * the slowest chunk of other scans.  Values handed between workgroups use
+ * device-scope relaxed atomics (sc1 loads and stores); each wave waits for its
+ * stores to complete before the counter or queue update that publishes them.
+ * Every wait is bounded in time and ends when all scans are finished. */
+
+__device__ inline void stores_complete() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__global__ void k_init_queue(const ScanDesc* __restrict__ scans, int32_t S, uint64_t total, WorkQueue* q,
+                             uint64_t* __restrict__ items, uint32_t* __restrict__ done_cnt) {
+    const int s = blockIdx.x;
+    if (s >= S) return;
+    const ScanDesc sd = scans[s];
+    for (int k = threadIdx.x; k < sd.nchunks; k += blockDim.x) items[sd.chunk_base + k] = make_item(0, s, k);
+    if (threadIdx.x == 0) done_cnt[s] = 0u;
+    if (s == 0 && threadIdx.x == 0) {
+        q->head = 0u;
+        q->tail = (uint32_t)total;
+        q->remaining = (uint32_t)S;
+        q->abort = 0u;
+    }
+}
+
+int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, WorkQueue* q, uint64_t* items,
+                      uint32_t* done_cnt, hipStream_t stream) {
+    hipLaunchKernelGGL(k_init_queue, dim3(S), dim3(256), 0, stream, scans, S, total_chunks, q, items, done_cnt);
+    return (int)hipGetLastError();
+}
+
+template <typename T, int SUM, int BLOCK>
+__global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align_wq(
+    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t chunk, AlignParams prm,
+    ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt, double* partials,
+    ScanOut* out, double* prof_out) {
+    __shared__ PassParams<T> pp;
+    __shared__ double lds_red[BLOCK / WAVE][NRED];
+    __shared__ double red[NRED];
+    __shared__ double gred[SLOT_GROUPS][32];
+    __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ uint64_t s_item, s_ppkey;
+    __shared__ uint32_t s_base;
+    __shared__ int s_last, s_done;
+    const int tid = threadIdx.x;
+    double prof[3] = {0, 0, 0};
+    int ok = 1;
+    if (tid == 0) s_ppkey = ~0ull;
+    for (;;) {
+        const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+        if (tid == 0) {
+            uint64_t v = 0;
+            const uint32_t pos = __hip_atomic_fetch_add(&q->head, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((uint64_t)pos < cap) {
+                uint32_t spins = 0;
+                for (;;) {
+                    v = ld_agent_u64(items + pos);
+                    if (v) break;
+                    if (__hip_atomic_load(&q->remaining, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u ||
+                        __hip_atomic_load(&q->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+                        break;
+                    __builtin_amdgcn_s_sleep(4);
+                    if ((++spins & 63u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                        __hip_atomic_store(&q->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        break;
+                    }
+                }
+            }
+            s_item = v;
+        }
+        __syncthreads();
+        const uint64_t item = s_item;
+        const uint64_t t1 = __builtin_amdgcn_s_memrealtime();
+        if (!item) break;
+        const int s = (int)((item >> ITEM_CHUNK_BITS) & ((1u << ITEM_SCAN_BITS) - 1));
+        const int k = (int)(item & ((1u << ITEM_CHUNK_BITS) - 1));
+        const uint32_t pass = (uint32_t)((item >> (ITEM_CHUNK_BITS + ITEM_SCAN_BITS)) & ((1u << ITEM_PASS_BITS) - 1));
+        const uint64_t ppkey = item >> ITEM_CHUNK_BITS;
+        const ScanDesc sd = scans[s];
+        if (ppkey != s_ppkey) {
+            if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
+            __syncthreads();
+            if (tid == 0) s_ppkey = ppkey;
+            __syncthreads();
+        }
+        Acc<T, SUM> acc;
+        acc.zero();
+        int pairs = 0;
+        {
+            const PassParams<T> u = load_uniform(pp);
+            const int64_t pb = sd.offset + (int64_t)k * chunk;
+            const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * chunk);
+            for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+        }
+        double v[NRED];
+#pragma unroll
+        for (int qq = 0; qq < NACC; qq++) v[qq] = acc.get(qq);
+        v[NACC] = (double)pairs;
+        block_reduce_acc<BLOCK>(v, lds_red, red);
+        if (tid < NRED) st_agent_f64(partials + (sd.chunk_base + k) * 32 + tid, red[tid]);
+        stores_complete();
+        __syncthreads();
+        if (tid == 0) {
+            const uint32_t old = __hip_atomic_fetch_add(&done_cnt[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_last = (old + 1u == (uint32_t)sd.nchunks) ? 1 : 0;
+        }
+        __syncthreads();
+        const uint64_t t2 = __builtin_amdgcn_s_memrealtime();
+        if (s_last) {
+            /* every chunk of this pass is in: reduce in chunk order, G interleaved groups */
+            if (tid == 0) __hip_atomic_store(&done_cnt[s], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int w = tid; w < STATE_WORDS; w += BLOCK) sstate[w] = ld_agent_u64((const uint64_t*)(states + s) + w);
+            reduce_slots(partials, sd.chunk_base, sd.nchunks, gred, red);
+            if (tid == 0) {
+                ScanState& st = *(ScanState*)sstate;
+                ctl_step(st, red, prm, out + s);
+                if (!st.done && (uint64_t)pass + 1 >= (1ull << ITEM_PASS_BITS)) {
+                    st.status = NDT_ERR_UNSUPPORTED;
+                    ctl_finish(st, prm, 0, out + s);
+                }
+                s_done = st.done;
+            }
+            __syncthreads();
+            for (int w = tid; w < STATE_WORDS; w += BLOCK) st_agent_u64((uint64_t*)(states + s) + w, sstate[w]);
+            stores_complete();
+            __syncthreads();
+            if (!s_done) {
+                if (tid == 0)
+                    s_base = __hip_atomic_fetch_add(&q->tail, (uint32_t)sd.nchunks, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT);
+                __syncthreads();
+                const uint64_t base = s_base;
+                if (base + (uint64_t)sd.nchunks > cap) {
+                    if (tid == 0) __hip_atomic_store(&q->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else {
+                    for (int c = tid; c < sd.nchunks; c += BLOCK) st_agent_u64(items + base + c, make_item(pass + 1, s, c));
+                }
+            } else if (tid == 0) {
+                __hip_atomic_fetch_sub(&q->remaining, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        const uint64_t t3 = __builtin_amdgcn_s_memrealtime();
+        if (blockIdx.x == 0 && tid == 0) {
+            prof[0] += (double)(t1 - t0);
+            prof[1] += (double)(t2 - t1);
+            prof[2] += (double)(t3 - t2);
+        }
+    }
+    if (tid == 0 && __hip_atomic_load(&q->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+    if (blockIdx.x == 0 && tid == 0) {
+        /* s_memrealtime counts at 100 MHz: active-list time is not used here */
+        prof_out[0] = 0.0;
+        prof_out[1] = prof[1] * 0.01;
+        prof_out[2] = prof[0] * 0.01;
+        prof_out[3] = prof[2] * 0.01;
+        prof_out[4] = ok ? 0.0 : 1.0;
+    }
+}
+
+template <typename T, int SUM, int BLOCK>
+static int launch_align_wq_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg,
+                             AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap,
+                             uint32_t* done_cnt, double* partials, ScanOut* out, double* prof, hipStream_t stream) {
+    void* args[] = {&g, &pts, &scans, &chunk, &prm, &states, &q, &items, &cap, &done_cnt, &partials, &out, &prof};
+    return (int)hipLaunchCooperativeKernel((const void*)k_align_wq<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
+                                           stream);
+}
+
+template <typename T, int SUM, int BLOCK>
+static int max_blocks_wq_t(int* bpc) {
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align_wq<T, SUM, BLOCK>, BLOCK, 0);
+}
+
+int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu) {
+#define NDT_MB(T, S, B) max_blocks_wq_t<T, S, B>(blocks_per_cu)
+    NDT_DISPATCH(block_size, point_fp64, sum_mode, NDT_MB);
+#undef NDT_MB
+}
+
+int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg, int block_size,
+                    AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
+                    double* partials, ScanOut* out, double* prof, hipStream_t stream) {
+#define NDT_LA(T, S_, B) \
+    launch_align_wq_t<T, S_, B>(g, pts, scans, chunk, nwg, prm, states, q, items, cap, done_cnt, partials, out, prof, stream)
+    NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
+#undef NDT_LA
+}
+
+/* ------------------------------------------------------------------ fitness score
+ * Mean squared distance from the transformed scan points to their nearest map
+ * points, over the points whose nearest map point lies within max_range (PCL's
+ * Registration::getFitnessScore, which Autoware's ndt_matching reports and which
+ * the reference declares as ndt_getFitnessScore, src/Cuda/ndt_mapping/kernel.cu:124).
+ * The search visits shells of cells at growing Chebyshev distance r around the
+ * point's cell and stops once no cell of the next shell can hold a closer point:
+ * every point of shell r >= 1 is at least (r - 1) res + dmin away, dmin being the
+ * point's distance to the nearest face of its own cell. */
+
+__global__ void k_gather_points(const float4* __restrict__ pts, const uint32_t* __restrict__ idx, int64_t n,
+                                float4* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = pts[idx[i]];
+}
+
+int launch_gather_points(const float4* pts, const uint32_t* idx, int64_t n, float4* out, hipStream_t stream) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_gather_points, dim3(grid_for(n, 256)), dim3(256), 0, stream, pts, idx, n, out);
+    return (int)hipGetLastError();
+}
+
+__device__ inline void fit_cell(const FitGridDev& fg, int64_t c0, int64_t c1, int64_t c2, double x0, double x1, double x2,
+                                double& best) {
+    c0 -= fg.origin_cell[0];
+    c1 -= fg.origin_cell[1];
+    c2 -= fg.origin_cell[2];
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= (1ll << fg.bits[0]) || c1 >= (1ll << fg.bits[1]) || c2 >= (1ll << fg.bits[2]))
+        return;
+    const uint64_t key = ((uint64_t)c2 << (fg.bits[0] + fg.bits[1])) | ((uint64_t)c1 << fg.bits[0]) | (uint64_t)c0;
+    uint32_t b, n;
+    if (!hash_find(fg.slots, fg.log2cap, key, b, n)) return;
+    for (uint32_t k = 0; k < n; k++) {
+        const float4 m = fg.pts[b + k];
+        const double d0 = x0 - (double)m.x, d1 = x1 - (double)m.y, d2 = x2 - (double)m.z;
+        const double d = d0 * d0 + d1 * d1 + d2 * d2;
+        best = d < best ? d : best;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_fitness(FitGridDev fg, const float4* __restrict__ pts,
+                                                 const int64_t* __restrict__ offsets,
+                                                 const int32_t* __restrict__ counts, const double* __restrict__ poses,
+                                                 double max_d2, int shells, double* __restrict__ partials) {
+    __shared__ double red[2][256 / WAVE];
+    const int s = blockIdx.y;
+    const int64_t off = offsets[s];
+    const int n = counts[s];
+    const double* P = poses + 16 * (int64_t)s;
+    const double res = fg.resolution;
+    double sum = 0.0, cnt = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float4 p = pts[off + i];
+        const double X0 = p.x, X1 = p.y, X2 = p.z;
+        const double x0 = P[0] * X0 + P[1] * X1 + P[2] * X2 + P[3];
+        const double x1 = P[4] * X0 + P[5] * X1 + P[6] * X2 + P[7];
+        const double x2 = P[8] * X0 + P[9] * X1 + P[10] * X2 + P[11];
+        if (!(fabs(x0) < 1e15) || !(fabs(x1) < 1e15) || !(fabs(x2) < 1e15)) continue;
+        const int64_t a0 = (int64_t)floor(x0 / res), a1 = (int64_t)floor(x1 / res), a2 = (int64_t)floor(x2 / res);
+        double dmin = fmin(fmin(x0 - a0 * res, (a0 + 1) * res - x0),
+                           fmin(fmin(x1 - a1 * res, (a1 + 1) * res - x1), fmin(x2 - a2 * res, (a2 + 1) * res - x2)));
+        dmin = fmax(dmin, 0.0);
+        double best = INFINITY;
+        fit_cell(fg, a0, a1, a2, x0, x1, x2, best);
+        for (int r = 1; r <= shells; r++) {
+            const double lb = (r - 1) * res + dmin;
+            if (lb * lb >= best) break;
+            for (int dz = -r; dz <= r; dz++)
+                for (int dy = -r; dy <= r; dy++) {
+                    const bool face = (dz == -r || dz == r || dy == -r || dy == r);
+                    for (int dx = -r; dx <= r; dx += face ? 1 : 2 * r)
+                        fit_cell(fg, a0 + dx, a1 + dy, a2 + dz, x0, x1, x2, best);
+                }
+        }
+        if (best <= max_d2) {
+            sum += best;
+            cnt += 1.0;
+        }
+    }
+    sum = wave_sum(sum);
+    cnt = wave_sum(cnt);
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane == 0) {
+        red[0][wave] = sum;
+        red[1][wave] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / WAVE; w++) t += red[threadIdx.x][w];
+        partials[((int64_t)s * gridDim.x + blockIdx.x) * 2 + threadIdx.x] = t;
+    }
+}
+
+int launch_fitness(FitGridDev fg, const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S,
+                   const double* poses, double max_d2, int shells, int nblk, double* partials, hipStream_t stream) {
+    if (S <= 0) return 0;
+    hipLaunchKernelGGL(k_fitness, dim3(nblk, S), dim3(256), 0, stream, fg, pts, offsets, counts, poses, max_d2, shells,
+                       partials);
+    return (int)hipGetLastError();
 }
 
 /* ------------------------------------------------------------------ single derivative pass (ndt) */

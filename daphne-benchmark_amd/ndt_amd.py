@@ -346,13 +346,15 @@ def check_outputs(results, references):
 NDT_MODE_COMPAT, NDT_MODE_NDT = 0, 1
 NDT_POINT_FP32, NDT_POINT_FP64 = 0, 1
 NDT_SUM_FP32, NDT_SUM_COMP32, NDT_SUM_FP64 = 0, 1, 2
+NDT_SCHED_DEFAULT, NDT_SCHED_ROUNDS, NDT_SCHED_QUEUE = 0, 1, 2
 
 
 class NdtParams(C.Structure):
     _fields_ = [("mode", C.c_int32), ("resolution", C.c_float), ("step_size", C.c_double), ("trans_eps", C.c_double),
                 ("max_iterations", C.c_int32), ("outlier_ratio", C.c_double), ("min_points", C.c_int32),
                 ("min_eig_ratio", C.c_double), ("point_precision", C.c_int32), ("sum_precision", C.c_int32),
-                ("origin", C.c_double * 3), ("block_size", C.c_int32), ("points_per_wg", C.c_int32)]
+                ("origin", C.c_double * 3), ("block_size", C.c_int32), ("points_per_wg", C.c_int32),
+                ("scheduler", C.c_int32)]
 
 
 class NdtResult(C.Structure):
@@ -410,6 +412,7 @@ def hip_lib():
         L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.ndt_ctx_last_profile.argtypes = [P, P]
+        L.ndt_fitness_score.argtypes = [P, P, P, P, C.c_double, P, P]
         # learned warm start (ndt_learn.h)
         L.ndt_bevmap_build.argtypes = [P, P, C.c_int64, P, C.POINTER(P)]
         L.ndt_bevmap_destroy.argtypes = [P]
@@ -440,7 +443,7 @@ EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_
                     "ndt_ctx_device_info", "ndt_grid_build", "ndt_grid_info_get", "ndt_grid_destroy",
                     "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
                     "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing",
-                    "ndt_ctx_last_profile")
+                    "ndt_ctx_last_profile", "ndt_fitness_score")
 LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_model_create", "ndt_model_destroy",
                  "ndt_model_dims", "ndt_model_save", "ndt_model_load", "ndt_model_get", "ndt_model_set",
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
@@ -620,6 +623,15 @@ class Scans:
         out = (NdtResult * self.S)()
         _check(hip_lib().ndt_align_scans(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm), out))
         return [_result(out[i]) for i in range(self.S)]
+
+    def fitness(self, grid: Grid, poses, max_range: float = 0.0):
+        """(fitness (S,), counts (S,)) at world poses (getFitnessScore)."""
+        poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(self.S, 16)
+        fit = np.zeros(self.S)
+        cnt = np.zeros(self.S, np.int64)
+        _check(hip_lib().ndt_fitness_score(self.ctx.h, grid.h, self.h, poses.ctypes.data, max_range,
+                                           fit.ctypes.data, cnt.ctypes.data))
+        return fit, cnt
 
 
 def partial_points_callback(ctx: Context, input_cloud, init_guess, target_cloud) -> CallbackResult:

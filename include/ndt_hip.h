@@ -47,6 +47,10 @@ enum { NDT_MODE_COMPAT = 0, NDT_MODE_NDT = 1 };
 enum { NDT_POINT_FP32 = 0, NDT_POINT_FP64 = 1 };
 /* accumulation of score / gradient / Hessian terms within a thread */
 enum { NDT_SUM_FP32 = 0, NDT_SUM_COMP32 = 1, NDT_SUM_FP64 = 2 };
+/* scheduling of batched alignment: a work queue in which every scan advances on
+ * its own (default), or rounds in which all active scans run one pass between
+ * grid barriers; both give bit-identical results */
+enum { NDT_SCHED_DEFAULT = 0, NDT_SCHED_ROUNDS = 1, NDT_SCHED_QUEUE = 2 };
 
 /* NDT parameters.  The reference keeps them as members of ndt_mapping
  * (src/CPU/ndt_mapping/kernel.cpp:37-41,56-57); ndt_params_default() fills them. */
@@ -62,8 +66,9 @@ typedef struct ndt_params {
     int32_t point_precision;  /* NDT_POINT_FP32 / NDT_POINT_FP64 (ndt) */
     int32_t sum_precision;    /* NDT_SUM_FP32 / NDT_SUM_COMP32 / NDT_SUM_FP64 (ndt) */
     double origin[3];         /* world position of the map-local frame (ndt) */
-    int32_t block_size;       /* 0 = tuned/default; threads per workgroup (ndt align) */
-    int32_t points_per_wg;    /* 0 = tuned/default; target scan points per workgroup */
+    int32_t block_size;       /* 0 = default (256); threads per workgroup (ndt align) */
+    int32_t points_per_wg;    /* 0 = default (512); scan points per partial sum (ndt align) */
+    int32_t scheduler;        /* NDT_SCHED_* (ndt align) */
 } ndt_params;
 
 typedef struct ndt_result {
@@ -167,6 +172,20 @@ int ndt_compat_callback(ndt_ctx* ctx, const float* scan_xyzw, int64_t n, const f
 int ndt_derivatives(ndt_ctx* ctx, const ndt_grid* grid, const float* scan_xyzw, int64_t n,
                     const double p[6], const float compat_transform[16], const ndt_params* params,
                     double* score, double gradient[6], double hessian[36], int64_t* pairs);
+
+/* Fitness score of S device-resident scans at world poses (row-major float64,
+ * S x 16): mean squared distance from each transformed scan point to its nearest
+ * map point, over the points whose nearest map point lies within max_range; the
+ * largest double when no point qualifies.  This is PCL's
+ * Registration::getFitnessScore, which the reference declares as
+ * ndt_getFitnessScore (src/Cuda/ndt_mapping/kernel.cu:124,
+ * src/OpenCl/ndt_mapping/kernel.cpp:212) but never implements.  max_range <= 0 or
+ * infinite searches the cells within Chebyshev distance 16 of each point's cell
+ * (points whose nearest map point lies beyond count as having none); a finite
+ * max_range up to 15 resolutions is exact.  counts (optional) receives the number of points used.
+ * Needs an ndt-mode grid. */
+int ndt_fitness_score(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans, const double* poses,
+                      double max_range, double* fitness, int64_t* counts);
 
 /* Timing of the last ndt_align / ndt_align_batch / ndt_align_scans call on this
  * context: device milliseconds of the alignment kernel(s), measured with HIP

@@ -350,3 +350,30 @@ def pose_error(T, T_ref):
     c = (np.trace(R) - 1.0) / 2.0
     c = min(1.0, max(-1.0, c))
     return dt, float(np.arccos(c))
+
+
+def fitness_score(map_xyzw, scan_xyzw, pose, max_range=np.inf):
+    """Fitness score (PCL Registration::getFitnessScore): mean squared distance from
+    the transformed scan points (float64 transform of the float32 points) to their
+    nearest map points, over points whose nearest map point lies within max_range;
+    the largest double when none does.  Brute force in float64, in blocks.
+    Returns (fitness, count).  The reference declares ndt_getFitnessScore
+    (src/Cuda/ndt_mapping/kernel.cu:124) without implementing it."""
+    m = np.asarray(map_xyzw, np.float32)[:, :3].astype(np.float64)
+    m = m[np.all(np.isfinite(m), axis=1)]
+    s = np.asarray(scan_xyzw, np.float32)[:, :3].astype(np.float64)
+    T = np.asarray(pose, np.float64).reshape(4, 4)
+    x = s @ T[:3, :3].T + T[:3, 3]
+    x = x[np.all(np.isfinite(x), axis=1)]
+    mm = np.sum(m * m, axis=1)
+    best = np.empty(len(x))
+    for b in range(0, len(x), 256):
+        xb = x[b:b + 256]
+        # exact differences for the candidate found by the expanded form
+        d2 = np.sum(xb * xb, axis=1)[:, None] - 2.0 * xb @ m.T + mm[None, :]
+        j = np.argsort(d2, axis=1)[:, :8]
+        exact = np.sum((xb[:, None, :] - m[j]) ** 2, axis=2)
+        best[b:b + 256] = exact.min(axis=1)
+    ok = best <= max_range * max_range if np.isfinite(max_range) else np.ones(len(best), bool)
+    n = int(ok.sum())
+    return (float(best[ok].sum() / n) if n else 1.7976931348623157e308), n

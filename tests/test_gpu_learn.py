@@ -1,0 +1,108 @@
+"""GPU tests of the learned warm start against its CPU oracle
+(oracle/learn_oracle.py): features bit-exact, forward / backward / Adam of the bf16
+MLP within stated tolerances, training that lowers the loss and a refined guess
+that helps NDT.
+
+Tolerances: the GPU accumulates in float32 (split-K partial sums, then bias), the
+oracle in float64 with the same bf16 roundings of weights, activations and
+backpropagated signals; a float32 difference that moves a value across a bf16
+rounding boundary changes it by one bf16 ulp (2^-8 relative).  Forward outputs
+must agree within 1e-2 of their largest magnitude and every gradient within 2e-2
+relative Frobenius error (observed: about 1e-3)."""
+import numpy as np
+import pytest
+
+import learn_oracle as L
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_scene(A):
+    w = A.make_workload("cfg0", seed=3, scans=6)
+    return w
+
+
+def test_features_match_oracle(A, ctx, small_scene):
+    w = small_scene
+    sc = A.Scans(ctx, w.scans)
+    bm = A.BevMap(ctx, w.map_xyzw)
+    g = A.perturb_poses(w.gt, np.random.default_rng(0))
+    idx = np.array([0, 1, 2, 3, 4, 5, 2, 0], np.int32)
+    gg = np.concatenate([g, g[:2]])
+    F = bm.features(sc, idx, gg)
+    lut = L.log1p_lut()
+    for b, s in enumerate(idx):
+        assert np.array_equal(F[b, :32768], L.scan_grid(w.scans[s], lut)), b
+        assert np.array_equal(F[b, 32768:], L.map_grid(w.map_xyzw, gg[b], lut)), b
+    assert (F[:, :32768] != 0).sum() > 1000 and (F[:, 32768:] != 0).sum() > 10000
+
+
+def test_features_world_origin(A, ctx, small_scene):
+    """Guesses in world coordinates: the map grid is taken relative to the origin."""
+    w = small_scene
+    origin = np.array([4.2e5, 5.6e6, 80.0])
+    sc = A.Scans(ctx, w.scans[:2])
+    bm = A.BevMap(ctx, w.map_xyzw, origin)
+    g = A.perturb_poses(w.gt[:2], np.random.default_rng(1))
+    gw = g.copy()
+    gw[:, :3, 3] += origin
+    F = bm.features(sc, np.array([0, 1], np.int32), gw)
+    for b in range(2):
+        assert np.array_equal(F[b, 32768:], L.map_grid(w.map_xyzw, g[b]))
+
+
+def _random_model(A, ctx, f0, f1, f2, seed):
+    m = A.Model(ctx, f0, f1, f2, seed=seed)
+    r = np.random.default_rng(seed)
+    P = {"W1": r.normal(size=(f1, f0)) * np.sqrt(2 / f0), "b1": r.normal(size=f1) * 0.1,
+         "W2": r.normal(size=(f2, f1)) * np.sqrt(2 / f1), "b2": r.normal(size=f2) * 0.1,
+         "W3": r.normal(size=(6, f2)) * np.sqrt(1 / f2), "b3": r.normal(size=6) * 0.1}
+    for k, key in enumerate(["W1", "b1", "W2", "b2", "W3", "b3"]):
+        m.set(k, P[key].astype(np.float32))
+        P[key] = P[key].astype(np.float32).astype(np.float64)
+    return m, P, r
+
+
+@pytest.mark.parametrize("dims", [(256, 128, 128, 128), (1024, 256, 128, 256), (65536, 1024, 256, 128)])
+def test_forward_backward_adam_match_oracle(A, ctx, dims):
+    f0, f1, f2, B = dims
+    m, P, r = _random_model(A, ctx, f0, f1, f2, f0 + B)
+    xb = L.bf16_bits(np.abs(r.normal(size=(B, f0))).astype(np.float32))
+    x = L.bf16_value(xb).astype(np.float64)
+    T = r.normal(size=(B, 6)).astype(np.float32)
+    wts = np.array([1, 1, 0.5, 2, 2, 1], np.float32)
+    y = m.forward(xb)
+    loss = m.backward(xb, T, wts)
+    lo, go, fo = L.mlp_backward(x, P, T.astype(np.float64), wts.astype(np.float64), bf16=True)
+    assert np.abs(y - fo["Y"]).max() <= 1e-2 * np.abs(fo["Y"]).max()
+    assert abs(loss - lo) <= 1e-2 * lo
+    for k, key in enumerate(["W1", "b1", "W2", "b2", "W3", "b3"]):
+        gg = m.grad(k).astype(np.float64)
+        rel = np.linalg.norm(gg - go[key]) / np.linalg.norm(go[key])
+        assert rel < 2e-2, (key, rel)
+    # one Adam step on the stored gradients
+    w0 = [m.get(k).astype(np.float64) for k in range(6)]
+    g0 = [m.grad(k).astype(np.float64) for k in range(6)]
+    m.adam_step(1e-3)
+    for k in range(6):
+        we, _, _ = L.adam_step(w0[k], np.zeros_like(w0[k]), np.zeros_like(w0[k]), g0[k], 1, 1e-3)
+        assert np.abs(m.get(k) - we).max() < 1e-6, k
+
+
+def test_model_save_load(A, ctx, tmp_path):
+    m, P, r = _random_model(A, ctx, 256, 128, 128, 7)
+    p = tmp_path / "m.bin"
+    m.save(str(p))
+    m2 = A.Model(ctx, path=str(p))
+    assert m2.dims == (256, 128, 128)
+    xb = L.bf16_bits(np.abs(r.normal(size=(128, 256))).astype(np.float32))
+    assert np.array_equal(m.forward(xb), m2.forward(xb))
+
+
+def test_invalid_model_arguments(A, ctx):
+    with pytest.raises(A.NdtError):
+        A.Model(ctx, 100, 128, 128)
+    m = A.Model(ctx, 256, 128, 128)
+    with pytest.raises(A.NdtError):
+        m.forward(np.zeros((100, 256), np.uint16))

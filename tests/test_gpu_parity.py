@@ -104,6 +104,28 @@ def test_batch_matches_single(A, orc, gpu_grid, cfg0):
     assert _compare_runs(rb, ref, 1e-3, 1e-4, orc) == []
 
 
+def test_schedulers_and_batch_composition_agree(A, gpu_grid, cfg0):
+    """Work queue and rounds reduce the same partial sums in the same order, so their
+    results are bit-identical (at two chunk sizes), and a scan's result does not
+    depend on the other scans of its batch."""
+    scene, mp, _ = cfg0
+    sizes = [700, 5000, 1300, 2600, 900, 4100, 1600, 2100, 3000]
+    scans = [A.make_scan(scene, mp, A.scan_cfg(n), 60 + k) for k, n in enumerate(sizes)]
+    pts = [s[0] for s in scans]
+    gs = np.stack([s[2] for s in scans])
+    runs = []
+    for sched in (A.NDT_SCHED_QUEUE, A.NDT_SCHED_ROUNDS):
+        for ppw in (0, 256):
+            prm = A.params(A.NDT_MODE_NDT, scheduler=sched, points_per_wg=ppw)
+            runs.append((sched, ppw, gpu_grid.align_batch(pts, gs, prm=prm)))
+    alone = gpu_grid.align_batch(pts[3:5], gs[3:5], prm=A.params(A.NDT_MODE_NDT))
+    for (sa, pa, ra), (sb, pb, rb) in [(runs[0], runs[2]), (runs[1], runs[3])]:
+        for x, y in zip(ra, rb):
+            assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"], (sa, sb, pa)
+    for x, y in zip(alone, runs[0][2][3:5]):
+        assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"]
+
+
 def test_deterministic(A, gpu_grid, cfg0):
     scene, mp, _ = cfg0
     scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(2000), 3)
@@ -224,3 +246,18 @@ def test_daphne_harness_end_to_end(tmp_path):
             (data / f).write_bytes(src.read())
     p = subprocess.run([exe, "-p", "2"], cwd=run, capture_output=True, text=True, timeout=120)
     assert p.returncode == 0 and "result ok" in p.stdout, p.stdout + p.stderr
+
+
+def test_fitness_score_matches_oracle(A, orc, ctx, gpu_grid, cfg0):
+    """getFitnessScore on the GPU (nearest map point by shell search over the
+    voxel-sorted map) against the oracle's brute force, at the truth, at the guess
+    and with a range limit."""
+    scene, mp, _ = cfg0
+    made = [A.make_scan(scene, mp, A.scan_cfg(n), 80 + k) for k, n in enumerate([1500, 400, 2500])]
+    sc = A.Scans(ctx, [m[0] for m in made])
+    for poses, rng in ((np.stack([m[1] for m in made]), 0.0), (np.stack([m[2] for m in made]), 0.0),
+                       (np.stack([m[2] for m in made]), 0.3)):
+        fit, cnt = sc.fitness(gpu_grid, poses, rng)
+        for k, m in enumerate(made):
+            want, n = orc.fitness_score(mp, m[0], poses[k], rng if rng > 0 else np.inf)
+            assert cnt[k] == n and abs(fit[k] - want) <= 1e-9 * want, (k, rng, fit[k], want, cnt[k], n)

@@ -145,3 +145,18 @@ def test_recovers_perturbed_guess(A, scene, orc):
         dt, dr = orc.pose_error(r["final_f64"], gt)
         ok += dt < 0.05 and dr < np.deg2rad(0.25) and r["converged"]
     assert ok >= 10
+
+
+def test_fitness_score_matches_kdtree(A, scene, orc):
+    """The oracle's fitness score (brute force) against scipy's k-d tree."""
+    from scipy.spatial import cKDTree
+    scn, mp = scene[0], scene[1]
+    scan, gt, guess = A.make_scan(scn, mp, A.scan_cfg(1500), 11)
+    tree = cKDTree(mp[:, :3].astype(np.float64))
+    for pose, rng in ((gt, np.inf), (guess, np.inf), (guess, 0.3)):
+        x = scan[:, :3].astype(np.float64) @ pose[:3, :3].T + pose[:3, 3]
+        d, _ = tree.query(x)
+        ok = d <= rng
+        want = float(np.mean(d[ok] ** 2))
+        got, n = orc.fitness_score(mp, scan, pose, rng)
+        assert n == ok.sum() and abs(got - want) <= 1e-12 * max(1.0, want), (got, want)
