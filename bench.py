@@ -1,17 +1,27 @@
-"""Benchmark: NDT scan alignments per second on one MI355X.
+"""Benchmark: NDT scan alignments per second on one MI355X, batched + learned warm
+start against plain single-scan NDT.
 
-Workload (default): a 1,000,000-point map of a 300 m x 300 m synthetic scene and a
-batch of 512 scans with point counts log-uniform in [1,000, 60,000], initial
-guesses perturbed by +-1 m, +-3 deg yaw, +-0.5 deg roll/pitch; NDT with 1 m
-voxels, step 0.1, epsilon 0.01, 30 iterations, outlier ratio 0.55.  The map's
-voxel grid is built once (reported, not timed per step); scans and guesses are
-resident in device memory before the timed region.  A step aligns the whole
-batch in one cooperative launch.  The same scans are also aligned one at a time
-("plain" single-scan GPU NDT) and the ratio is reported.
+Workload (default, the batched configuration): a 1,000,000-point map of a
+300 m x 300 m synthetic scene and a batch of 512 scans with point counts
+log-uniform in [1,000, 60,000], initial guesses perturbed by +-1 m, +-3 deg yaw,
++-0.5 deg roll/pitch; NDT with 1 m voxels, step 0.1, epsilon 0.01, 30 iterations,
+outlier ratio 0.55.  The map's voxel grid and the pose proposer are built before
+the timed region (both reported); scans are resident in device memory.
 
-Prints one JSON line (the benchmark contract).  Accuracy of the batch against the
-float64 CPU oracle is checked on a sample of scans, and the oracle's throughput on
-the host cores is reported as the CPU baseline.
+A step aligns the whole batch with the learned warm start: map and scan grids,
+the bf16 MLP forward pass, then one work-queue launch that runs NDT from the given
+and from the refined guess of every scan and keeps the better final score.  The
+same scans are also aligned batched without the proposer and one at a time
+("plain" single-scan GPU NDT, one launch and host sync per scan).
+
+The proposer is trained on the GPU before the timed region on 200,000 (scan,
+guess, NDT-converged pose) triples from scans of the same map at other sensor
+poses (guesses within +-3 m / +-10 deg yaw, or +-1 m / +-3 deg), and evaluated on
+the batch's held-out scans.
+
+Prints one JSON line (the benchmark contract).  The chosen runs are checked
+against the float64 CPU oracle started from the same guesses on a sample of
+scans, and the oracle's throughput on the host cores is the CPU baseline.
 """
 from __future__ import annotations
 
@@ -32,13 +42,56 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
 def algorithmic_work(results, counts, pairs):
-    """Per-pass work of the derivative kernel summed over the batch: bytes = 16 N
-    (scan point) + 36 P (voxel mean and inverse covariance per pair), flops =
-    133 N + 340 P (transform and point Jacobian per point, score / gradient /
-    Hessian per pair)."""
+    """Work of the derivative passes summed over a launch: bytes = 16 N (scan
+    point) + 36 P (voxel mean and inverse covariance per pair), flops = 133 N +
+    340 P (transform and point Jacobian per point, score / gradient / Hessian per
+    pair); N counts every point of every pass, P every (point, voxel) pair."""
     pts = float(sum(r["passes"] * n for r, n in zip(results, counts)))
     return 16.0 * pts + 36.0 * pairs, 133.0 * pts + 340.0 * pairs
+
+
+def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, seed=5):
+    """Training triples from scans of the workload's map at other sensor poses
+    (generator seeds disjoint from the batch's), labelled with NDT run from the
+    true pose; half the guesses within +-3 m / +-10 deg yaw, half within
+    +-1 m / +-3 deg."""
+    rng = np.random.default_rng(seed)
+    ds, t_gen, t_lab = None, 0.0, 0.0
+    chunk = 10000
+    for c0 in range(0, n_scans, chunk):
+        n = min(chunk, n_scans - c0)
+        t = time.time()
+        tr, tr_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, n, 99 + c0),
+                                    100000 + c0 + np.arange(n))
+        t_gen += time.time() - t
+        t = time.time()
+        trS = A.Scans(ctx, tr)
+        lab = trS.align(grid, tr_gt, prm)
+        t_lab += time.time() - t
+        labels = np.stack([r["final_f64"] for r in lab])
+        ok = np.array([r["converged"] and np.linalg.norm(r["final_f64"][:3, 3] - g[:3, 3]) < 0.2
+                       for r, g in zip(lab, tr_gt)])
+        idx = np.repeat(np.nonzero(ok)[0], per_scan).astype(np.int32)
+        wide = rng.uniform(size=len(idx)) < 0.5
+        g1 = A.perturb_poses(tr_gt[idx], rng)
+        g2 = A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0)
+        guesses = np.where(wide[:, None, None], g1, g2)
+        if ds is None:
+            ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
+        else:
+            ds.add(trS, idx, guesses, labels[idx])
+        del trS, tr
+    model = A.Model(ctx, seed=1)
+    t = time.time()
+    losses = model.train(ds, A.train_params(epochs=epochs, batch=512))
+    t_train = time.time() - t
+    return model, dict(samples=int(ds.n), scans=n_scans, epochs=epochs, generate_s=t_gen, label_s=t_lab,
+                       train_s=t_train, loss_first=float(losses[0]), loss_last=float(losses[-1]))
 
 
 def main():
@@ -52,6 +105,11 @@ def main():
     ap.add_argument("--plain-scans", type=int, default=128, help="scans timed one at a time (subset of the batch)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="scans aligned by the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="batched NDT only (no pose proposer)")
+    ap.add_argument("--train-scans", type=int, default=50000)
+    ap.add_argument("--train-per-scan", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=8)
+    ap.add_argument("--policy", choices=["race", "fallback"], default="race")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every box here has one GPU", file=sys.stderr)
@@ -61,6 +119,7 @@ def main():
     w = A.make_workload(args.workload, seed=1, scans=args.scans)
     gen_s = time.time() - t0
     counts = [s.shape[0] for s in w.scans]
+    S = len(w.scans)
     point_prec = A.NDT_POINT_FP64 if args.precision == "fp64" else A.NDT_POINT_FP32
     prm = A.params(A.NDT_MODE_NDT, point_precision=point_prec, origin=w.origin)
 
@@ -69,22 +128,40 @@ def main():
     grid = A.Grid(ctx, w.map_xyzw, prm)
     ginfo = grid.info()
     scans = A.Scans(ctx, w.scans)
+    log(f"workload {gen_s:.1f}s, grid {ginfo['build_ms']:.1f} ms")
 
+    # ---- batched NDT without the proposer (also the roofline measurement)
     for _ in range(args.warmup):
         scans.align(grid, w.guesses, prm)
-    kernel_ms, pairs_tot, results = [], 0, None
-    t_start = time.perf_counter()
+    kernel_ms, t_b = [], time.perf_counter()
     for _ in range(args.steps):
-        results = scans.align(grid, w.guesses, prm)
-        ms, launches, pairs = ctx.last_timing()
+        res_b = scans.align(grid, w.guesses, prm)
+        ms, launches, pairs_b = ctx.last_timing()
         kernel_ms.append(ms)
-        pairs_tot = pairs
-    elapsed = time.perf_counter() - t_start
-    S = len(w.scans)
-    value = S * args.steps / elapsed
+    batched_s = (time.perf_counter() - t_b) / args.steps
     profile = ctx.last_profile()
+    log(f"batched {S / batched_s:.0f}/s")
 
-    # plain single-scan GPU NDT on the same scans: one launch and host sync per scan
+    # ---- batched + learned warm start
+    learned = None
+    if not args.no_learned:
+        bm = A.BevMap(ctx, w.map_xyzw, w.origin)
+        model, tinfo = train_proposer(ctx, w, grid, bm, prm, args.train_scans, args.train_per_scan, args.epochs)
+        log(f"proposer trained: {tinfo}")
+        policy = A.LEARNED_RACE if args.policy == "race" else A.LEARNED_FALLBACK
+        for _ in range(args.warmup):
+            model.align(grid, bm, scans, w.guesses, prm, policy)
+        t_l = time.perf_counter()
+        for _ in range(args.steps):
+            res_l, used = model.align(grid, bm, scans, w.guesses, prm, policy)
+        learned_s = (time.perf_counter() - t_l) / args.steps
+        learned_ms = ctx.last_timing()[0]
+        refined = model.predict(bm, scans, w.guesses)
+        learned = dict(model=model, bm=bm, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
+                       refined=refined, train=tinfo, policy=args.policy)
+        log(f"batched+learned {S / learned_s:.0f}/s")
+
+    # ---- plain single-scan GPU NDT on the same scans: one launch and host sync per scan
     n_plain = max(1, min(args.plain_scans, S))
     plain_scans = [A.Scans(ctx, [w.scans[i]]) for i in range(n_plain)]
     for i in range(min(4, n_plain)):
@@ -93,19 +170,23 @@ def main():
     t1 = time.perf_counter()
     for i in range(n_plain):
         plain_res += plain_scans[i].align(grid, w.guesses[i:i + 1], prm)
-    plain_s = time.perf_counter() - t1
-    plain_rate = n_plain / plain_s
-    batch_sub_agree = sum(
-        np.array_equal(a["final_f64"], b["final_f64"]) or (
-            np.abs(a["final_f64"] - b["final_f64"]).max() < 1e-6) for a, b in zip(plain_res, results[:n_plain]))
+    plain_rate = n_plain / (time.perf_counter() - t1)
+    same_as_batch = sum(np.array_equal(a["final_f64"], b["final_f64"]) for a, b in zip(plain_res, res_b[:n_plain]))
 
-    bytes_alg, flops_alg = algorithmic_work(results, counts, pairs_tot)
+    def success(R):
+        return float(np.mean([np.linalg.norm(R[i]["final_f64"][:3, 3] - w.gt[i][:3, 3]) < 0.05 and
+                              abs(np.arctan2(-R[i]["final_f64"][0, 1], R[i]["final_f64"][0, 0]) -
+                                  np.arctan2(-w.gt[i][0, 1], w.gt[i][0, 0])) < np.deg2rad(0.25)
+                              for i in range(len(R))]))
+
+    bytes_alg, flops_alg = algorithmic_work(res_b, counts, pairs_b)
     avg_ms = float(np.mean(kernel_ms))
     achieved_gbs = bytes_alg / (avg_ms * 1e-3) / 1e9
     peak_flops = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
     valu_frac = flops_alg / (avg_ms * 1e-3) / (peak_flops * 1e12)
-    conv = float(np.mean([r["converged"] for r in results]))
-    conv_plain = float(np.mean([r["converged"] for r in plain_res]))
+    final = learned["res"] if learned else res_b
+    value = S / (learned["step_s"] if learned else batched_s)
+    ws = w.meta
 
     out = {
         "metric": "scan alignments/sec at <=1 cm/0.05 deg vs FP64: batched+learned vs plain NDT",
@@ -114,31 +195,36 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": S / value * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f32",
-        "data": "synthetic (seeded scene generator, seed 1)",
+        "data": "synthetic (seeded scene generator, seed 1); proposer trained on synthetic scans of the same map",
         "config": {
-            "workload": f"{args.workload}: {w.map_xyzw.shape[0]}-point map of a {w.meta['extent']:.0f} m scene, "
+            "workload": f"{args.workload}: {w.map_xyzw.shape[0]}-point map of a {ws['extent']:.0f} m scene, "
                         f"{S} scans of {min(counts)}-{max(counts)} points (log-uniform), guesses "
-                        f"+-{w.meta['perturb_xyz']} m / +-{w.meta['perturb_yaw_deg']} deg yaw / "
-                        f"+-{w.meta['perturb_rollpitch_deg']} deg roll-pitch; batched, no learned warm start",
+                        f"+-{ws['perturb_xyz']} m / +-{ws['perturb_yaw_deg']} deg yaw / "
+                        f"+-{ws['perturb_rollpitch_deg']} deg roll-pitch; "
+                        + (f"batched + learned warm start ({args.policy})" if learned else "batched, no proposer"),
             "scans_per_step": S,
             "scan_points_total": int(sum(counts)),
             "map_points": int(w.map_xyzw.shape[0]),
             "voxel": 1.0, "step": 0.1, "epsilon": 0.01, "max_iterations": 30, "outlier_ratio": 0.55,
         },
         "plain_single_scan": {"value": plain_rate, "unit": "alignments/s", "scans": n_plain,
-                              "converged": conv_plain, "same_result_as_batch": int(batch_sub_agree)},
-        "batched_vs_plain": value / plain_rate,
-        "converged_fraction": conv,
-        "mean_iterations": float(np.mean([r["iterations"] for r in results])),
-        "mean_passes": float(np.mean([r["passes"] for r in results])),
+                              "same_result_as_batched": int(same_as_batch)},
+        "batched_no_proposer": {"value": S / batched_s, "unit": "alignments/s", "kernel_ms": avg_ms,
+                                "launches": launches},
+        "vs_plain_single_scan": value / plain_rate,
+        "converged_fraction": float(np.mean([r["converged"] for r in final])),
+        "converged_fraction_batched_plain": float(np.mean([r["converged"] for r in res_b])),
+        "success_vs_truth": {"criterion": "within 5 cm and 0.25 deg yaw of the generator's true pose",
+                             "plain": success(res_b), "chosen": success(final)},
+        "mean_passes": float(np.mean([r["passes"] for r in final])),
+        "mean_passes_batched_plain": float(np.mean([r["passes"] for r in res_b])),
         "grid_build_ms": ginfo["build_ms"],
-        "align_kernel_ms": avg_ms,
-        "profile_us_per_scan": {k: float(v / S) for k, v in zip(("params", "compute", "wait", "control"), profile)},
+        "profile_us_batched_plain": {k: float(v) for k, v in zip(("lists", "compute", "wait", "control"), profile)},
         "roofline": {
             "bound": "hbm",
             "achieved": achieved_gbs,
@@ -147,11 +233,15 @@ def main():
             "frac": achieved_gbs / HBM_PEAK_GBS,
             "traffic": None,
             "valu_frac": valu_frac,
-            "kernel": "ndt::k_align (persistent cooperative align, all passes of the batch)",
+            "kernel": "ndt::k_align_wq (work-queue align launch of the 512-scan batch, no proposer)",
         },
         "device": f"{dev_name}, {cus} CUs",
         "generation_s": gen_s,
     }
+    if learned:
+        out["learned"] = dict(learned["train"], policy=learned["policy"],
+                              used_refined=float(np.mean(learned["used"])),
+                              device_ms_per_step=learned["kernel_ms"])
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -159,42 +249,41 @@ def main():
         og = orc.NdtGrid(w.map_xyzw)
         rng = np.random.default_rng(0)
         idx = np.sort(rng.choice(S, size=min(args.cpu_sample, S), replace=False))
-        local_guess = w.guesses[idx].copy()
-        local_guess[:, :3, 3] -= np.array(w.origin)
+        # the oracle starts where the chosen GPU run started
+        starts = np.stack([learned["refined"][i] if (learned and learned["used"][i]) else w.guesses[i] for i in idx])
+        local = starts.copy()
+        local[:, :3, 3] -= np.array(w.origin)
         threads = os.cpu_count() or 1
         tc = time.perf_counter()
-        ref = og.align_many([w.scans[i] for i in idx], local_guess, threads=threads)
+        ref = og.align_many([w.scans[i] for i in idx], local, threads=threads)
         cpu_s = time.perf_counter() - tc
-        # the oracle's own reproducibility: the same scans from a guess shifted by 1e-12 m
-        shifted = local_guess.copy()
+        shifted = local.copy()
         shifted[:, 0, 3] += 1e-12
         ref2 = og.align_many([w.scans[i] for i in idx], shifted, threads=threads)
         errs = []
         for k, i in enumerate(idx):
             o = ref[k]["final_f64"].copy()
             o[:3, 3] += np.array(w.origin)
-            dt, dr = orc.pose_error(results[i]["final_f64"], o)
-            st = orc.pose_error(ref[k]["final_f64"], ref2[k]["final_f64"])[0] < 1e-4
-            errs.append((dt, dr, results[i]["iterations"] == ref[k]["iterations"],
-                         results[i]["converged"] == ref[k]["converged"], st))
+            chosen = final[i]
+            dt, dr = orc.pose_error(chosen["final_f64"], o)
+            stable = orc.pose_error(ref[k]["final_f64"], ref2[k]["final_f64"])[0] < 1e-4
+            errs.append((dt, dr, chosen["converged"] == ref[k]["converged"], stable))
         errs = np.array(errs, dtype=np.float64)
-        stable = errs[:, 4] > 0
-        strict = (errs[:, 0] <= 1e-3) & (errs[:, 1] <= 1e-4) & (errs[:, 2] > 0) & (errs[:, 3] > 0)
+        ok = (errs[:, 0] <= 0.01) & (errs[:, 1] <= np.deg2rad(0.05))
         out["cpu_baseline"] = {
             "value": len(idx) / cpu_s, "unit": "alignments/s", "cores": threads, "kind": "port",
-            "sample": f"{len(idx)} scans of the same batch ({int(sum(counts[i] for i in idx))} points), float64 "
-                      f"oracle (oracle/ndt_oracle.c), one scan per OpenMP thread",
+            "sample": f"{len(idx)} scans of the same batch ({int(sum(counts[i] for i in idx))} points) from the "
+                      f"same starting guesses, float64 oracle (oracle/ndt_oracle.c), one scan per OpenMP thread",
         }
         out["accuracy_vs_fp64_oracle"] = {
             "scans": len(idx),
-            "within_1cm_0.05deg": int(np.sum((errs[:, 0] <= 0.01) & (errs[:, 1] <= np.deg2rad(0.05)))),
+            "within_1cm_0.05deg": int(ok.sum()),
             "within_1mm_1e-4rad": int(np.sum((errs[:, 0] <= 1e-3) & (errs[:, 1] <= 1e-4))),
             "max_translation_m": float(errs[:, 0].max()),
             "max_rotation_rad": float(errs[:, 1].max()),
-            "same_iterations": int(errs[:, 2].sum()),
-            "same_converged": int(errs[:, 3].sum()),
-            "oracle_reproducible_scans": int(stable.sum()),
-            "strict_parity_on_reproducible": int(np.sum(strict & stable)),
+            "same_converged": int(errs[:, 2].sum()),
+            "oracle_reproducible_scans": int(errs[:, 3].sum()),
+            "within_1cm_0.05deg_on_reproducible": int(np.sum(ok & (errs[:, 3] > 0))),
         }
     print(json.dumps(out))
 

@@ -299,12 +299,14 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     int64_t cap = 1ll << log2cap;
     BUF_TRY(g->slots, sizeof(HashSlot) * cap);
     LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
-    BUF_TRY(g->cand, sizeof(float4) * (ne + 1));
+    BUF_TRY(g->cand32, sizeof(CandRec32) * (ne + 1));
+    BUF_TRY(g->cand64, sizeof(CandRec64) * (ne + 1));
     if (nq > 0) {
         LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, 5, g->slots.as<HashSlot>(),
                                       log2cap, c->stream));
-        LAUNCH_TRY(launch_cand_fill(keys.as<uint64_t>(), vals.as<uint32_t>(), ne, g->mean_c.as<double>(), g->bits,
-                                    g->origin_cell, (double)g->params.resolution, g->cand.as<float4>(), c->stream));
+        LAUNCH_TRY(launch_cand_fill(keys.as<uint64_t>(), vals.as<uint32_t>(), ne, g->mean_c.as<double>(),
+                                    g->icov_c.as<double>(), g->bits, g->origin_cell, (double)g->params.resolution,
+                                    g->cand32.as<CandRec32>(), g->cand64.as<CandRec64>(), c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     g->info.query_cells = nq;
@@ -312,10 +314,8 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     g->info.hash_capacity = cap;
     NdtGridDev& d = g->dev;
     d.slots = g->slots.as<HashSlot>();
-    d.cand = g->cand.as<float4>();
-    d.icov_f = g->icov_f.as<float4>();
-    d.mean_d = g->mean_c.as<double>();
-    d.icov_d = g->icov_c.as<double>();
+    d.cand32 = g->cand32.as<CandRec32>();
+    d.cand64 = g->cand64.as<CandRec64>();
     d.log2cap = log2cap;
     for (int k = 0; k < 3; k++) {
         d.bits[k] = g->bits[k];
@@ -534,10 +534,10 @@ static uint64_t queue_capacity(int64_t total_chunks, int32_t max_iterations) {
 constexpr uint64_t MAX_QUEUE_ITEMS = 1ull << 28;
 
 /* Aligns scans already packed in `pts` (offsets/counts) with guesses on the host.
- * The chunk size (points per partial sum) is a constant unless the caller sets
- * points_per_wg, and partial sums are reduced in a fixed order, so a scan's result
- * does not depend on the other scans of the batch, on the number of workgroups
- * the device runs, nor on the scheduler.  The work-queue scheduler runs the whole
+ * The chunk size (points per partial sum) depends only on the scan's own size
+ * unless the caller sets points_per_wg, and partial sums are reduced in a fixed
+ * order, so a scan's result does not depend on the other scans of the batch, on
+ * the number of workgroups the device runs, nor on the scheduler.  The work-queue scheduler runs the whole
  * batch in one launch; the round scheduler runs groups of up to MAX_BATCH_SCANS
  * scans and is also used when the queue would exceed MAX_QUEUE_ITEMS. */
 static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
@@ -547,12 +547,14 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
     const int block = p->block_size ? p->block_size : 256;
     AlignParams prm = make_align_params(p, S);
-    const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : ALIGN_CHUNK_POINTS;
     std::vector<ScanDesc> desc(S);
     int64_t slots = 0, max_chunks = 0;
     for (int32_t s = 0; s < S; s++) {
+        const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : align_chunk_points(counts[s]);
         desc[s].offset = offsets[s];
         desc[s].n = (int32_t)counts[s];
+        desc[s].chunk = (int32_t)chunk;
+        desc[s].pad = 0;
         desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
         desc[s].chunk_base = slots;
         slots += desc[s].nchunks;
@@ -592,7 +594,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         LAUNCH_TRY(launch_init_queue(c->desc.as<ScanDesc>(), S, (uint64_t)slots, c->queue.as<WorkQueue>(),
                                      c->items.as<uint64_t>(), c->done.as<uint32_t>(), c->stream));
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
-        LAUNCH_TRY(launch_align_wq(g->dev, pts, c->desc.as<ScanDesc>(), (int32_t)chunk, nwg, block, prm,
+        LAUNCH_TRY(launch_align_wq(g->dev, pts, c->desc.as<ScanDesc>(), nwg, block, prm,
                                    c->states.as<ScanState>(), c->queue.as<WorkQueue>(), c->items.as<uint64_t>(), cap,
                                    c->done.as<uint32_t>(), c->partials.as<double>(), c->out.as<ScanOut>(),
                                    c->small.as<double>(), c->stream));
@@ -611,7 +613,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
             LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0, n, prm,
                                           c->states.as<ScanState>() + s0, rs, c->stream));
             HIP_TRY(hipEventRecord(c->ev0, c->stream));
-            LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, (int32_t)chunk, nwg, block, prm,
+            LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, nwg, block, prm,
                                     c->states.as<ScanState>() + s0, rs, c->partials.as<double>(),
                                     c->out.as<ScanOut>() + s0, c->small.as<double>(), c->stream));
             HIP_TRY(hipEventRecord(c->ev1, c->stream));

@@ -222,12 +222,31 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
 }
 
 /* Device view of an ndt-mode grid. */
+/* Candidate records: one per (query cell, valid voxel within reach), the lists of
+ * a query cell contiguous in probe order.  Each carries the voxel's mean relative
+ * to the query cell's lower corner (float32, for the range test) and the voxel's
+ * statistics, so that a scan point reaches everything it needs through its hash
+ * slot alone. */
+struct alignas(16) CandRec32 {
+    float rx, ry, rz;        /* mean - query cell corner */
+    uint32_t vid;            /* valid-voxel id */
+    float c00, c01, c02, c11;
+    float c12, c22, pad0, pad1;
+};
+struct alignas(16) CandRec64 {
+    float rx, ry, rz;
+    uint32_t vid;
+    double mx, my, mz;       /* mean, map-local */
+    double c00, c01, c02, c11, c12, c22;
+    double pad;
+};
+static_assert(sizeof(CandRec32) == 48, "CandRec32 layout");
+static_assert(sizeof(CandRec64) == 96, "CandRec64 layout");
+
 struct NdtGridDev {
     const HashSlot* slots;   /* query cell -> candidate list */
-    const float4* cand;      /* candidates: (mean - query cell corner, voxel id bits), lists in probe order */
-    const float4* icov_f;    /* 2 per voxel: (c00 c01 c02 c11) (c12 c22 - -) */
-    const double* mean_d;    /* 3 per voxel */
-    const double* icov_d;    /* 6 per voxel: c00 c01 c02 c11 c12 c22 */
+    const CandRec32* cand32; /* candidate records, float32 statistics */
+    const CandRec64* cand64; /* the same records with float64 statistics */
     int log2cap;
     int bits[3];             /* key bit widths x, y, z */
     int64_t origin_cell[3];  /* cell coordinates of key 0 */
@@ -290,7 +309,9 @@ struct ScanDesc {
     int64_t offset;     /* first point in the packed point buffer */
     int64_t chunk_base; /* first partial-sum slot of this scan */
     int32_t n;          /* points */
-    int32_t nchunks;    /* chunks of the launch's chunk size */
+    int32_t nchunks;    /* chunks of `chunk` points */
+    int32_t chunk;      /* points per partial sum */
+    int32_t pad;
 };
 
 struct AlignParams {
@@ -380,7 +401,8 @@ int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int6
                        int shift, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
-                     const int* bits, const int64_t* origin_cell, double res, float4* cand, hipStream_t stream);
+                     const double* icov, const int* bits, const int64_t* origin_cell, double res, CandRec32* c32,
+                     CandRec64* c64, hipStream_t stream);
 int launch_hash_clear(HashSlot* slots, int64_t cap, hipStream_t stream);
 int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
                               const uint32_t* seg_start, int64_t nseg, int64_t npts, double ncells,
@@ -392,7 +414,7 @@ int launch_fitness(FitGridDev fg, const float4* pts, const int64_t* offsets, con
                    const double* poses, double max_d2, int shells, int nblk, double* partials, hipStream_t stream);
 int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, WorkQueue* q, uint64_t* items,
                       uint32_t* done_cnt, hipStream_t stream);
-int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg, int block_size,
+int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg, int block_size,
                     AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
                     double* partials, ScanOut* out, double* prof, hipStream_t stream);
 int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
@@ -400,11 +422,15 @@ int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanSt
                        hipStream_t stream);
 int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
 constexpr int MAX_BATCH_SCANS = 2048;
-/* scan points per partial sum of the align kernel (ndt_params.points_per_wg = 0) */
-constexpr int ALIGN_CHUNK_POINTS = 512;
+/* Scan points per partial sum of the align kernels when ndt_params.points_per_wg
+ * is 0.  A constant, so that a scan's result never depends on the batch it is
+ * aligned in; 2048 maximises batched throughput (256 minimises the latency of a
+ * single large scan, at a different rounding of its sums). */
+constexpr int32_t ALIGN_CHUNK_POINTS = 2048;
+NDT_HD int32_t align_chunk_points(int64_t /*n*/) { return ALIGN_CHUNK_POINTS; }
 /* prof receives 5 doubles: phase microseconds (active list, compute, barriers,
  * control) of workgroup 0 and an abort flag */
-int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk, int32_t nwg,
+int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t nwg,
                  int block_size, AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out,
                  double* prof, hipStream_t stream);
 

@@ -641,13 +641,14 @@ int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, 
     return (int)hipGetLastError();
 }
 
-/* Candidate entry i of the sorted dilation keys: the voxel's mean relative to the
- * lower corner of the query cell the entry belongs to, in float32, and the voxel
- * id.  Relative to the cell the values lie in [-res, 2 res), so float32 keeps them
- * to about 1e-7 m at any map extent. */
+/* Candidate record i of the sorted dilation keys (query cell << 5 | probe): the
+ * voxel's mean relative to the lower corner of the query cell, in float32 (values
+ * in [-res, 2 res), kept to about 1e-7 m at any map extent), the voxel id, and the
+ * voxel's mean and inverse covariance in float32 and float64. */
 __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t n,
-                            const double* __restrict__ mean, Bits3 B, int64_t o0, int64_t o1, int64_t o2, double res,
-                            float4* __restrict__ cand) {
+                            const double* __restrict__ mean, const double* __restrict__ icov, Bits3 B, int64_t o0,
+                            int64_t o1, int64_t o2, double res, CandRec32* __restrict__ c32,
+                            CandRec64* __restrict__ c64) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = svals[i];
@@ -655,16 +656,48 @@ __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* 
     const int64_t c0 = (int64_t)(q & ((1ull << B.b[0]) - 1)) + o0;
     const int64_t c1 = (int64_t)((q >> B.b[0]) & ((1ull << B.b[1]) - 1)) + o1;
     const int64_t c2 = (int64_t)(q >> (B.b[0] + B.b[1])) + o2;
-    cand[i] = make_float4((float)(mean[3 * v] - (double)c0 * res), (float)(mean[3 * v + 1] - (double)c1 * res),
-                          (float)(mean[3 * v + 2] - (double)c2 * res), __uint_as_float(v));
+    const double* m = mean + 3 * (int64_t)v;
+    const double* C = icov + 6 * (int64_t)v;
+    const float rx = (float)(m[0] - (double)c0 * res), ry = (float)(m[1] - (double)c1 * res),
+                rz = (float)(m[2] - (double)c2 * res);
+    CandRec32 a;
+    a.rx = rx;
+    a.ry = ry;
+    a.rz = rz;
+    a.vid = v;
+    a.c00 = (float)C[0];
+    a.c01 = (float)C[1];
+    a.c02 = (float)C[2];
+    a.c11 = (float)C[3];
+    a.c12 = (float)C[4];
+    a.c22 = (float)C[5];
+    a.pad0 = a.pad1 = 0.0f;
+    c32[i] = a;
+    CandRec64 d;
+    d.rx = rx;
+    d.ry = ry;
+    d.rz = rz;
+    d.vid = v;
+    d.mx = m[0];
+    d.my = m[1];
+    d.mz = m[2];
+    d.c00 = C[0];
+    d.c01 = C[1];
+    d.c02 = C[2];
+    d.c11 = C[3];
+    d.c12 = C[4];
+    d.c22 = C[5];
+    d.pad = 0.0;
+    c64[i] = d;
 }
 
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
-                     const int* bits, const int64_t* origin_cell, double res, float4* cand, hipStream_t stream) {
+                     const double* icov, const int* bits, const int64_t* origin_cell, double res, CandRec32* c32,
+                     CandRec64* c64, hipStream_t stream) {
     Bits3 B;
     for (int k = 0; k < 3; k++) B.b[k] = bits[k];
     hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_keys, sorted_vals,
-                       nentries, mean, B, origin_cell[0], origin_cell[1], origin_cell[2], res, cand);
+                       nentries, mean, icov, B, origin_cell[0], origin_cell[1], origin_cell[2], res, c32, c64);
     return (int)hipGetLastError();
 }
 
@@ -926,9 +959,11 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
      * fp64 mode, where the exact test below decides. */
     const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
     uint32_t mask = 0;
+    const float4* heads = (sizeof(T) == 4) ? (const float4*)(g.cand32 + begin) : (const float4*)(g.cand64 + begin);
+    constexpr int HSTRIDE = (sizeof(T) == 4 ? sizeof(CandRec32) : sizeof(CandRec64)) / sizeof(float4);
 #pragma unroll 4
     for (uint32_t k = 0; k < count; k++) {
-        const float4 cd = g.cand[begin + k];
+        const float4 cd = heads[k * HSTRIDE];
         const float e0 = r0 - cd.x, e1 = r1 - cd.y, e2 = r2 - cd.z;
         mask |= (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
     }
@@ -936,27 +971,24 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     while (mask) {
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1;
-        const float4 cd = g.cand[begin + k];
-        const uint32_t v = __float_as_uint(cd.w);
         T d0, d1, d2;
-        if constexpr (sizeof(T) == 4) {
-            d0 = r0 - cd.x;
-            d1 = r1 - cd.y;
-            d2 = r2 - cd.z;
-            if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
-        } else {
-            d0 = xd0 - g.mean_d[3 * v];
-            d1 = xd1 - g.mean_d[3 * v + 1];
-            d2 = xd2 - g.mean_d[3 * v + 2];
-            if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) continue;
-        }
         T c00, c01, c02, c11, c12, c22;
         if constexpr (sizeof(T) == 4) {
-            const float4 a = g.icov_f[2 * v], b = g.icov_f[2 * v + 1];
+            const float4* r = (const float4*)(g.cand32 + begin + k);
+            const float4 h = r[0], a = r[1], b = r[2];
+            d0 = r0 - h.x;
+            d1 = r1 - h.y;
+            d2 = r2 - h.z;
+            if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
             c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
         } else {
-            const double* C = g.icov_d + 6 * v;
-            c00 = C[0]; c01 = C[1]; c02 = C[2]; c11 = C[3]; c12 = C[4]; c22 = C[5];
+            const double2* r = (const double2*)(g.cand64 + begin + k);
+            const double2 m01 = r[1], m2c0 = r[2], c12v = r[3], c34 = r[4], c5p = r[5];
+            d0 = xd0 - m01.x;
+            d1 = xd1 - m01.y;
+            d2 = xd2 - m2c0.x;
+            if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) continue;
+            c00 = m2c0.y; c01 = c12v.x; c02 = c12v.y; c11 = c34.x; c12 = c34.y; c22 = c5p.x;
         }
         /* q = C x' */
         const T q0 = c00 * d0 + c01 * d1 + c02 * d2;
@@ -1405,7 +1437,7 @@ __device__ inline void reduce_slots(const double* partials, int64_t base, int nc
 
 template <typename T, int SUM, int BLOCK>
 __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(
-    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t S, int32_t chunk,
+    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t S,
     AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out, double* prof_out) {
     __shared__ PassParams<T> pp;
     __shared__ double lds_red[BLOCK / WAVE][NRED];
@@ -1469,8 +1501,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             int pairs = 0;
             {
                 const PassParams<T> u = load_uniform(pp);
-                const int64_t pb = sd.offset + k * chunk;
-                const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * (int64_t)chunk);
+                const int64_t pb = sd.offset + k * sd.chunk;
+                const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * (int64_t)sd.chunk);
                 for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
             }
             double v[NRED];
@@ -1517,10 +1549,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
 }
 
 template <typename T, int SUM, int BLOCK>
-static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk,
+static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t S,
                           int32_t nwg, AlignParams prm, ScanState* states, RoundSync* rs, double* partials,
                           ScanOut* out, double* prof, hipStream_t stream) {
-    void* args[] = {&g, &pts, &scans, &S, &chunk, &prm, &states, &rs, &partials, &out, &prof};
+    void* args[] = {&g, &pts, &scans, &S, &prm, &states, &rs, &partials, &out, &prof};
     return (int)hipLaunchCooperativeKernel((const void*)k_align<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
                                            stream);
 }
@@ -1558,12 +1590,12 @@ int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* b
 #undef NDT_MB
 }
 
-int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t chunk, int32_t nwg,
+int launch_align(NdtGridDev grid, const float4* pts, const ScanDesc* scans, int32_t S, int32_t nwg,
                  int block_size, AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out,
                  double* prof, hipStream_t stream) {
     if (S > MAX_BATCH) return (int)hipErrorInvalidValue;
 #define NDT_LA(T, S_, B) \
-    launch_align_t<T, S_, B>(grid, pts, scans, S, chunk, nwg, prm, states, rs, partials, out, prof, stream)
+    launch_align_t<T, S_, B>(grid, pts, scans, S, nwg, prm, states, rs, partials, out, prof, stream)
     NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
 #undef NDT_LA
 }
@@ -1611,7 +1643,7 @@ int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, W
 
 template <typename T, int SUM, int BLOCK>
 __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align_wq(
-    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t chunk, AlignParams prm,
+    NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, AlignParams prm,
     ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt, double* partials,
     ScanOut* out, double* prof_out) {
     __shared__ PassParams<T> pp;
@@ -1668,8 +1700,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         int pairs = 0;
         {
             const PassParams<T> u = load_uniform(pp);
-            const int64_t pb = sd.offset + (int64_t)k * chunk;
-            const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * chunk);
+            const int64_t pb = sd.offset + (int64_t)k * sd.chunk;
+            const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * sd.chunk);
             for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
         }
         double v[NRED];
@@ -1738,10 +1770,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
 }
 
 template <typename T, int SUM, int BLOCK>
-static int launch_align_wq_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg,
+static int launch_align_wq_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg,
                              AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap,
                              uint32_t* done_cnt, double* partials, ScanOut* out, double* prof, hipStream_t stream) {
-    void* args[] = {&g, &pts, &scans, &chunk, &prm, &states, &q, &items, &cap, &done_cnt, &partials, &out, &prof};
+    void* args[] = {&g, &pts, &scans, &prm, &states, &q, &items, &cap, &done_cnt, &partials, &out, &prof};
     return (int)hipLaunchCooperativeKernel((const void*)k_align_wq<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
                                            stream);
 }
@@ -1757,11 +1789,11 @@ int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* block
 #undef NDT_MB
 }
 
-int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t chunk, int32_t nwg, int block_size,
+int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg, int block_size,
                     AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
                     double* partials, ScanOut* out, double* prof, hipStream_t stream) {
 #define NDT_LA(T, S_, B) \
-    launch_align_wq_t<T, S_, B>(g, pts, scans, chunk, nwg, prm, states, q, items, cap, done_cnt, partials, out, prof, stream)
+    launch_align_wq_t<T, S_, B>(g, pts, scans, nwg, prm, states, q, items, cap, done_cnt, partials, out, prof, stream)
     NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
 #undef NDT_LA
 }

@@ -119,8 +119,8 @@ struct ndt_dataset {
     ndt_ctx* ctx = nullptr;
     const ndt_bevmap* bm = nullptr;
     int64_t n = 0;
-    int32_t nscans = 0;
-    DevBuf scan_bev;
+    int64_t nscans = 0;
+    DevBuf scan_bev; /* nscans grids; capacity grows by doubling */
     std::vector<int32_t> scan_index;
     std::vector<double> guesses;  /* map-local, 16 per sample */
     std::vector<float> targets;   /* scaled corrections, 6 per sample */
@@ -470,7 +470,7 @@ static int backward_dev(ndt_model* md, int B) {
     return 0;
 }
 
-static int adam_dev(ndt_model* md, float lr, float b1, float b2, float eps) {
+static int adam_dev(ndt_model* md, float lr, float b1, float b2, float eps, float wd) {
     hipStream_t st = md->ctx->stream;
     md->step++;
     const float bc1 = (float)(1.0 - std::pow((double)b1, (double)md->step));
@@ -480,8 +480,9 @@ static int adam_dev(ndt_model* md, float lr, float b1, float b2, float eps) {
                        : k == NDT_W2 ? md->wb2.as<uint16_t>()
                        : k == NDT_W3 ? md->wb3.as<uint16_t>()
                                      : nullptr;
+        const float wdk = (k == NDT_W1 || k == NDT_W2 || k == NDT_W3) ? wd : 0.0f;
         LAUNCH_TRY(launch_adam(md->w[k].as<float>(), md->mom[k].as<float>(), md->vel[k].as<float>(),
-                               md->g[k].as<float>(), tensor_size(md, k), lr, b1, b2, eps, bc1, bc2, wb, st));
+                               md->g[k].as<float>(), tensor_size(md, k), lr, b1, b2, eps, bc1, bc2, wdk, wb, st));
     }
     LAUNCH_TRY(launch_transpose_u16(md->wb2.as<uint16_t>(), md->f2, md->f1, md->wb2t.as<uint16_t>(), st));
     return 0;
@@ -531,10 +532,11 @@ extern "C" int ndt_model_backward(ndt_ctx* c, ndt_model* md, const uint16_t* x, 
     return 0;
 }
 
-extern "C" int ndt_model_adam_step(ndt_ctx* c, ndt_model* md, float lr, float beta1, float beta2, float eps) {
+extern "C" int ndt_model_adam_step(ndt_ctx* c, ndt_model* md, float lr, float beta1, float beta2, float eps,
+                                   float weight_decay) {
     if (!c || !md) return fail(NDT_ERR_INVALID, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    int e = adam_dev(md, lr, beta1, beta2, eps);
+    int e = adam_dev(md, lr, beta1, beta2, eps, weight_decay);
     if (e) return e;
     HIP_TRY(hipStreamSynchronize(c->stream));
     return 0;
@@ -625,38 +627,79 @@ extern "C" void ndt_train_params_default(ndt_train_params* tp) {
     tp->eps = 1e-8f;
     tp->seed = 1;
     for (int k = 0; k < 6; k++) tp->weights[k] = 1.0f;
+    tp->weight_decay = 0.0f;
+    tp->cosine = 1;
+}
+
+/* appends the grids of every scan of `sc` to the dataset's grid store */
+static int dataset_add_grids(ndt_dataset* ds, const ndt_scans* sc) {
+    const size_t row = 2 * (size_t)BEV_CELLS;
+    const size_t need = row * (size_t)(ds->nscans + sc->S);
+    if (need > ds->scan_bev.bytes) {
+        DevBuf grown;
+        BUF_TRY(grown, std::max(need, 2 * ds->scan_bev.bytes));
+        if (ds->nscans)
+            HIP_TRY(hipMemcpyAsync(grown.p, ds->scan_bev.p, row * (size_t)ds->nscans, hipMemcpyDeviceToDevice,
+                                   ds->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ds->ctx->stream));
+        std::swap(grown.p, ds->scan_bev.p);
+        std::swap(grown.bytes, ds->scan_bev.bytes);
+    }
+    ndt_model tmp;
+    tmp.ctx = ds->ctx;
+    DevBuf dst; /* a view of the store's tail */
+    int e = scan_grids(&tmp, ds->bm, sc, dst);
+    if (e) return e;
+    HIP_TRY(hipMemcpyAsync((char*)ds->scan_bev.p + row * (size_t)ds->nscans, dst.p, row * (size_t)sc->S,
+                           hipMemcpyDeviceToDevice, ds->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ds->ctx->stream));
+    return 0;
+}
+
+extern "C" int ndt_dataset_add(ndt_dataset* ds, const ndt_scans* sc, const int32_t* scan_index, const double* guesses,
+                               const double* targets, int64_t n) {
+    if (!ds || !sc || !scan_index || !guesses || !targets) return fail(NDT_ERR_INVALID, "null argument");
+    if (n <= 0) return fail(NDT_ERR_INVALID, "no samples");
+    if (ds->nscans + sc->S > 0x7fffffff) return fail(NDT_ERR_UNSUPPORTED, "too many scans in one dataset");
+    for (int64_t i = 0; i < n; i++)
+        if (scan_index[i] < 0 || scan_index[i] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
+    HIP_TRY(hipSetDevice(ds->ctx->device));
+    int e = dataset_add_grids(ds, sc);
+    if (e) return e;
+    std::vector<double> gl;
+    guesses_local(ds->bm, guesses, n, gl);
+    for (int64_t i = 0; i < n; i++) {
+        ds->scan_index.push_back((int32_t)(ds->nscans + scan_index[i]));
+        ds->guesses.insert(ds->guesses.end(), gl.begin() + 16 * i, gl.begin() + 16 * (i + 1));
+        double p6[6];
+        correction_params(guesses + 16 * i, targets + 16 * i, p6);
+        for (int k = 0; k < 6; k++) ds->targets.push_back((float)(p6[k] / CORR_SCALE[k]));
+    }
+    ds->nscans += sc->S;
+    ds->n += n;
+    return 0;
 }
 
 extern "C" int ndt_dataset_create(ndt_ctx* c, const ndt_bevmap* bm, const ndt_scans* sc, const int32_t* scan_index,
                                   const double* guesses, const double* targets, int64_t n, ndt_dataset** out) {
     if (!c || !bm || !sc || !scan_index || !guesses || !targets || !out) return fail(NDT_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (n <= 0) return fail(NDT_ERR_INVALID, "empty dataset");
-    for (int64_t i = 0; i < n; i++)
-        if (scan_index[i] < 0 || scan_index[i] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
-    HIP_TRY(hipSetDevice(c->device));
     ndt_dataset* ds = new ndt_dataset();
     ds->ctx = c;
     ds->bm = bm;
-    ds->n = n;
-    ds->nscans = sc->S;
-    ds->scan_index.assign(scan_index, scan_index + n);
-    guesses_local(bm, guesses, n, ds->guesses);
-    ds->targets.resize((size_t)(6 * n));
-    for (int64_t i = 0; i < n; i++) {
-        double p6[6];
-        correction_params(guesses + 16 * i, targets + 16 * i, p6);
-        for (int k = 0; k < 6; k++) ds->targets[(size_t)(6 * i + k)] = (float)(p6[k] / CORR_SCALE[k]);
-    }
-    /* scan grids, computed once */
-    ndt_model tmp;
-    tmp.ctx = c;
-    int e = scan_grids(&tmp, bm, sc, ds->scan_bev);
+    int e = ndt_dataset_add(ds, sc, scan_index, guesses, targets, n);
     if (e) {
         delete ds;
         return e;
     }
     *out = ds;
+    return 0;
+}
+
+extern "C" int ndt_dataset_size(const ndt_dataset* ds, int64_t* samples, int64_t* scans) {
+    if (!ds) return fail(NDT_ERR_INVALID, "null argument");
+    if (samples) *samples = ds->n;
+    if (scans) *scans = ds->nscans;
     return 0;
 }
 
@@ -690,6 +733,8 @@ extern "C" int ndt_model_train(ndt_ctx* c, ndt_model* md, const ndt_dataset* dsc
     std::vector<float> tg((size_t)(6 * n));
     std::vector<int64_t> order((size_t)n);
     Rng rng(tp->seed);
+    const int64_t total_steps = (int64_t)tp->epochs * nb;
+    int64_t step = 0;
     for (int ep = 0; ep < tp->epochs; ep++) {
         for (int64_t i = 0; i < n; i++) order[(size_t)i] = i;
         for (int64_t i = n - 1; i > 0; i--) { /* Fisher-Yates */
@@ -714,7 +759,13 @@ extern "C" int ndt_model_train(ndt_ctx* c, ndt_model* md, const ndt_dataset* dsc
                                    hipMemcpyDeviceToDevice, st));
             if ((e = forward_dev(md, B, true))) return e;
             if ((e = backward_dev(md, B))) return e;
-            if ((e = adam_dev(md, tp->lr, tp->beta1, tp->beta2, tp->eps))) return e;
+            float lr = tp->lr;
+            if (tp->cosine) {
+                const double lo = tp->lr / 20.0;
+                lr = (float)(lo + (tp->lr - lo) * 0.5 * (1.0 + std::cos(3.141592653589793 * (double)step / (double)total_steps)));
+            }
+            step++;
+            if ((e = adam_dev(md, lr, tp->beta1, tp->beta2, tp->eps, tp->weight_decay))) return e;
             LAUNCH_TRY(launch_sum_ordered(md->lossb.as<float>(), B, md->lossacc.as<float>(), st));
         }
         float acc = 0.f;
@@ -774,14 +825,20 @@ extern "C" int ndt_model_predict(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm
 }
 
 extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap* bm, ndt_model* md,
-                                       const ndt_scans* sc, const double* guesses, const ndt_params* p,
+                                       const ndt_scans* sc, const double* guesses, const ndt_params* p, int32_t policy,
                                        ndt_result* results, int32_t* used_learned) {
     if (!c || !g || !bm || !md || !sc || !guesses || !p || !results) return fail(NDT_ERR_INVALID, "null argument");
+    if (policy != NDT_LEARNED_RACE && policy != NDT_LEARNED_FALLBACK) return fail(NDT_ERR_INVALID, "unknown policy");
     HIP_TRY(hipSetDevice(c->device));
     const int S = sc->S;
     double ms_total = 0.0;
     int32_t launches = 0;
     int64_t pairs = 0;
+    auto account = [&]() {
+        ms_total += c->last_ms;
+        launches += c->last_launches;
+        pairs += c->last_pairs;
+    };
     /* refined guesses */
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     std::vector<double> refined((size_t)(16 * S));
@@ -792,40 +849,56 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
     float fms = 0.f;
     HIP_TRY(hipEventElapsedTime(&fms, c->ev0, c->ev1));
     ms_total += fms;
-    /* NDT score at the given guesses (one pass) */
-    ndt_params probe = *p;
-    probe.max_iterations = 0;
-    std::vector<ndt_result> r0((size_t)S);
-    if ((e = ndt_align_scans(c, g, sc, guesses, &probe, r0.data()))) return e;
-    ms_total += c->last_ms;
-    launches += c->last_launches;
-    pairs += c->last_pairs;
-    /* NDT from the refined guesses */
-    if ((e = ndt_align_scans(c, g, sc, refined.data(), p, results))) return e;
-    ms_total += c->last_ms;
-    launches += c->last_launches;
-    pairs += c->last_pairs;
-    /* fallback: plain NDT where the learned start ended worse than the given guess
-     * scored, or did not converge */
-    std::vector<int32_t> fb;
-    for (int s = 0; s < S; s++) {
-        const bool keep = results[s].converged && results[s].score >= r0[(size_t)s].score;
-        if (used_learned) used_learned[s] = keep ? 1 : 0;
-        if (!keep) fb.push_back(s);
-    }
-    if (!fb.empty()) {
-        std::vector<double> fg(fb.size() * 16);
-        for (size_t k = 0; k < fb.size(); k++) std::memcpy(&fg[16 * k], guesses + 16 * (size_t)fb[k], 16 * sizeof(double));
-        std::vector<ndt_result> rf(fb.size());
-        if ((e = align_scan_subset(c, g, sc, fb.data(), (int32_t)fb.size(), fg.data(), p, rf.data()))) return e;
-        ms_total += c->last_ms;
-        launches += c->last_launches;
-        pairs += c->last_pairs;
-        for (size_t k = 0; k < fb.size(); k++) {
-            /* the learned run's passes are spent either way */
-            const int32_t spent = results[fb[k]].passes;
-            results[fb[k]] = rf[k];
-            results[fb[k]].passes += spent;
+    launches += 1;
+    if (policy == NDT_LEARNED_RACE) {
+        /* both starts of every scan in one batch: plain first, learned second */
+        std::vector<int32_t> idx((size_t)(2 * S));
+        std::vector<double> gs((size_t)(32 * S));
+        for (int s = 0; s < S; s++) {
+            idx[(size_t)s] = s;
+            idx[(size_t)(S + s)] = s;
+        }
+        std::memcpy(gs.data(), guesses, 16 * sizeof(double) * (size_t)S);
+        std::memcpy(gs.data() + 16 * (size_t)S, refined.data(), 16 * sizeof(double) * (size_t)S);
+        std::vector<ndt_result> r2((size_t)(2 * S));
+        if ((e = align_scan_subset(c, g, sc, idx.data(), 2 * S, gs.data(), p, r2.data()))) return e;
+        account();
+        for (int s = 0; s < S; s++) {
+            const ndt_result& a = r2[(size_t)s];
+            const ndt_result& b = r2[(size_t)(S + s)];
+            const bool learned = (b.converged && !a.converged) || (b.converged == a.converged && b.score > a.score);
+            results[s] = learned ? b : a;
+            results[s].passes = a.passes + b.passes;
+            if (used_learned) used_learned[s] = learned ? 1 : 0;
+        }
+    } else {
+        /* NDT score at the given guesses (one pass) */
+        ndt_params probe = *p;
+        probe.max_iterations = 0;
+        std::vector<ndt_result> r0((size_t)S);
+        if ((e = ndt_align_scans(c, g, sc, guesses, &probe, r0.data()))) return e;
+        account();
+        if ((e = ndt_align_scans(c, g, sc, refined.data(), p, results))) return e;
+        account();
+        std::vector<int32_t> fb;
+        for (int s = 0; s < S; s++) {
+            const bool keep = results[s].converged && results[s].score >= r0[(size_t)s].score;
+            if (used_learned) used_learned[s] = keep ? 1 : 0;
+            results[s].passes += 1; /* the probe pass */
+            if (!keep) fb.push_back(s);
+        }
+        if (!fb.empty()) {
+            std::vector<double> fg(fb.size() * 16);
+            for (size_t k = 0; k < fb.size(); k++)
+                std::memcpy(&fg[16 * k], guesses + 16 * (size_t)fb[k], 16 * sizeof(double));
+            std::vector<ndt_result> rf(fb.size());
+            if ((e = align_scan_subset(c, g, sc, fb.data(), (int32_t)fb.size(), fg.data(), p, rf.data()))) return e;
+            account();
+            for (size_t k = 0; k < fb.size(); k++) {
+                const int32_t spent = results[fb[k]].passes;
+                results[fb[k]] = rf[k];
+                results[fb[k]].passes += spent;
+            }
         }
     }
     c->last_ms = ms_total;

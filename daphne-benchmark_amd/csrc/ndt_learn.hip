@@ -437,10 +437,11 @@ int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* 
     return (int)hipGetLastError();
 }
 
-/* Adam (Kingma & Ba 2015) on float32 master weights; bc1 = 1 - beta1^t and
- * bc2 = 1 - beta2^t; wb (optional) receives the bf16 copy used by the GEMMs. */
+/* Adam (Kingma & Ba 2015) with decoupled weight decay wd (Loshchilov & Hutter
+ * 2019) on float32 master weights; bc1 = 1 - beta1^t and bc2 = 1 - beta2^t; wb
+ * (optional) receives the bf16 copy used by the GEMMs. */
 __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v, const float* __restrict__ g,
-                       int64_t n, float lr, float beta1, float beta2, float eps, float bc1, float bc2,
+                       int64_t n, float lr, float beta1, float beta2, float eps, float bc1, float bc2, float wd,
                        uint16_t* __restrict__ wb) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -449,15 +450,15 @@ __global__ void k_adam(float* __restrict__ w, float* __restrict__ m, float* __re
     const float vi = beta2 * v[i] + (1.f - beta2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    const float wi = w[i] - lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+    const float wi = w[i] - lr * ((mi / bc1) / (sqrtf(vi / bc2) + eps) + wd * w[i]);
     w[i] = wi;
     if (wb) wb[i] = f2bf(wi);
 }
 
 int launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, float beta1, float beta2, float eps,
-                float bc1, float bc2, uint16_t* wb, hipStream_t stream) {
+                float bc1, float bc2, float wd, uint16_t* wb, hipStream_t stream) {
     hipLaunchKernelGGL(k_adam, dim3(blocks_for(n, 256)), dim3(256), 0, stream, w, m, v, g, n, lr, beta1, beta2, eps, bc1,
-                       bc2, wb);
+                       bc2, wd, wb);
     return (int)hipGetLastError();
 }
 

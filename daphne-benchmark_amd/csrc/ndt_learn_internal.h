@@ -60,7 +60,7 @@ int launch_head_bwd(const float* dY, const uint16_t* H2, const uint16_t* W3, int
                     uint16_t* dZ2, uint16_t* dZ2T, float* db2, hipStream_t stream);
 int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, hipStream_t stream);
 int launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, float beta1, float beta2, float eps,
-                float bc1, float bc2, uint16_t* wb, hipStream_t stream);
+                float bc1, float bc2, float wd, uint16_t* wb, hipStream_t stream);
 int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream);
 int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream);
 

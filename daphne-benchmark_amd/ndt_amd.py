@@ -372,7 +372,8 @@ class NdtGridInfo(C.Structure):
 
 class TrainParams(C.Structure):
     _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("weights", C.c_float * 6)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("weights", C.c_float * 6),
+                ("weight_decay", C.c_float), ("cosine", C.c_int32)]
 
 
 class NdtError(RuntimeError):
@@ -427,14 +428,16 @@ def hip_lib():
         L.ndt_model_forward.argtypes = [P, P, P, C.c_int32, P]
         L.ndt_model_backward.argtypes = [P, P, P, P, P, C.c_int32, C.POINTER(C.c_float)]
         L.ndt_model_grad.argtypes = [P, C.c_int32, P, C.c_int64]
-        L.ndt_model_adam_step.argtypes = [P, P, C.c_float, C.c_float, C.c_float, C.c_float]
+        L.ndt_model_adam_step.argtypes = [P, P, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float]
         L.ndt_train_params_default.argtypes = [C.POINTER(TrainParams)]
         L.ndt_train_params_default.restype = None
         L.ndt_dataset_create.argtypes = [P, P, P, P, P, P, C.c_int64, C.POINTER(P)]
         L.ndt_dataset_destroy.argtypes = [P]
+        L.ndt_dataset_add.argtypes = [P, P, P, P, P, C.c_int64]
+        L.ndt_dataset_size.argtypes = [P, P, P]
         L.ndt_model_train.argtypes = [P, P, P, C.POINTER(TrainParams), P]
         L.ndt_model_predict.argtypes = [P, P, P, P, P, P]
-        L.ndt_align_scans_learned.argtypes = [P, P, P, P, P, P, C.POINTER(NdtParams), P, P]
+        L.ndt_align_scans_learned.argtypes = [P, P, P, P, P, P, C.POINTER(NdtParams), C.c_int32, P, P]
         _hip_lib = L
     return _hip_lib
 
@@ -447,7 +450,8 @@ EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_
 LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_model_create", "ndt_model_destroy",
                  "ndt_model_dims", "ndt_model_save", "ndt_model_load", "ndt_model_get", "ndt_model_set",
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
-                 "ndt_train_params_default", "ndt_dataset_create", "ndt_dataset_destroy", "ndt_model_train",
+                 "ndt_train_params_default", "ndt_dataset_create", "ndt_dataset_destroy", "ndt_dataset_add",
+                 "ndt_dataset_size", "ndt_model_train",
                  "ndt_model_predict", "ndt_align_scans_learned")
 
 
@@ -646,6 +650,7 @@ BEV_CELLS = 32768
 FEATURES = 65536
 W1, B1, W2, B2, W3, B3 = range(6)
 CORR_SCALE = np.array([3.0, 3.0, 1.0, 0.05, 0.05, 0.2])
+LEARNED_RACE, LEARNED_FALLBACK = 0, 1
 
 
 def train_params(**kw) -> TrainParams:
@@ -749,8 +754,8 @@ class Model(_Handle):
                                             None if w is None else w.ctypes.data, x.shape[0], C.byref(loss)))
         return loss.value
 
-    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
-        _check(hip_lib().ndt_model_adam_step(self.ctx.h, self.h, lr, beta1, beta2, eps))
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        _check(hip_lib().ndt_model_adam_step(self.ctx.h, self.h, lr, beta1, beta2, eps, weight_decay))
 
     def train(self, dataset: "Dataset", tp: TrainParams | None = None) -> np.ndarray:
         tp = tp or train_params()
@@ -764,14 +769,15 @@ class Model(_Handle):
         _check(hip_lib().ndt_model_predict(self.ctx.h, self.h, bevmap.h, scans.h, g.ctypes.data, out.ctypes.data))
         return out.reshape(scans.S, 4, 4)
 
-    def align(self, grid: Grid, bevmap: BevMap, scans: "Scans", guesses, prm: NdtParams | None = None):
-        """Batched NDT from learned guesses with fallback; returns (results, used_learned)."""
+    def align(self, grid: Grid, bevmap: BevMap, scans: "Scans", guesses, prm: NdtParams | None = None,
+              policy: int = LEARNED_RACE):
+        """Batched NDT with the learned warm start; returns (results, used_learned)."""
         prm = prm or grid.params
         g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(scans.S, 16)
         out = (NdtResult * scans.S)()
         used = np.zeros(scans.S, np.int32)
         _check(hip_lib().ndt_align_scans_learned(self.ctx.h, grid.h, bevmap.h, self.h, scans.h, g.ctypes.data,
-                                                 C.byref(prm), out, used.ctypes.data))
+                                                 C.byref(prm), policy, out, used.ctypes.data))
         return [_result(out[i]) for i in range(scans.S)], used
 
 
@@ -785,7 +791,19 @@ class Dataset(_Handle):
         n = idx.shape[0]
         g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(n, 16)
         t = np.ascontiguousarray(targets, dtype=np.float64).reshape(n, 16)
-        self.n = n
         self.h = C.c_void_p()
         _check(hip_lib().ndt_dataset_create(ctx.h, bevmap.h, scans.h, idx.ctypes.data, g.ctypes.data, t.ctypes.data,
                                             n, C.byref(self.h)))
+
+    def add(self, scans: "Scans", scan_index, guesses, targets):
+        idx = np.ascontiguousarray(scan_index, dtype=np.int32)
+        n = idx.shape[0]
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(n, 16)
+        t = np.ascontiguousarray(targets, dtype=np.float64).reshape(n, 16)
+        _check(hip_lib().ndt_dataset_add(self.h, scans.h, idx.ctypes.data, g.ctypes.data, t.ctypes.data, n))
+
+    @property
+    def n(self):
+        a, b = C.c_int64(), C.c_int64()
+        _check(hip_lib().ndt_dataset_size(self.h, C.byref(a), C.byref(b)))
+        return a.value

@@ -67,7 +67,8 @@ typedef struct ndt_params {
     int32_t sum_precision;    /* NDT_SUM_FP32 / NDT_SUM_COMP32 / NDT_SUM_FP64 (ndt) */
     double origin[3];         /* world position of the map-local frame (ndt) */
     int32_t block_size;       /* 0 = default (256); threads per workgroup (ndt align) */
-    int32_t points_per_wg;    /* 0 = default (512); scan points per partial sum (ndt align) */
+    int32_t points_per_wg;    /* scan points per partial sum (ndt align); 0 = 2048 (batched
+                                 throughput), 256 suits single-scan latency */
     int32_t scheduler;        /* NDT_SCHED_* (ndt align) */
 } ndt_params;
 

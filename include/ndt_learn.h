@@ -4,9 +4,9 @@
  * A small learned pose proposer refines the initial guess before NDT runs.  The
  * reference has no counterpart: it aligns from the guess it is given
  * (src/CPU/ndt_mapping/kernel.cpp:1187-1206, where init_guess becomes the start
- * pose of computeTransformation).  NDT stays the judge: a learned start whose
- * final NDT score is worse than the score at the given guess, or that does not
- * converge, is replaced by plain NDT from the given guess.
+ * pose of computeTransformation).  NDT stays the judge: the final NDT score decides
+ * between the learned start and plain NDT from the given guess (see
+ * NDT_LEARNED_RACE and NDT_LEARNED_FALLBACK).
  *
  * Features: two bird's-eye-view occupancy grids of 64 x 64 x 8 cells
  * (0.5 m x 0.5 m x 1.5 m; x, y in [-16, 16) m, z in [-2.5, 9.5) m): the scan in its
@@ -51,7 +51,19 @@ typedef struct ndt_train_params {
     float beta1, beta2, eps;
     uint64_t seed;         /* sample order */
     float weights[6];      /* loss weight per output component */
+    float weight_decay;    /* decoupled weight decay of the weight matrices (AdamW) */
+    int32_t cosine;        /* 1: step size follows a cosine from lr down to lr / 20 */
 } ndt_train_params;
+
+/* How ndt_align_scans_learned uses the refined guess:
+ *   NDT_LEARNED_RACE     NDT runs from the given and from the refined guess in one
+ *                        batch; the run with the better final score is kept (a
+ *                        converged run beats one that did not converge), so the
+ *                        result is never worse, by the NDT score, than plain NDT;
+ *   NDT_LEARNED_FALLBACK NDT runs from the refined guess; plain NDT from the given
+ *                        guess replaces it where it did not converge or ended with
+ *                        a lower score than the given guess itself has. */
+enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1 };
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m
  * tiles for the map grids; origin = world position of the map-local frame. */
@@ -83,8 +95,9 @@ int ndt_model_forward(ndt_ctx* ctx, ndt_model* model, const uint16_t* x, int32_t
 int ndt_model_backward(ndt_ctx* ctx, ndt_model* model, const uint16_t* x, const float* target,
                        const float* weights, int32_t B, float* loss);
 int ndt_model_grad(const ndt_model* model, int32_t which, float* out, int64_t n);
-/* One Adam step with the stored gradients. */
-int ndt_model_adam_step(ndt_ctx* ctx, ndt_model* model, float lr, float beta1, float beta2, float eps);
+/* One Adam step with the stored gradients (weight_decay applies to W1, W2, W3). */
+int ndt_model_adam_step(ndt_ctx* ctx, ndt_model* model, float lr, float beta1, float beta2, float eps,
+                        float weight_decay);
 
 void ndt_train_params_default(ndt_train_params* tp);
 
@@ -93,6 +106,12 @@ void ndt_train_params_default(ndt_train_params* tp);
  * computed once and kept on the device with the guesses and scaled corrections. */
 int ndt_dataset_create(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
                        const double* guesses, const double* targets, int64_t n, ndt_dataset** out);
+/* Appends triples that refer to another scan batch (same map), so that datasets
+ * larger than one resident batch of scan points can be built batch by batch. */
+int ndt_dataset_add(ndt_dataset* ds, const ndt_scans* scans, const int32_t* scan_index, const double* guesses,
+                    const double* targets, int64_t n);
+/* Number of samples and of scan grids held. */
+int ndt_dataset_size(const ndt_dataset* ds, int64_t* samples, int64_t* scans);
 int ndt_dataset_destroy(ndt_dataset* ds);
 
 /* Trains on the dataset; epoch_loss (optional, `epochs` values) receives the mean
@@ -104,12 +123,13 @@ int ndt_model_train(ndt_ctx* ctx, ndt_model* model, const ndt_dataset* ds, const
 int ndt_model_predict(ndt_ctx* ctx, ndt_model* model, const ndt_bevmap* bm, const ndt_scans* scans,
                       const double* guesses, double* out_guesses);
 
-/* Batched NDT from learned guesses with the fallback described above.
- * used_learned[s] (optional) = 1 where the learned start was kept.  Timing of
- * the whole call, device milliseconds of feature + model + NDT kernels, is
+/* Batched NDT with the learned warm start under `policy` (NDT_LEARNED_*).
+ * used_learned[s] (optional) = 1 where the result comes from the refined guess.
+ * passes in the results count every derivative pass spent on the scan.  Device
+ * milliseconds of the feature, model and NDT kernels of the whole call are
  * available through ndt_ctx_last_timing. */
 int ndt_align_scans_learned(ndt_ctx* ctx, const ndt_grid* grid, const ndt_bevmap* bm, ndt_model* model,
-                            const ndt_scans* scans, const double* guesses, const ndt_params* params,
+                            const ndt_scans* scans, const double* guesses, const ndt_params* params, int32_t policy,
                             ndt_result* results, int32_t* used_learned);
 
 #ifdef __cplusplus

@@ -174,10 +174,11 @@ def mlp_backward(x, P, T, w=None, bf16=True) -> tuple:
     return mlp_loss(f["Y"], T, w), g, f
 
 
-def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8):
-    """One Adam update at step t (1-based); returns new (w, m, v)."""
+def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
+    """One Adam update at step t (1-based) with decoupled weight decay wd
+    (AdamW); returns new (w, m, v)."""
     m = beta1 * m + (1 - beta1) * g
     v = beta2 * v + (1 - beta2) * g * g
     mh = m / (1 - beta1 ** t)
     vh = v / (1 - beta2 ** t)
-    return w - lr * mh / (np.sqrt(vh) + eps), m, v
+    return w - lr * (mh / (np.sqrt(vh) + eps) + wd * w), m, v
