@@ -14,6 +14,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "ndt_hip.h"
@@ -299,14 +300,16 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     int64_t cap = 1ll << log2cap;
     BUF_TRY(g->slots, sizeof(HashSlot) * cap);
     LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
-    BUF_TRY(g->cand32, sizeof(CandRec32) * (ne + 1));
-    BUF_TRY(g->cand64, sizeof(CandRec64) * (ne + 1));
+    BUF_TRY(g->cand_head, sizeof(float4) * (ne + 1));
+    BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
+    BUF_TRY(g->cand_s64, 5 * sizeof(double2) * (ne + 1));
     if (nq > 0) {
         LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, 5, g->slots.as<HashSlot>(),
                                       log2cap, c->stream));
         LAUNCH_TRY(launch_cand_fill(keys.as<uint64_t>(), vals.as<uint32_t>(), ne, g->mean_c.as<double>(),
                                     g->icov_c.as<double>(), g->bits, g->origin_cell, (double)g->params.resolution,
-                                    g->cand32.as<CandRec32>(), g->cand64.as<CandRec64>(), c->stream));
+                                    g->cand_head.as<float4>(), g->cand_s32.as<float4>(), g->cand_s64.as<double2>(),
+                                    c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     g->info.query_cells = nq;
@@ -314,8 +317,9 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     g->info.hash_capacity = cap;
     NdtGridDev& d = g->dev;
     d.slots = g->slots.as<HashSlot>();
-    d.cand32 = g->cand32.as<CandRec32>();
-    d.cand64 = g->cand64.as<CandRec64>();
+    d.head = g->cand_head.as<float4>();
+    d.s32 = g->cand_s32.as<float4>();
+    d.s64 = g->cand_s64.as<double2>();
     d.log2cap = log2cap;
     for (int k = 0; k < 3; k++) {
         d.bits[k] = g->bits[k];
@@ -661,26 +665,39 @@ static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const 
     }
     BUF_TRY(dst, sizeof(float4) * t);
     std::vector<float> staging((size_t)(4 * t));
-    std::vector<std::pair<uint32_t, uint32_t>> order;
-    for (int32_t s = 0; s < S; s++) {
-        const float* src = scans[s];
-        const int64_t n = counts[s];
-        order.resize((size_t)n);
-        for (int64_t i = 0; i < n; i++) {
-            uint32_t q[3];
-            for (int k = 0; k < 3; k++) {
-                float v = src[4 * i + k];
-                double cell = std::isfinite(v) ? std::floor((double)v) + 512.0 : 0.0;
-                q[k] = (uint32_t)std::min(1023.0, std::max(0.0, cell));
+    auto work = [&](int32_t s0, int32_t s1) {
+        std::vector<std::pair<uint32_t, uint32_t>> order;
+        for (int32_t s = s0; s < s1; s++) {
+            const float* src = scans[s];
+            const int64_t n = counts[s];
+            order.resize((size_t)n);
+            for (int64_t i = 0; i < n; i++) {
+                uint32_t q[3];
+                for (int k = 0; k < 3; k++) {
+                    float v = src[4 * i + k];
+                    double cell = std::isfinite(v) ? std::floor((double)v) + 512.0 : 0.0;
+                    q[k] = (uint32_t)std::min(1023.0, std::max(0.0, cell));
+                }
+                order[(size_t)i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), (uint32_t)i};
             }
-            order[(size_t)i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), (uint32_t)i};
+            std::stable_sort(order.begin(), order.end(),
+                             [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                                 return a.first < b.first;
+                             });
+            float* d = staging.data() + 4 * offsets[s];
+            for (int64_t i = 0; i < n; i++) std::memcpy(d + 4 * i, src + 4 * (size_t)order[(size_t)i].second, 16);
         }
-        std::stable_sort(order.begin(), order.end(),
-                         [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-                             return a.first < b.first;
-                         });
-        float* d = staging.data() + 4 * offsets[s];
-        for (int64_t i = 0; i < n; i++) std::memcpy(d + 4 * i, src + 4 * (size_t)order[(size_t)i].second, 16);
+    };
+    /* scans are independent: reorder them on up to 16 host threads */
+    const int32_t nthreads = (int32_t)std::max(1u, std::min(16u, std::min(std::thread::hardware_concurrency(),
+                                                                           (unsigned)((S + 3) / 4))));
+    if (nthreads <= 1 || t < (1 << 16)) {
+        work(0, S);
+    } else {
+        std::vector<std::thread> pool;
+        for (int32_t k = 0; k < nthreads; k++)
+            pool.emplace_back(work, (int32_t)((int64_t)S * k / nthreads), (int32_t)((int64_t)S * (k + 1) / nthreads));
+        for (auto& th : pool) th.join();
     }
     HIP_TRY(hipMemcpyAsync(dst.p, staging.data(), sizeof(float4) * t, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));

@@ -222,31 +222,19 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
 }
 
 /* Device view of an ndt-mode grid. */
-/* Candidate records: one per (query cell, valid voxel within reach), the lists of
- * a query cell contiguous in probe order.  Each carries the voxel's mean relative
- * to the query cell's lower corner (float32, for the range test) and the voxel's
- * statistics, so that a scan point reaches everything it needs through its hash
- * slot alone. */
-struct alignas(16) CandRec32 {
-    float rx, ry, rz;        /* mean - query cell corner */
-    uint32_t vid;            /* valid-voxel id */
-    float c00, c01, c02, c11;
-    float c12, c22, pad0, pad1;
-};
-struct alignas(16) CandRec64 {
-    float rx, ry, rz;
-    uint32_t vid;
-    double mx, my, mz;       /* mean, map-local */
-    double c00, c01, c02, c11, c12, c22;
-    double pad;
-};
-static_assert(sizeof(CandRec32) == 48, "CandRec32 layout");
-static_assert(sizeof(CandRec64) == 96, "CandRec64 layout");
-
+/* Candidates: one entry per (query cell, valid voxel within reach), the entries
+ * of a query cell contiguous in probe order, in three parallel arrays:
+ *   head  float4 (mean - query cell corner as float32, voxel id bits): the range
+ *         prefilter reads only these, 16 bytes per candidate;
+ *   s32   2 x float4 (c00 c01 c02 c11)(c12 c22 0 0): float32 inverse covariance;
+ *   s64   5 x double2 (mx my)(mz c00)(c01 c02)(c11 c12)(c22 0): float64 mean and
+ *         inverse covariance.
+ * A scan point reaches everything it needs from its hash slot alone. */
 struct NdtGridDev {
-    const HashSlot* slots;   /* query cell -> candidate list */
-    const CandRec32* cand32; /* candidate records, float32 statistics */
-    const CandRec64* cand64; /* the same records with float64 statistics */
+    const HashSlot* slots;   /* query cell -> candidate range */
+    const float4* head;      /* candidates: relative mean, voxel id */
+    const float4* s32;       /* candidates: float32 statistics (2 per entry) */
+    const double2* s64;      /* candidates: float64 statistics (5 per entry) */
     int log2cap;
     int bits[3];             /* key bit widths x, y, z */
     int64_t origin_cell[3];  /* cell coordinates of key 0 */
@@ -401,8 +389,8 @@ int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int6
                        int shift, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
-                     const double* icov, const int* bits, const int64_t* origin_cell, double res, CandRec32* c32,
-                     CandRec64* c64, hipStream_t stream);
+                     const double* icov, const int* bits, const int64_t* origin_cell, double res, float4* head,
+                     float4* s32, double2* s64, hipStream_t stream);
 int launch_hash_clear(HashSlot* slots, int64_t cap, hipStream_t stream);
 int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
                               const uint32_t* seg_start, int64_t nseg, int64_t npts, double ncells,

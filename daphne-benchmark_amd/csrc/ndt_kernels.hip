@@ -641,14 +641,14 @@ int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, 
     return (int)hipGetLastError();
 }
 
-/* Candidate record i of the sorted dilation keys (query cell << 5 | probe): the
- * voxel's mean relative to the lower corner of the query cell, in float32 (values
- * in [-res, 2 res), kept to about 1e-7 m at any map extent), the voxel id, and the
- * voxel's mean and inverse covariance in float32 and float64. */
+/* Candidate i of the sorted dilation keys (query cell << 5 | probe): the voxel's
+ * mean relative to the lower corner of the query cell in float32 (values in
+ * [-res, 2 res), kept to about 1e-7 m at any map extent) with the voxel id, and
+ * the voxel's statistics in float32 and float64. */
 __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t n,
                             const double* __restrict__ mean, const double* __restrict__ icov, Bits3 B, int64_t o0,
-                            int64_t o1, int64_t o2, double res, CandRec32* __restrict__ c32,
-                            CandRec64* __restrict__ c64) {
+                            int64_t o1, int64_t o2, double res, float4* __restrict__ head, float4* __restrict__ s32,
+                            double2* __restrict__ s64) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = svals[i];
@@ -658,46 +658,24 @@ __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* 
     const int64_t c2 = (int64_t)(q >> (B.b[0] + B.b[1])) + o2;
     const double* m = mean + 3 * (int64_t)v;
     const double* C = icov + 6 * (int64_t)v;
-    const float rx = (float)(m[0] - (double)c0 * res), ry = (float)(m[1] - (double)c1 * res),
-                rz = (float)(m[2] - (double)c2 * res);
-    CandRec32 a;
-    a.rx = rx;
-    a.ry = ry;
-    a.rz = rz;
-    a.vid = v;
-    a.c00 = (float)C[0];
-    a.c01 = (float)C[1];
-    a.c02 = (float)C[2];
-    a.c11 = (float)C[3];
-    a.c12 = (float)C[4];
-    a.c22 = (float)C[5];
-    a.pad0 = a.pad1 = 0.0f;
-    c32[i] = a;
-    CandRec64 d;
-    d.rx = rx;
-    d.ry = ry;
-    d.rz = rz;
-    d.vid = v;
-    d.mx = m[0];
-    d.my = m[1];
-    d.mz = m[2];
-    d.c00 = C[0];
-    d.c01 = C[1];
-    d.c02 = C[2];
-    d.c11 = C[3];
-    d.c12 = C[4];
-    d.c22 = C[5];
-    d.pad = 0.0;
-    c64[i] = d;
+    head[i] = make_float4((float)(m[0] - (double)c0 * res), (float)(m[1] - (double)c1 * res),
+                          (float)(m[2] - (double)c2 * res), __uint_as_float(v));
+    s32[2 * i] = make_float4((float)C[0], (float)C[1], (float)C[2], (float)C[3]);
+    s32[2 * i + 1] = make_float4((float)C[4], (float)C[5], 0.0f, 0.0f);
+    s64[5 * i] = make_double2(m[0], m[1]);
+    s64[5 * i + 1] = make_double2(m[2], C[0]);
+    s64[5 * i + 2] = make_double2(C[1], C[2]);
+    s64[5 * i + 3] = make_double2(C[3], C[4]);
+    s64[5 * i + 4] = make_double2(C[5], 0.0);
 }
 
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
-                     const double* icov, const int* bits, const int64_t* origin_cell, double res, CandRec32* c32,
-                     CandRec64* c64, hipStream_t stream) {
+                     const double* icov, const int* bits, const int64_t* origin_cell, double res, float4* head,
+                     float4* s32, double2* s64, hipStream_t stream) {
     Bits3 B;
     for (int k = 0; k < 3; k++) B.b[k] = bits[k];
     hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_keys, sorted_vals,
-                       nentries, mean, icov, B, origin_cell[0], origin_cell[1], origin_cell[2], res, c32, c64);
+                       nentries, mean, icov, B, origin_cell[0], origin_cell[1], origin_cell[2], res, head, s32, s64);
     return (int)hipGetLastError();
 }
 
@@ -959,11 +937,10 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
      * fp64 mode, where the exact test below decides. */
     const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
     uint32_t mask = 0;
-    const float4* heads = (sizeof(T) == 4) ? (const float4*)(g.cand32 + begin) : (const float4*)(g.cand64 + begin);
-    constexpr int HSTRIDE = (sizeof(T) == 4 ? sizeof(CandRec32) : sizeof(CandRec64)) / sizeof(float4);
+    const float4* heads = g.head + begin;
 #pragma unroll 4
     for (uint32_t k = 0; k < count; k++) {
-        const float4 cd = heads[k * HSTRIDE];
+        const float4 cd = heads[k];
         const float e0 = r0 - cd.x, e1 = r1 - cd.y, e2 = r2 - cd.z;
         mask |= (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
     }
@@ -974,16 +951,17 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
         T d0, d1, d2;
         T c00, c01, c02, c11, c12, c22;
         if constexpr (sizeof(T) == 4) {
-            const float4* r = (const float4*)(g.cand32 + begin + k);
-            const float4 h = r[0], a = r[1], b = r[2];
+            const float4 h = heads[k];
+            const float4* r = g.s32 + 2 * (int64_t)(begin + k);
+            const float4 a = r[0], b = r[1];
             d0 = r0 - h.x;
             d1 = r1 - h.y;
             d2 = r2 - h.z;
             if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
             c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
         } else {
-            const double2* r = (const double2*)(g.cand64 + begin + k);
-            const double2 m01 = r[1], m2c0 = r[2], c12v = r[3], c34 = r[4], c5p = r[5];
+            const double2* r = g.s64 + 5 * (int64_t)(begin + k);
+            const double2 m01 = r[0], m2c0 = r[1], c12v = r[2], c34 = r[3], c5p = r[4];
             d0 = xd0 - m01.x;
             d1 = xd1 - m01.y;
             d2 = xd2 - m2c0.x;

@@ -11,6 +11,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define SCENE_PI 3.14159265358979323846
 
@@ -222,9 +225,140 @@ int ndt_scene_map(const ndt_scene_cfg* cfg, int64_t m, float* out) {
     return 0;
 }
 
-int ndt_scene_scan(const ndt_scene_cfg* scene, const float* map, int64_t m,
-                   const ndt_scan_cfg* cfg, uint64_t scan_seed,
-                   float* out, double gt_pose[16], double guess_pose[16]) {
+/* Map points bucketed on an 8 m grid in the x-y plane, indices increasing
+ * within each bucket, so that the points within sensor range can be gathered
+ * without visiting the whole map. */
+typedef struct {
+    double x0, y0, cell;
+    int64_t nx, ny;
+    int64_t* start; /* nx * ny + 1 */
+    int64_t* idx;
+} bucket_index;
+
+static void bucket_free(bucket_index* b) {
+    free(b->start);
+    free(b->idx);
+    b->start = b->idx = NULL;
+}
+
+static int64_t bucket_of(const bucket_index* b, double x, double y) {
+    int64_t ix = (int64_t)floor((x - b->x0) / b->cell), iy = (int64_t)floor((y - b->y0) / b->cell);
+    if (ix < 0) ix = 0;
+    if (iy < 0) iy = 0;
+    if (ix >= b->nx) ix = b->nx - 1;
+    if (iy >= b->ny) iy = b->ny - 1;
+    return iy * b->nx + ix;
+}
+
+static int bucket_build(const float* map, int64_t m, bucket_index* b) {
+    memset(b, 0, sizeof(*b));
+    double lo[2] = {1e300, 1e300}, hi[2] = {-1e300, -1e300};
+    for (int64_t i = 0; i < m; i++) {
+        const double x = map[4 * i], y = map[4 * i + 1], z = map[4 * i + 2];
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) continue;
+        if (x < lo[0]) lo[0] = x;
+        if (x > hi[0]) hi[0] = x;
+        if (y < lo[1]) lo[1] = y;
+        if (y > hi[1]) hi[1] = y;
+    }
+    if (lo[0] > hi[0]) return 1;
+    b->cell = 8.0;
+    b->x0 = lo[0];
+    b->y0 = lo[1];
+    b->nx = (int64_t)floor((hi[0] - lo[0]) / b->cell) + 1;
+    b->ny = (int64_t)floor((hi[1] - lo[1]) / b->cell) + 1;
+    if (b->nx * b->ny > (1ll << 26)) return 1;
+    b->start = (int64_t*)calloc((size_t)(b->nx * b->ny + 1), sizeof(int64_t));
+    b->idx = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m > 0 ? m : 1));
+    if (!b->start || !b->idx) {
+        bucket_free(b);
+        return 1;
+    }
+    for (int64_t i = 0; i < m; i++) {
+        const double x = map[4 * i], y = map[4 * i + 1], z = map[4 * i + 2];
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) continue;
+        b->start[bucket_of(b, x, y) + 1]++;
+    }
+    for (int64_t k = 1; k <= b->nx * b->ny; k++) b->start[k] += b->start[k - 1];
+    int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)(b->nx * b->ny));
+    if (!fill) {
+        bucket_free(b);
+        return 1;
+    }
+    memcpy(fill, b->start, sizeof(int64_t) * (size_t)(b->nx * b->ny));
+    for (int64_t i = 0; i < m; i++) {
+        const double x = map[4 * i], y = map[4 * i + 1], z = map[4 * i + 2];
+        if (!isfinite(x) || !isfinite(y) || !isfinite(z)) continue;
+        b->idx[fill[bucket_of(b, x, y)]++] = i;
+    }
+    free(fill);
+    return 0;
+}
+
+/* ascending sort of n non-negative indices below 2^48: LSD radix sort, 16-bit
+ * digits, through tmp (n entries); digits that are zero everywhere are skipped */
+static void sort_indices(int64_t* a, int64_t* tmp, int64_t n) {
+    int64_t hi = 0;
+    for (int64_t i = 0; i < n; i++) hi |= a[i];
+    static const int B = 16;
+    size_t cnt[1 << 16];
+    for (int shift = 0; shift < 48 && (hi >> shift); shift += B) {
+        memset(cnt, 0, sizeof(cnt));
+        for (int64_t i = 0; i < n; i++) cnt[(a[i] >> shift) & 0xFFFF]++;
+        size_t sum = 0;
+        for (int d = 0; d < (1 << B); d++) {
+            size_t c = cnt[d];
+            cnt[d] = sum;
+            sum += c;
+        }
+        for (int64_t i = 0; i < n; i++) tmp[cnt[(a[i] >> shift) & 0xFFFF]++] = a[i];
+        memcpy(a, tmp, sizeof(int64_t) * (size_t)n);
+    }
+}
+
+/* Indices of the map points within range of (sx, sy, sz), in increasing order:
+ * every point (bidx == NULL) or the points of the buckets the range box touches.
+ * Both give the same list.  Returns the count, or -1 when out of memory. */
+static int64_t gather_candidates(const float* map, int64_t m, const bucket_index* bidx, double sx, double sy,
+                                 double sz, double range, int64_t** cand, int64_t* cap) {
+    const double r2 = range * range;
+    int64_t nc = 0;
+#define NDT_SCENE_KEEP(i)                                                                                     \
+    do {                                                                                                      \
+        double dx = (double)map[4 * (i)] - sx, dy = (double)map[4 * (i) + 1] - sy,                            \
+               dz = (double)map[4 * (i) + 2] - sz;                                                            \
+        if (dx * dx + dy * dy + dz * dz <= r2) {                                                              \
+            if (nc == *cap) {                                                                                 \
+                int64_t* nb = (int64_t*)realloc(*cand, sizeof(int64_t) * (size_t)(*cap * 2));                 \
+                if (!nb) return -1;                                                                           \
+                *cand = nb;                                                                                   \
+                *cap *= 2;                                                                                    \
+            }                                                                                                 \
+            (*cand)[nc++] = (i);                                                                              \
+        }                                                                                                     \
+    } while (0)
+    if (!bidx) {
+        for (int64_t i = 0; i < m; i++) NDT_SCENE_KEEP(i);
+        return nc;
+    }
+    const int64_t b0 = bucket_of(bidx, sx - range, sy - range), b1 = bucket_of(bidx, sx + range, sy + range);
+    const int64_t x0 = b0 % bidx->nx, y0 = b0 / bidx->nx, x1 = b1 % bidx->nx, y1 = b1 / bidx->nx;
+    for (int64_t by = y0; by <= y1; by++)
+        for (int64_t bx = x0; bx <= x1; bx++) {
+            const int64_t k = by * bidx->nx + bx;
+            for (int64_t j = bidx->start[k]; j < bidx->start[k + 1]; j++) NDT_SCENE_KEEP(bidx->idx[j]);
+        }
+#undef NDT_SCENE_KEEP
+    int64_t* tmp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nc > 0 ? nc : 1));
+    if (!tmp) return -1;
+    sort_indices(*cand, tmp, nc);
+    free(tmp);
+    return nc;
+}
+
+static int scan_impl(const ndt_scene_cfg* scene, const float* map, int64_t m, const bucket_index* bidx,
+                     const ndt_scan_cfg* cfg, uint64_t scan_seed, float* out, double gt_pose[16],
+                     double guess_pose[16]) {
     if (!scene || !map || m <= 0 || !cfg || cfg->n_points <= 0 || !out || !gt_pose || !guess_pose)
         return 1;
     rng_t r;
@@ -241,22 +375,14 @@ int ndt_scene_scan(const ndt_scene_cfg* scene, const float* map, int64_t m,
     ndt_scene_pose(sx + scene->origin[0], sy + scene->origin[1], sz + scene->origin[2], 0.0, 0.0, yaw, R);
     memcpy(gt_pose, R, sizeof(double) * 16);
 
-    /* candidate map points within range (local frame) */
-    int64_t cap = 1024, nc = 0;
+    /* candidate map points within range (local frame), in map order */
+    int64_t cap = 1024;
     int64_t* cand = (int64_t*)malloc(sizeof(int64_t) * cap);
     if (!cand) return 1;
-    double r2 = cfg->max_range * cfg->max_range;
-    for (int64_t i = 0; i < m; i++) {
-        double dx = (double)map[4 * i] - sx, dy = (double)map[4 * i + 1] - sy, dz = (double)map[4 * i + 2] - sz;
-        if (dx * dx + dy * dy + dz * dz <= r2) {
-            if (nc == cap) {
-                cap *= 2;
-                int64_t* nb = (int64_t*)realloc(cand, sizeof(int64_t) * cap);
-                if (!nb) { free(cand); return 1; }
-                cand = nb;
-            }
-            cand[nc++] = i;
-        }
+    int64_t nc = gather_candidates(map, m, bidx, sx, sy, sz, cfg->max_range, &cand, &cap);
+    if (nc < 0) {
+        free(cand);
+        return 1;
     }
     if (nc == 0) {
         free(cand);
@@ -296,5 +422,30 @@ int ndt_scene_scan(const ndt_scene_cfg* scene, const float* map, int64_t m,
     double epitch = rng_uniform(&r, -cfg->perturb_rollpitch_deg, cfg->perturb_rollpitch_deg) * deg;
     ndt_scene_pose(sx + scene->origin[0] + ex, sy + scene->origin[1] + ey, sz + scene->origin[2] + ez,
                    eroll, epitch, yaw + eyaw, guess_pose);
+    return 0;
+}
+
+int ndt_scene_scan(const ndt_scene_cfg* scene, const float* map, int64_t m, const ndt_scan_cfg* cfg,
+                   uint64_t scan_seed, float* out, double gt_pose[16], double guess_pose[16]) {
+    return scan_impl(scene, map, m, NULL, cfg, scan_seed, out, gt_pose, guess_pose);
+}
+
+int ndt_scene_scans(const ndt_scene_cfg* scene, const float* map, int64_t m, const ndt_scan_cfg* cfgs,
+                    const uint64_t* seeds, int32_t n, const int64_t* offsets, float* out, double* gt_poses,
+                    double* guess_poses, int32_t threads, int32_t* status) {
+    if (!scene || !map || m <= 0 || !cfgs || !seeds || n < 0 || !offsets || !out || !gt_poses || !guess_poses ||
+        !status)
+        return 1;
+    bucket_index b;
+    if (bucket_build(map, m, &b)) return 1;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (int32_t k = 0; k < n; k++)
+        status[k] = scan_impl(scene, map, m, &b, &cfgs[k], seeds[k], out + 4 * offsets[k], gt_poses + 16 * (int64_t)k,
+                              guess_poses + 16 * (int64_t)k);
+    (void)threads;
+    bucket_free(&b);
     return 0;
 }

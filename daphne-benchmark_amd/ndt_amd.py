@@ -75,6 +75,8 @@ def scene_lib():
                                        C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.ndt_scene_scan.restype = C.c_int
         lib.ndt_scene_pose.argtypes = [C.c_double] * 6 + [C.c_void_p]
+        lib.ndt_scene_scans.argtypes = [C.POINTER(SceneCfg), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
         _scene_lib = lib
     return _scene_lib
 
@@ -199,18 +201,27 @@ def make_workload(name: str, seed: int = 1, scans: int | None = None, map_points
 
 def make_scans(scene: SceneCfg, map_xyzw: np.ndarray, sizes, seeds, perturb_xyz=1.0, perturb_yaw_deg=3.0,
                perturb_rollpitch_deg=0.5, threads: int | None = None):
-    """Scans of the given sizes and seeds, drawn on host threads (the generator
-    call releases the GIL).  Returns (scans, gt (S,4,4), guesses (S,4,4))."""
-    from concurrent.futures import ThreadPoolExecutor
-    threads = threads or min(16, os.cpu_count() or 1)
-
-    def one(k):
-        cfg = scan_cfg(int(sizes[k]), perturb_xyz, perturb_yaw_deg, perturb_rollpitch_deg)
-        return make_scan(scene, map_xyzw, cfg, int(seeds[k]))
-
-    with ThreadPoolExecutor(threads) as ex:
-        out = list(ex.map(one, range(len(sizes))))
-    return [o[0] for o in out], np.stack([o[1] for o in out]), np.stack([o[2] for o in out])
+    """Scans of the given sizes and seeds, drawn on host threads; each equals
+    make_scan with the same arguments.  Returns (scans, gt (S,4,4), guesses (S,4,4))."""
+    n = len(sizes)
+    cfgs = (ScanCfg * n)()
+    for k in range(n):
+        cfgs[k] = scan_cfg(int(sizes[k]), perturb_xyz, perturb_yaw_deg, perturb_rollpitch_deg)
+    seeds = np.ascontiguousarray(seeds, dtype=np.uint64)
+    counts = np.asarray(sizes, dtype=np.int64)
+    offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    out = np.zeros((int(counts.sum()), 4), np.float32)
+    gt = np.zeros((n, 4, 4))
+    gs = np.zeros((n, 4, 4))
+    status = np.zeros(n, np.int32)
+    m = np.ascontiguousarray(map_xyzw, dtype=np.float32)
+    rc = scene_lib().ndt_scene_scans(C.byref(scene), m.ctypes.data, m.shape[0], cfgs, seeds.ctypes.data, n,
+                                     offsets.ctypes.data, out.ctypes.data, gt.ctypes.data, gs.ctypes.data,
+                                     threads or 0, status.ctypes.data)
+    if rc != 0 or np.any(status != 0):
+        raise RuntimeError(f"ndt_scene_scans failed ({rc}, statuses {sorted(set(status.tolist()))})")
+    scans = [out[o:o + c] for o, c in zip(offsets, counts)]
+    return scans, gt, gs
 
 
 def perturb_poses(gt, rng, xy=3.0, z=0.3, yaw_deg=10.0, rollpitch_deg=0.5):

@@ -73,6 +73,16 @@ int ndt_scene_scan(const ndt_scene_cfg* scene, const float* map_xyzw, int64_t m,
                    const ndt_scan_cfg* cfg, uint64_t scan_seed,
                    float* out_scan_xyzw, double gt_pose[16], double guess_pose[16]);
 
+/* Draws n scans on `threads` host threads (<= 0: all), scan k with cfgs[k] and
+ * seeds[k], exactly as ndt_scene_scan would: its points go to
+ * out_xyzw + 4 * offsets[k], its poses to gt_poses + 16 k and guess_poses + 16 k,
+ * its ndt_scene_scan return code to status[k].  The map is bucketed once, so a
+ * scan visits only the map points near its sensor.  Returns non-zero on invalid
+ * arguments. */
+int ndt_scene_scans(const ndt_scene_cfg* scene, const float* map_xyzw, int64_t m, const ndt_scan_cfg* cfgs,
+                    const uint64_t* seeds, int32_t n, const int64_t* offsets, float* out_xyzw, double* gt_poses,
+                    double* guess_poses, int32_t threads, int32_t* status);
+
 /* Row-major 4x4 pose from translation (x, y, z) and Euler angles with
  * R = Rx(roll) * Ry(pitch) * Rz(yaw). */
 void ndt_scene_pose(double x, double y, double z, double roll, double pitch, double yaw,
