@@ -106,3 +106,78 @@ def test_invalid_model_arguments(A, ctx):
     m = A.Model(ctx, 256, 128, 128)
     with pytest.raises(A.NdtError):
         m.forward(np.zeros((100, 256), np.uint16))
+
+
+@pytest.fixture(scope="module")
+def trained(A, ctx):
+    """A proposer trained on 24,000 scans of the cfg2 map (8 guesses each, half within
+    +-3 m / +-10 deg yaw, half within +-1 m / +-3 deg), labelled by NDT from the true
+    pose, built in three batches of scans; 256 held-out scans (other sensor poses)
+    for evaluation."""
+    w = A.make_workload("cfg2", seed=1, scans=4)
+    prm = A.params(A.NDT_MODE_NDT, origin=w.origin)
+    grid = A.Grid(ctx, w.map_xyzw, prm)
+    bm = A.BevMap(ctx, w.map_xyzw, w.origin)
+    rng = np.random.default_rng(5)
+    ds = None
+    for c0 in range(0, 24000, 8000):
+        tr, tr_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 8000, 99 + c0),
+                                    100000 + c0 + np.arange(8000))
+        trS = A.Scans(ctx, tr)
+        lab = trS.align(grid, tr_gt, prm)
+        labels = np.stack([r["final_f64"] for r in lab])
+        keep = np.nonzero([r["converged"] for r in lab])[0]
+        idx = np.repeat(keep, 8).astype(np.int32)
+        wide = rng.uniform(size=len(idx)) < 0.5
+        guesses = np.where(wide[:, None, None], A.perturb_poses(tr_gt[idx], rng),
+                           A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0))
+        if ds is None:
+            ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
+        else:
+            ds.add(trS, idx, guesses, labels[idx])
+        del tr, trS
+    model = A.Model(ctx, seed=1)
+    losses = model.train(ds, A.train_params(epochs=8, batch=512))
+    ev, ev_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 256, 77), 900000 + np.arange(256))
+    evS = A.Scans(ctx, ev)
+    eg = A.perturb_poses(ev_gt, np.random.default_rng(9))
+    return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg)
+
+
+def _success(results, gt):
+    return np.array([np.linalg.norm(r["final_f64"][:3, 3] - g[:3, 3]) < 0.05 for r, g in zip(results, gt)])
+
+
+def test_learned_warm_start_gains(A, trained):
+    """What the trained network adds on held-out scans with +-3 m / +-10 deg guesses:
+    refined guesses closer to the truth, and NDT that reaches the truth more often
+    and converges at least as often as plain NDT."""
+    t = trained
+    assert t["losses"][-1] < 0.5 * t["losses"][0]
+    pred = t["model"].predict(t["bm"], t["evS"], t["eg"])
+    raw_err = np.linalg.norm(t["eg"][:, :2, 3] - t["ev_gt"][:, :2, 3], axis=1)
+    pred_err = np.linalg.norm(pred[:, :2, 3] - t["ev_gt"][:, :2, 3], axis=1)
+    assert pred_err.mean() < 0.85 * raw_err.mean(), (raw_err.mean(), pred_err.mean())
+    plain = t["evS"].align(t["grid"], t["eg"], t["prm"])
+    race, used = t["model"].align(t["grid"], t["bm"], t["evS"], t["eg"], t["prm"], A.LEARNED_RACE)
+    sp, sr = _success(plain, t["ev_gt"]).mean(), _success(race, t["ev_gt"]).mean()
+    assert sr >= sp + 0.08, (sp, sr)
+    assert np.mean([r["converged"] for r in race]) >= np.mean([r["converged"] for r in plain])
+
+
+def test_policies_return_plain_or_learned_runs(A, trained):
+    """Every result of either policy is bit for bit the NDT run from the given guess
+    or the one from the refined guess, as used_learned says; the race keeps the
+    better final score."""
+    t = trained
+    ev, eg = t["evS"], t["eg"]
+    pred = t["model"].predict(t["bm"], ev, eg)
+    plain = ev.align(t["grid"], eg, t["prm"])
+    from_pred = ev.align(t["grid"], pred, t["prm"])
+    for policy in (A.LEARNED_RACE, A.LEARNED_FALLBACK):
+        res, used = t["model"].align(t["grid"], t["bm"], ev, eg, t["prm"], policy)
+        for s, r in enumerate(res):
+            ref = from_pred[s] if used[s] else plain[s]
+            assert np.array_equal(r["final_f64"], ref["final_f64"]) and r["iterations"] == ref["iterations"], (policy, s)
+            if policy == A.LEARNED_RACE and plain[s]["converged"] == from_pred[s]["converged"]:
+                assert r["score"] == max(plain[s]["score"], from_pred[s]["score"])
