@@ -211,6 +211,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
         hi_cell[k] = (int64_t)std::floor((double)ord2f(h_mm[3 + k]) / res);
         g->origin_cell[k] = lo_cell[k] - 1;
         g->bits[k] = std::max(1, ceil_log2(hi_cell[k] - lo_cell[k] + 3));
+        g->dims[k] = (int32_t)std::min<int64_t>(hi_cell[k] - lo_cell[k] + 3, 0x7fffffff);
     }
     const int B = g->bits[0] + g->bits[1] + g->bits[2];
     if (B + 5 > 62) return fail(NDT_ERR_UNSUPPORTED, "map extent too large for the cell key");
@@ -303,6 +304,17 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     BUF_TRY(g->cand_head, sizeof(float4) * (ne + 1));
     BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
     BUF_TRY(g->cand_s64, 5 * sizeof(double2) * (ne + 1));
+    /* dense query-cell index when the map's cell box is at most 2^28 cells (2 GiB) */
+    const double box = (double)g->dims[0] * (double)g->dims[1] * (double)g->dims[2];
+    const bool dense = box <= (double)(1 << 28);
+    if (dense) {
+        const size_t cells = (size_t)g->dims[0] * (size_t)g->dims[1] * (size_t)g->dims[2];
+        BUF_TRY(g->dense, sizeof(uint2) * cells);
+        HIP_TRY(hipMemsetAsync(g->dense.p, 0, sizeof(uint2) * cells, c->stream));
+        if (nq > 0)
+            LAUNCH_TRY(launch_dense_fill(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, g->bits, g->dims,
+                                         g->dense.as<uint2>(), c->stream));
+    }
     if (nq > 0) {
         LAUNCH_TRY(launch_hash_insert(keys.as<uint64_t>(), starts.as<uint32_t>(), nq, ne, 5, g->slots.as<HashSlot>(),
                                       log2cap, c->stream));
@@ -316,6 +328,8 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     g->info.candidates = ne;
     g->info.hash_capacity = cap;
     NdtGridDev& d = g->dev;
+    d.dense = dense ? g->dense.as<uint2>() : nullptr;
+    for (int k = 0; k < 3; k++) d.dims[k] = g->dims[k];
     d.slots = g->slots.as<HashSlot>();
     d.head = g->cand_head.as<float4>();
     d.s32 = g->cand_s32.as<float4>();

@@ -100,7 +100,8 @@ struct ndt_grid {
     ndt_grid_info info;
     /* ndt */
     ndt::NdtGridDev dev;
-    DevBuf slots, cand_head, cand_s32, cand_s64, icov_f, mean_c, icov_c;
+    DevBuf slots, dense, cand_head, cand_s32, cand_s64, icov_f, mean_c, icov_c;
+    int32_t dims[3] = {0, 0, 0};
     /* map points sorted by voxel + occupied-voxel hash (fitness score) */
     ndt::FitGridDev fdev;
     DevBuf fit_pts, fit_slots;

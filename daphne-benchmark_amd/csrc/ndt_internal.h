@@ -231,7 +231,9 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
  *         inverse covariance.
  * A scan point reaches everything it needs from its hash slot alone. */
 struct NdtGridDev {
-    const HashSlot* slots;   /* query cell -> candidate range */
+    const uint2* dense;      /* query cell -> (begin, count), dense over the map's cell box, or null */
+    int32_t dims[3];         /* cells per axis of the dense index */
+    const HashSlot* slots;   /* query cell -> candidate range (when dense is null) */
     const float4* head;      /* candidates: relative mean, voxel id */
     const float4* s32;       /* candidates: float32 statistics (2 per entry) */
     const double2* s64;      /* candidates: float64 statistics (5 per entry) */
@@ -397,6 +399,8 @@ int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, con
                               double* mean, double* icov, int32_t* count, uint64_t* vkey, hipStream_t stream);
 
 /* ndt alignment: init states from guesses, then the cooperative persistent kernel */
+int launch_dense_fill(const uint64_t* sorted_keys, const uint32_t* starts, int64_t nq, int64_t nentries,
+                      const int* bits, const int32_t* dims, uint2* dense, hipStream_t stream);
 int launch_gather_points(const float4* pts, const uint32_t* idx, int64_t n, float4* out, hipStream_t stream);
 int launch_fitness(FitGridDev fg, const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S,
                    const double* poses, double max_d2, int shells, int nblk, double* partials, hipStream_t stream);

@@ -610,6 +610,30 @@ __global__ void k_hash_insert(const uint64_t* __restrict__ skeys, const uint32_t
     }
 }
 
+/* Dense query-cell index: entry (c2 * D1 + c1) * D0 + c0 = (begin, count) of the
+ * candidate run of query cell (c0, c1, c2); cells without candidates stay (0, 0). */
+__global__ void k_dense_fill(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ starts, int64_t nq,
+                             int64_t nentries, Bits3 B, int32_t d0, int32_t d1, uint2* __restrict__ dense) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const uint32_t b = starts[q];
+    const uint32_t e = (q + 1 < nq) ? starts[q + 1] : (uint32_t)nentries;
+    const uint64_t key = skeys[b] >> 5;
+    const int64_t c0 = (int64_t)(key & ((1ull << B.b[0]) - 1));
+    const int64_t c1 = (int64_t)((key >> B.b[0]) & ((1ull << B.b[1]) - 1));
+    const int64_t c2 = (int64_t)(key >> (B.b[0] + B.b[1]));
+    dense[(c2 * d1 + c1) * d0 + c0] = make_uint2(b, e - b);
+}
+
+int launch_dense_fill(const uint64_t* sorted_keys, const uint32_t* starts, int64_t nq, int64_t nentries,
+                      const int* bits, const int32_t* dims, uint2* dense, hipStream_t stream) {
+    Bits3 B;
+    for (int k = 0; k < 3; k++) B.b[k] = bits[k];
+    hipLaunchKernelGGL(k_dense_fill, dim3(grid_for(nq, 256)), dim3(256), 0, stream, sorted_keys, starts, nq, nentries, B,
+                       dims[0], dims[1], dense);
+    return (int)hipGetLastError();
+}
+
 int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int64_t nq, int64_t nentries, int shift,
                        HashSlot* slots, int log2cap, hipStream_t stream) {
     hipLaunchKernelGGL(k_hash_insert, dim3(grid_for(nq, 256)), dim3(256), 0, stream, sorted_keys, starts, nq, nentries,
@@ -898,32 +922,79 @@ __device__ inline bool hash_find(const HashSlot* __restrict__ slots, int log2cap
     return false;
 }
 
-/* One scan point: transform, hashed lookup of its cell's candidate list, and the
- * score / gradient / Hessian (Magnusson 2009 eqs. 6.9-6.13) of every candidate
- * voxel whose mean lies within one resolution.  Returns the number of pairs. */
-template <typename T, int SUM>
-__device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, const PassParams<T>& u, Acc<T, SUM>& acc) {
+/* Index lookup of one scan point: the transformed point (float64), its position
+ * relative to its cell's lower corner (float32, where the candidate means are
+ * stored) and its cell's candidate run. */
+struct PointLook {
+    double x0, x1, x2;
+    float r0, r1, r2;
+    uint32_t begin, count;
+};
+
+/* Transform, cell and index lookup; false when the point has no candidates. */
+__device__ inline bool point_lookup(const NdtGridDev& g, const float4 p, const double* R, const double* t,
+                                    PointLook& L) {
 #pragma clang fp contract(fast)
-    /* transform and cell in float64 in every precision */
     const double X0 = p.x, X1 = p.y, X2 = p.z;
-    const double xd0 = u.R[0] * X0 + u.R[1] * X1 + u.R[2] * X2 + u.t[0];
-    const double xd1 = u.R[3] * X0 + u.R[4] * X1 + u.R[5] * X2 + u.t[1];
-    const double xd2 = u.R[6] * X0 + u.R[7] * X1 + u.R[8] * X2 + u.t[2];
-    if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return 0;
+    const double xd0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
+    const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
+    const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
+    if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return false;
     const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
     const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
     const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
     const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
-    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= (1ll << g.bits[0]) || c1 >= (1ll << g.bits[1]) || c2 >= (1ll << g.bits[2]))
-        return 0;
-    const uint64_t key = ((uint64_t)c2 << (g.bits[0] + g.bits[1])) | ((uint64_t)c1 << g.bits[0]) | (uint64_t)c0;
     uint32_t begin, count;
-    if (!hash_find(g.slots, g.log2cap, key, begin, count)) return 0;
-    /* the point relative to the lower corner of its cell, where the candidate
-     * means are stored: float32 differences of values below 2 resolutions */
-    const float r0 = (float)(xd0 - (double)a0 * g.resolution_d);
-    const float r1 = (float)(xd1 - (double)a1 * g.resolution_d);
-    const float r2 = (float)(xd2 - (double)a2 * g.resolution_d);
+    if (g.dense) {
+        /* direct index: neighbouring points read neighbouring entries */
+        if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= g.dims[0] || c1 >= g.dims[1] || c2 >= g.dims[2]) return false;
+        const uint2 e = g.dense[(c2 * g.dims[1] + c1) * g.dims[0] + c0];
+        begin = e.x;
+        count = e.y;
+        if (!count) return false;
+    } else {
+        if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= (1ll << g.bits[0]) || c1 >= (1ll << g.bits[1]) ||
+            c2 >= (1ll << g.bits[2]))
+            return false;
+        const uint64_t key = ((uint64_t)c2 << (g.bits[0] + g.bits[1])) | ((uint64_t)c1 << g.bits[0]) | (uint64_t)c0;
+        if (!hash_find(g.slots, g.log2cap, key, begin, count)) return false;
+    }
+    L.x0 = xd0;
+    L.x1 = xd1;
+    L.x2 = xd2;
+    L.r0 = (float)(xd0 - (double)a0 * g.resolution_d);
+    L.r1 = (float)(xd1 - (double)a1 * g.resolution_d);
+    L.r2 = (float)(xd2 - (double)a2 * g.resolution_d);
+    L.begin = begin;
+    L.count = count;
+    return true;
+}
+
+/* range prefilter of candidate k of a point: bit k of the result */
+__device__ inline uint32_t prefilter_bit(const NdtGridDev& g, const PointLook& L, uint32_t k, float lim) {
+    const float4 cd = g.head[L.begin + k];
+    const float e0 = L.r0 - cd.x, e1 = L.r1 - cd.y, e2 = L.r2 - cd.z;
+    return (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
+}
+
+/* The float prefilter keeps every candidate the exact test can keep: in fp64 mode
+ * it carries a margin and the exact test decides. */
+template <typename T>
+__device__ inline float prefilter_limit(const PassParams<T>& u) {
+    return (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
+}
+
+/* Score / gradient / Hessian (Magnusson 2009 eqs. 6.9-6.13) of the candidates of
+ * one looked-up point selected by mask whose mean lies within one resolution.
+ * Returns the number of pairs. */
+template <typename T, int SUM>
+__device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, uint32_t mask,
+                                  const PassParams<T>& u, Acc<T, SUM>& acc) {
+#pragma clang fp contract(fast)
+    const double xd0 = L.x0, xd1 = L.x1, xd2 = L.x2;
+    const float r0 = L.r0, r1 = L.r1, r2 = L.r2;
+    const uint32_t begin = L.begin;
+    const float4* heads = g.head + begin;
     const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
     /* point Jacobian columns J3 = e_x x ya, J4 = a4 x ya, J5 = a5 x ya */
     const T y0 = u.Ra[0] * x0 + u.Ra[1] * x1 + u.Ra[2] * x2;
@@ -932,18 +1003,6 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     const T J3y = -y2, J3z = y1;
     const T J4x = u.a4[1] * y2 - u.a4[2] * y1, J4y = u.a4[2] * y0 - u.a4[0] * y2, J4z = u.a4[0] * y1 - u.a4[1] * y0;
     const T J5x = u.a5[1] * y2 - u.a5[2] * y1, J5y = u.a5[2] * y0 - u.a5[0] * y2, J5z = u.a5[0] * y1 - u.a5[1] * y0;
-    /* Candidate filter first: branch-free over the (at most 27) list entries so
-     * that their loads are in flight together; the float test carries a margin in
-     * fp64 mode, where the exact test below decides. */
-    const float lim = (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
-    uint32_t mask = 0;
-    const float4* heads = g.head + begin;
-#pragma unroll 4
-    for (uint32_t k = 0; k < count; k++) {
-        const float4 cd = heads[k];
-        const float e0 = r0 - cd.x, e1 = r1 - cd.y, e2 = r2 - cd.z;
-        mask |= (e0 * e0 + e1 * e1 + e2 * e2 < lim ? 1u : 0u) << k;
-    }
     int pairs = 0;
     while (mask) {
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
@@ -1042,6 +1101,20 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     }
     return pairs;
 }
+
+/* One scan point: lookup, prefilter over the (at most 27) candidate heads with
+ * their loads in flight together, then the pairs.  Returns the number of pairs. */
+template <typename T, int SUM>
+__device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, const PassParams<T>& u, Acc<T, SUM>& acc) {
+    PointLook L;
+    if (!point_lookup(g, p, u.R, u.t, L)) return 0;
+    const float lim = prefilter_limit(u);
+    uint32_t mask = 0;
+#pragma unroll 4
+    for (uint32_t k = 0; k < L.count; k++) mask |= prefilter_bit(g, L, k, lim);
+    return point_pairs<T, SUM>(g, p, L, mask, u, acc);
+}
+
 
 /* Reduces one value per thread per accumulator over the workgroup in a fixed
  * order; thread k < NACC ends with sum k in out[k] (LDS). */
