@@ -161,12 +161,12 @@ def main():
                        refined=refined, train=tinfo, policy=args.policy)
         log(f"batched+learned {S / learned_s:.0f}/s")
 
-    # ---- plain single-scan GPU NDT on the same scans: one launch and host sync per
-    # scan, with the chunk size that minimises single-scan latency (256 points)
+    # ---- plain single-scan GPU NDT on the same scans: one submission and host sync
+    # per scan, with the chunk size that minimises single-scan latency (256 points)
     n_plain = max(1, min(args.plain_scans, S))
     prm_plain = A.params(A.NDT_MODE_NDT, point_precision=point_prec, origin=w.origin, points_per_wg=256)
     plain_scans = [A.Scans(ctx, [w.scans[i]]) for i in range(n_plain)]
-    for i in range(min(4, n_plain)):
+    for i in range(n_plain):  # warm-up: each handle captures its submission once
         plain_scans[i].align(grid, w.guesses[i:i + 1], prm_plain)
     plain_res = []
     t1 = time.perf_counter()

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <atomic>
 #include <string>
 #include <thread>
 #include <vector>
@@ -427,6 +428,8 @@ extern "C" int ndt_grid_build(ndt_ctx* c, const float* map, int64_t m, const ndt
     if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     ndt_grid* g = new (std::nothrow) ndt_grid();
+    static std::atomic<uint64_t> next_serial{1};
+    if (g) g->serial = next_serial.fetch_add(1);
     if (!g) return fail(NDT_ERR_NOMEM, "host allocation failed");
     g->ctx = c;
     g->params = *params;
@@ -515,12 +518,10 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     return a;
 }
 
-/* Fills results from the device outputs of S scans; aborted marks scans that
+/* Fills results from the outputs of S scans (host copy); aborted marks scans that
  * never finished as timed out. */
-static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results) {
-    std::vector<ScanOut> out(S);
-    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
+static int results_from_out(const ScanOut* out, const int64_t* counts, int32_t S, int aborted, ndt_ctx* c,
+                            ndt_result* results) {
     int64_t pairs = 0;
     for (int32_t s = 0; s < S; s++) {
         ndt_result& r = results[s];
@@ -544,6 +545,34 @@ static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int abo
     return 0;
 }
 
+/* Fills results from the device outputs of S scans in the context's buffers. */
+static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results) {
+    std::vector<ScanOut> out(S);
+    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return results_from_out(out.data(), counts, S, aborted, c, results);
+}
+
+/* Chunk descriptors of a batch: chunk size, count and first partial slot per scan. */
+static void make_descs(const int64_t* offsets, const int64_t* counts, int32_t S, const ndt_params* p,
+                       std::vector<ScanDesc>& desc, int64_t* slots_out, int64_t* max_chunks_out) {
+    desc.resize(S);
+    int64_t slots = 0, max_chunks = 0;
+    for (int32_t s = 0; s < S; s++) {
+        const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : align_chunk_points(counts[s]);
+        desc[s].offset = offsets[s];
+        desc[s].n = (int32_t)counts[s];
+        desc[s].chunk = (int32_t)chunk;
+        desc[s].pad = 0;
+        desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
+        desc[s].chunk_base = slots;
+        slots += desc[s].nchunks;
+        max_chunks = std::max<int64_t>(max_chunks, desc[s].nchunks);
+    }
+    *slots_out = slots;
+    *max_chunks_out = max_chunks;
+}
+
 /* Work-queue items needed by a launch: every scan's chunks for each of its
  * possible passes. */
 static uint64_t queue_capacity(int64_t total_chunks, int32_t max_iterations) {
@@ -565,19 +594,9 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
     const int block = p->block_size ? p->block_size : 256;
     AlignParams prm = make_align_params(p, S);
-    std::vector<ScanDesc> desc(S);
+    std::vector<ScanDesc> desc;
     int64_t slots = 0, max_chunks = 0;
-    for (int32_t s = 0; s < S; s++) {
-        const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : align_chunk_points(counts[s]);
-        desc[s].offset = offsets[s];
-        desc[s].n = (int32_t)counts[s];
-        desc[s].chunk = (int32_t)chunk;
-        desc[s].pad = 0;
-        desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
-        desc[s].chunk_base = slots;
-        slots += desc[s].nchunks;
-        max_chunks = std::max<int64_t>(max_chunks, desc[s].nchunks);
-    }
+    make_descs(offsets, counts, S, p, desc, &slots, &max_chunks);
     const uint64_t cap = queue_capacity(slots, p->max_iterations);
     const bool queue = p->scheduler != NDT_SCHED_ROUNDS && cap <= MAX_QUEUE_ITEMS && S < (1 << ITEM_SCAN_BITS) &&
                        max_chunks < (1 << ITEM_CHUNK_BITS);
@@ -615,7 +634,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         LAUNCH_TRY(launch_align_wq(g->dev, pts, c->desc.as<ScanDesc>(), nwg, block, prm,
                                    c->states.as<ScanState>(), c->queue.as<WorkQueue>(), c->items.as<uint64_t>(), cap,
                                    c->done.as<uint32_t>(), c->partials.as<double>(), c->out.as<ScanOut>(),
-                                   c->small.as<double>(), c->stream));
+                                   c->small.as<double>(), 1, c->stream));
         HIP_TRY(hipEventRecord(c->ev1, c->stream));
         double prof[5];
         HIP_TRY(hipMemcpyAsync(prof, c->small.p, sizeof(prof), hipMemcpyDeviceToHost, c->stream));
@@ -773,6 +792,109 @@ extern "C" int ndt_scans_destroy(ndt_scans* sc) {
     return 0;
 }
 
+/* ------------------------------------------------------------------ captured submission */
+
+static bool same_align_params(const ndt_params& a, const ndt_params& b) {
+    return a.mode == b.mode && a.resolution == b.resolution && a.step_size == b.step_size &&
+           a.trans_eps == b.trans_eps && a.max_iterations == b.max_iterations && a.outlier_ratio == b.outlier_ratio &&
+           a.point_precision == b.point_precision && a.sum_precision == b.sum_precision &&
+           a.origin[0] == b.origin[0] && a.origin[1] == b.origin[1] && a.origin[2] == b.origin[2] &&
+           a.block_size == b.block_size && a.points_per_wg == b.points_per_wg && a.scheduler == b.scheduler;
+}
+
+/* the captured launch sequence; the caller ends the capture whatever this returns */
+static int capture_body(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, GraphPlan& gp, const AlignParams& prm,
+                        int64_t slots, uint64_t cap, int32_t nwg, int block) {
+    const int32_t S = sc->S;
+    HIP_TRY(hipMemcpyAsync(gp.guesses.p, gp.h_guess, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_zero_u32(gp.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
+    LAUNCH_TRY(launch_zero_u32(gp.items.p, (int64_t)(2 * cap), c->stream));
+    LAUNCH_TRY(launch_init_states(gp.guesses.as<double>(), S, prm, gp.states.as<ScanState>(), gp.sync.as<RoundSync>(),
+                                  c->stream));
+    LAUNCH_TRY(launch_init_queue(gp.desc.as<ScanDesc>(), S, (uint64_t)slots, gp.queue.as<WorkQueue>(),
+                                 gp.items.as<uint64_t>(), gp.done.as<uint32_t>(), c->stream));
+    LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), gp.desc.as<ScanDesc>(), nwg, block, prm,
+                               gp.states.as<ScanState>(), gp.queue.as<WorkQueue>(), gp.items.as<uint64_t>(), cap,
+                               gp.done.as<uint32_t>(), gp.partials.as<double>(), gp.out.as<ScanOut>(),
+                               gp.prof.as<double>(), 0, c->stream));
+    HIP_TRY(hipMemcpyAsync(gp.h_out, gp.out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(gp.h_prof, gp.prof.p, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+/* Captures the submission of a device-resident batch into a graph (once per scans
+ * handle, grid and parameters).  Returns NDT_ERR_UNSUPPORTED when the batch needs
+ * the round scheduler. */
+static int graph_prepare(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, const ndt_params* p) {
+    GraphPlan& gp = sc->plan;
+    if (gp.valid && gp.grid_serial == g->serial && gp.device == c->device && same_align_params(gp.params, *p))
+        return 0;
+    gp.release();
+    const int32_t S = sc->S;
+    std::vector<ScanDesc> desc;
+    int64_t slots = 0, max_chunks = 0;
+    make_descs(sc->offsets.data(), sc->counts.data(), S, p, desc, &slots, &max_chunks);
+    const uint64_t cap = queue_capacity(slots, p->max_iterations);
+    if (p->scheduler == NDT_SCHED_ROUNDS || cap > MAX_QUEUE_ITEMS || S >= (1 << ITEM_SCAN_BITS) ||
+        max_chunks >= (1 << ITEM_CHUNK_BITS))
+        return NDT_ERR_UNSUPPORTED;
+    const int block = p->block_size ? p->block_size : 256;
+    AlignParams prm = make_align_params(p, S);
+    int bpc = 0;
+    LAUNCH_TRY(align_wq_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
+    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
+    const int32_t nwg = bpc * c->cus;
+    BUF_TRY(gp.desc, sizeof(ScanDesc) * S);
+    BUF_TRY(gp.states, sizeof(ScanState) * S);
+    BUF_TRY(gp.sync, sizeof(RoundSync));
+    BUF_TRY(gp.out, sizeof(ScanOut) * S);
+    BUF_TRY(gp.partials, sizeof(double) * 32 * slots);
+    BUF_TRY(gp.queue, sizeof(WorkQueue));
+    BUF_TRY(gp.items, sizeof(uint64_t) * cap);
+    BUF_TRY(gp.done, sizeof(uint32_t) * S);
+    BUF_TRY(gp.guesses, sizeof(double) * 16 * S);
+    BUF_TRY(gp.prof, sizeof(double) * 8);
+    HIP_TRY(hipHostMalloc((void**)&gp.h_guess, sizeof(double) * 16 * S, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc(&gp.h_out, sizeof(ScanOut) * S, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&gp.h_prof, sizeof(double) * 8, hipHostMallocDefault));
+    HIP_TRY(hipMemcpy(gp.desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int e = capture_body(c, g, sc, gp, prm, slots, cap, nwg, block);
+    hipGraph_t graph = nullptr;
+    hipError_t he = hipStreamEndCapture(c->stream, &graph);
+    if (e) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return e;
+    }
+    if (he != hipSuccess) return fail(NDT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+    gp.graph = graph;
+    HIP_TRY(hipGraphInstantiate(&gp.exec, gp.graph, nullptr, nullptr, 0));
+    gp.valid = true;
+    gp.grid_serial = g->serial;
+    gp.device = c->device;
+    gp.params = *p;
+    return 0;
+}
+
+/* Replays the captured submission with new guesses. */
+static int graph_align(ndt_ctx* c, ndt_scans* sc, const double* guesses, ndt_result* results) {
+    GraphPlan& gp = sc->plan;
+    const int32_t S = sc->S;
+    std::memcpy(gp.h_guess, guesses, sizeof(double) * 16 * S);
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(hipGraphLaunch(gp.exec, c->stream));
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 1;
+    for (int k = 0; k < 4; k++) c->last_profile[k] = gp.h_prof[k];
+    const int aborted = gp.h_prof[4] != 0.0;
+    return results_from_out((const ScanOut*)gp.h_out, sc->counts.data(), S, aborted, c, results);
+}
+
 extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* guesses,
                                const ndt_params* p, ndt_result* results) {
     if (!c || !g || !sc || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
@@ -780,6 +902,11 @@ extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* s
     if (e) return e;
     if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
     HIP_TRY(hipSetDevice(c->device));
+    if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
+    ndt_scans* msc = const_cast<ndt_scans*>(sc);
+    e = graph_prepare(c, g, msc, p);
+    if (e == 0) return graph_align(c, msc, guesses, results);
+    if (e != NDT_ERR_UNSUPPORTED) return e;
     return align_packed(c, g, sc->pts.as<float4>(), sc->offsets.data(), sc->counts.data(), sc->S, guesses, p, results);
 }
 

@@ -96,6 +96,7 @@ struct ndt_ctx {
 
 struct ndt_grid {
     ndt_ctx* ctx = nullptr;
+    uint64_t serial = 0; /* unique per built grid (captured submissions refer to it) */
     ndt_params params;
     ndt_grid_info info;
     /* ndt */
@@ -113,12 +114,43 @@ struct ndt_grid {
     int64_t origin_cell[3] = {0, 0, 0};
 };
 
+/* A captured submission of a device-resident batch (ndt_align_scans): guesses in
+ * from pinned memory, state and queue set-up, the work-queue launch, results out to
+ * pinned memory.  Its buffers are its own, so other calls cannot disturb it. */
+struct GraphPlan {
+    bool valid = false;
+    uint64_t grid_serial = 0;
+    int device = -1;
+    ndt_params params{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf desc, states, sync, out, partials, queue, items, done, guesses, prof;
+    double* h_guess = nullptr;
+    void* h_out = nullptr;
+    double* h_prof = nullptr;
+    void release() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (h_guess) (void)hipHostFree(h_guess);
+        if (h_out) (void)hipHostFree(h_out);
+        if (h_prof) (void)hipHostFree(h_prof);
+        exec = nullptr;
+        graph = nullptr;
+        h_guess = nullptr;
+        h_out = nullptr;
+        h_prof = nullptr;
+        valid = false;
+    }
+    ~GraphPlan() { release(); }
+};
+
 struct ndt_scans {
     ndt_ctx* ctx = nullptr;
     int32_t S = 0;
     std::vector<int64_t> counts, offsets;
     int64_t total = 0;
     DevBuf pts;
+    GraphPlan plan;
 };
 
 #endif

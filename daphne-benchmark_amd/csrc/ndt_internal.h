@@ -408,7 +408,9 @@ int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, W
                       uint32_t* done_cnt, hipStream_t stream);
 int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg, int block_size,
                     AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
-                    double* partials, ScanOut* out, double* prof, hipStream_t stream);
+                    double* partials, ScanOut* out, double* prof, int cooperative, hipStream_t stream);
+/* writes n zero words (32-bit) from dst: a kernel, so that graphs can hold it */
+int launch_zero_u32(void* dst, int64_t n, hipStream_t stream);
 int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
 int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
                        hipStream_t stream);

@@ -1820,13 +1820,36 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     }
 }
 
+/* The work queue needs no co-residency: a workgroup only waits for an item that a
+ * running workgroup will produce (the claims are taken in order by running
+ * workgroups and no running workgroup waits while holding an item), so the kernel
+ * may also be launched normally, which graphs can capture. */
 template <typename T, int SUM, int BLOCK>
 static int launch_align_wq_t(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg,
                              AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap,
-                             uint32_t* done_cnt, double* partials, ScanOut* out, double* prof, hipStream_t stream) {
+                             uint32_t* done_cnt, double* partials, ScanOut* out, double* prof, int cooperative,
+                             hipStream_t stream) {
+    if (!cooperative) {
+        hipLaunchKernelGGL((k_align_wq<T, SUM, BLOCK>), dim3(nwg), dim3(BLOCK), 0, stream, g, pts, scans, prm, states, q,
+                           items, cap, done_cnt, partials, out, prof);
+        return (int)hipGetLastError();
+    }
     void* args[] = {&g, &pts, &scans, &prm, &states, &q, &items, &cap, &done_cnt, &partials, &out, &prof};
     return (int)hipLaunchCooperativeKernel((const void*)k_align_wq<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
                                            stream);
+}
+
+__global__ void k_zero_u32(uint32_t* __restrict__ dst, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = 0u;
+}
+
+int launch_zero_u32(void* dst, int64_t n, hipStream_t stream) {
+    if (n <= 0) return 0;
+    const int64_t want = (n + 255) / 256;
+    const int blocks = (int)(want < 4096 ? want : 4096);
+    hipLaunchKernelGGL(k_zero_u32, dim3(blocks), dim3(256), 0, stream, (uint32_t*)dst, n);
+    return (int)hipGetLastError();
 }
 
 template <typename T, int SUM, int BLOCK>
@@ -1842,9 +1865,10 @@ int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* block
 
 int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int32_t nwg, int block_size,
                     AlignParams prm, ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt,
-                    double* partials, ScanOut* out, double* prof, hipStream_t stream) {
-#define NDT_LA(T, S_, B) \
-    launch_align_wq_t<T, S_, B>(g, pts, scans, nwg, prm, states, q, items, cap, done_cnt, partials, out, prof, stream)
+                    double* partials, ScanOut* out, double* prof, int cooperative, hipStream_t stream) {
+#define NDT_LA(T, S_, B)                                                                                        \
+    launch_align_wq_t<T, S_, B>(g, pts, scans, nwg, prm, states, q, items, cap, done_cnt, partials, out, prof, \
+                                cooperative, stream)
     NDT_DISPATCH(block_size, prm.point_fp64, prm.sum_mode, NDT_LA);
 #undef NDT_LA
 }

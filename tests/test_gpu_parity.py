@@ -261,3 +261,23 @@ def test_fitness_score_matches_oracle(A, orc, ctx, gpu_grid, cfg0):
         for k, m in enumerate(made):
             want, n = orc.fitness_score(mp, m[0], poses[k], rng if rng > 0 else np.inf)
             assert cnt[k] == n and abs(fit[k] - want) <= 1e-9 * want, (k, rng, fit[k], want, cnt[k], n)
+
+
+def test_captured_submission_replays(A, ctx, gpu_grid, cfg0):
+    """Device-resident batches are submitted as a captured graph, replayed for new
+    guesses.  Every replay, with eager work and synchronisations in between, equals
+    the eager (uncaptured) alignment bit for bit."""
+    scene, mp, _ = cfg0
+    made = [A.make_scan(scene, mp, A.scan_cfg(n), 120 + k) for k, n in enumerate([900, 2600, 1700, 3100, 600])]
+    pts = [m[0] for m in made]
+    g1 = np.stack([m[2] for m in made])
+    g2 = A.perturb_poses(np.stack([m[1] for m in made]), np.random.default_rng(3), xy=0.5, yaw_deg=2.0)
+    prm = A.params(A.NDT_MODE_NDT)
+    eager = {1: gpu_grid.align_batch(pts, g1, prm=prm), 2: gpu_grid.align_batch(pts, g2, prm=prm)}
+    sc = A.Scans(ctx, pts)
+    for k, which in enumerate([1, 2, 1, 1, 2]):
+        res = sc.align(gpu_grid, g1 if which == 1 else g2, prm)
+        for r, e in zip(res, eager[which]):
+            assert np.array_equal(r["final_f64"], e["final_f64"]) and r["passes"] == e["passes"], (k, which)
+        # eager work between replays
+        gpu_grid.align_batch(pts[:2], g2[:2], prm=prm)
