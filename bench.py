@@ -110,6 +110,7 @@ def main():
     ap.add_argument("--train-per-scan", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--policy", choices=["race", "fallback"], default="race")
+    ap.add_argument("--no-autotune", action="store_true", help="default block size and chunk instead of the tuned ones")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every box here has one GPU", file=sys.stderr)
@@ -129,6 +130,10 @@ def main():
     ginfo = grid.info()
     scans = A.Scans(ctx, w.scans)
     log(f"workload {gen_s:.1f}s, grid {ginfo['build_ms']:.1f} ms")
+    tuned_ms = None
+    if not args.no_autotune:
+        prm, tuned_ms = scans.autotune(grid, w.guesses, prm)
+        log(f"autotuned: block {prm.block_size}, chunk {prm.points_per_wg} ({tuned_ms:.2f} ms)")
 
     # ---- batched NDT without the proposer (also the roofline measurement)
     for _ in range(args.warmup):
@@ -228,6 +233,8 @@ def main():
         "mean_passes": float(np.mean([r["passes"] for r in final])),
         "mean_passes_batched_plain": float(np.mean([r["passes"] for r in res_b])),
         "grid_build_ms": ginfo["build_ms"],
+        "launch_shape": {"block_size": prm.block_size or 256, "points_per_wg": prm.points_per_wg or 2048,
+                         "autotuned": tuned_ms is not None},
         "profile_us_batched_plain": {k: float(v) for k, v in zip(("lists", "compute", "wait", "control"), profile)},
         "roofline": {
             "bound": "hbm",

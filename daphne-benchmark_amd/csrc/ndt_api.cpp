@@ -927,6 +927,84 @@ int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
     return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results);
 }
 
+/* ------------------------------------------------------------------ autotuner */
+
+static std::string tune_key(ndt_ctx* c, const ndt_scans* sc, const ndt_params* p) {
+    char buf[512];
+    double total = (double)sc->total, mag = 1.0;
+    while (total >= 1000.0 * mag) mag *= 10.0;
+    const double rounded = std::round(total / mag) * mag;
+    std::snprintf(buf, sizeof(buf), "dev=%.120s;cus=%d;pp=%d;sp=%d;S=%d;N=%.0f", c->name, c->cus, p->point_precision,
+                  p->point_precision == NDT_POINT_FP64 ? NDT_SUM_FP64 : p->sum_precision, sc->S, rounded);
+    std::string k(buf);
+    for (auto& ch : k)
+        if (ch == ' ' || ch == '\n' || ch == '\t') ch = '_';
+    return k;
+}
+
+extern "C" int ndt_autotune(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* guesses,
+                            ndt_params* params, const char* cache_path, double* best_ms) {
+    if (!c || !g || !sc || !guesses || !params) return fail(NDT_ERR_INVALID, "null argument");
+    int e = check_params(params);
+    if (e) return e;
+    if (params->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "the autotuner tunes ndt-mode alignment");
+    HIP_TRY(hipSetDevice(c->device));
+    const std::string key = tune_key(c, sc, params);
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "r")) {
+            char line[1024];
+            while (std::fgets(line, sizeof(line), f)) {
+                char k[700];
+                int blk = 0, ch = 0;
+                double ms = 0.0;
+                if (std::sscanf(line, "%699s %d %d %lf", k, &blk, &ch, &ms) == 4 && key == k) {
+                    std::fclose(f);
+                    params->block_size = blk;
+                    params->points_per_wg = ch;
+                    if (best_ms) *best_ms = ms;
+                    return 0;
+                }
+            }
+            std::fclose(f);
+        }
+    }
+    static const int blocks[] = {128, 256, 512};
+    static const int chunks[] = {1024, 2048, 4096};
+    std::vector<ndt_result> res((size_t)sc->S);
+    double best = 1e300;
+    int best_blk = 256, best_ch = 2048;
+    for (int blk : blocks)
+        for (int ch : chunks) {
+            ndt_params q = *params;
+            q.block_size = blk;
+            q.points_per_wg = ch;
+            q.scheduler = NDT_SCHED_QUEUE;
+            double t = 1e300;
+            for (int rep = 0; rep < 3; rep++) { /* the first run captures the submission */
+                e = ndt_align_scans(c, g, sc, guesses, &q, res.data());
+                if (e) return e;
+                if (rep > 0) t = std::min(t, c->last_ms);
+            }
+            if (t < best) {
+                best = t;
+                best_blk = blk;
+                best_ch = ch;
+            }
+        }
+    params->block_size = best_blk;
+    params->points_per_wg = best_ch;
+    if (best_ms) *best_ms = best;
+    if (cache_path) {
+        if (FILE* f = std::fopen(cache_path, "a")) {
+            std::fprintf(f, "%s %d %d %.4f\n", key.c_str(), best_blk, best_ch, best);
+            std::fclose(f);
+        } else {
+            return fail(NDT_ERR_INVALID, std::string("cannot write the autotuner cache ") + cache_path);
+        }
+    }
+    return 0;
+}
+
 /* ------------------------------------------------------------------ fitness score */
 
 extern "C" int ndt_fitness_score(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* poses,

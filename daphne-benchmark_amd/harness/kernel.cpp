@@ -15,10 +15,12 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "benchmark.h"
@@ -64,14 +66,14 @@ public:
     void run(int p = 1) override;
     bool check_output() override;
     ~ndt_mapping() override {
-        if (grid_) ndt_grid_destroy(grid_);
-        if (ctx_) ndt_ctx_destroy(ctx_);
+        for (ndt_ctx* c : ctxs_) ndt_ctx_destroy(c);
     }
 
 protected:
     int read_next_testcases(int count) override;
     void check_next_outputs(int count);
-    CallbackResult partial_points_callback(PointCloud& input_cloud, Matrix4f& init_guess, PointCloud& target_cloud);
+    CallbackResult partial_points_callback(PointCloud& input_cloud, Matrix4f& init_guess, PointCloud& target_cloud,
+                                           ndt_ctx* ctx);
 
 private:
     int read_testcases = 0;
@@ -81,8 +83,7 @@ private:
     std::ifstream input_file, output_file;
     bool error_so_far = false;
     double max_delta = 0.0;
-    ndt_ctx* ctx_ = nullptr;
-    ndt_grid* grid_ = nullptr;
+    std::vector<ndt_ctx*> ctxs_; /* one context (HIP stream) per concurrent testcase */
     bool textbook = false;
 };
 
@@ -119,7 +120,12 @@ void ndt_mapping::init() {
     }
     const char* mode = std::getenv("NDT_HARNESS_MODE");
     textbook = mode && std::strcmp(mode, "ndt") == 0;
-    if (ndt_ctx_create(0, &ctx_) != NDT_OK) die_abi("ndt_ctx_create");
+    /* testcases of a block run concurrently, one context and host thread each */
+    int streams = 4;
+    if (const char* e = std::getenv("NDT_HARNESS_STREAMS")) streams = std::max(1, std::min(32, std::atoi(e)));
+    ctxs_.assign((size_t)streams, nullptr);
+    for (auto& c : ctxs_)
+        if (ndt_ctx_create(0, &c) != NDT_OK) die_abi("ndt_ctx_create");
     error_so_far = false;
     max_delta = 0.0;
     std::cout << "done\n" << std::endl;
@@ -153,25 +159,25 @@ int ndt_mapping::read_next_testcases(int count) {
 }
 
 CallbackResult ndt_mapping::partial_points_callback(PointCloud& input_cloud, Matrix4f& init_guess,
-                                                    PointCloud& target_cloud) {
+                                                    PointCloud& target_cloud, ndt_ctx* ctx) {
     CallbackResult result;
     std::memset(&result, 0, sizeof(result));
     ndt_result r;
     if (!textbook) {
-        if (ndt_compat_callback(ctx_, input_cloud.data()->data, (int64_t)input_cloud.size(), &init_guess.data[0][0],
+        if (ndt_compat_callback(ctx, input_cloud.data()->data, (int64_t)input_cloud.size(), &init_guess.data[0][0],
                                 target_cloud.data()->data, (int64_t)target_cloud.size(), &r) != NDT_OK)
             die_abi("ndt_compat_callback");
     } else {
         ndt_params prm;
         ndt_params_default(&prm, NDT_MODE_NDT);
-        if (grid_) ndt_grid_destroy(grid_);
-        grid_ = nullptr;
-        if (ndt_grid_build(ctx_, target_cloud.data()->data, (int64_t)target_cloud.size(), &prm, &grid_) != NDT_OK)
+        ndt_grid* grid = nullptr;
+        if (ndt_grid_build(ctx, target_cloud.data()->data, (int64_t)target_cloud.size(), &prm, &grid) != NDT_OK)
             die_abi("ndt_grid_build");
         double g[16];
         for (int k = 0; k < 16; k++) g[k] = (&init_guess.data[0][0])[k];
-        if (ndt_align(ctx_, grid_, input_cloud.data()->data, (int64_t)input_cloud.size(), g, &prm, &r) != NDT_OK)
-            die_abi("ndt_align");
+        const int rc = ndt_align(ctx, grid, input_cloud.data()->data, (int64_t)input_cloud.size(), g, &prm, &r);
+        ndt_grid_destroy(grid);
+        if (rc != NDT_OK) die_abi("ndt_align");
     }
     std::memcpy(&result.final_transformation.data[0][0], r.final_transformation, sizeof(float) * 16);
     result.converged = r.converged != 0;
@@ -179,12 +185,29 @@ CallbackResult ndt_mapping::partial_points_callback(PointCloud& input_cloud, Mat
     return result;
 }
 
+/* The reference reads p testcases per block and computes them one after the other
+ * (kernel.cpp:1416-1434).  Here the testcases of a block are spread over the
+ * contexts, one host thread each: a testcase's uploads and host-side controller
+ * steps overlap with other testcases' kernels on other streams.  Testcases are
+ * independent, so the results equal sequential processing. */
 void ndt_mapping::run(int p) {
     pause_func();
     while (read_testcases < (int)testcases) {
         int count = read_next_testcases(p);
         unpause_func();
-        for (int i = 0; i < count; i++) results[i] = partial_points_callback(scans[i], guesses[i], maps[i]);
+        const int workers = std::min<int>((int)ctxs_.size(), count);
+        if (workers <= 1) {
+            for (int i = 0; i < count; i++)
+                results[i] = partial_points_callback(scans[i], guesses[i], maps[i], ctxs_[0]);
+        } else {
+            std::vector<std::thread> pool;
+            for (int w = 0; w < workers; w++)
+                pool.emplace_back([this, w, workers, count]() {
+                    for (int i = w; i < count; i += workers)
+                        results[i] = partial_points_callback(scans[i], guesses[i], maps[i], ctxs_[w]);
+                });
+            for (auto& t : pool) t.join();
+        }
         pause_func();
         check_next_outputs(count);
     }

@@ -425,6 +425,7 @@ def hip_lib():
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.ndt_ctx_last_profile.argtypes = [P, P]
         L.ndt_fitness_score.argtypes = [P, P, P, P, C.c_double, P, P]
+        L.ndt_autotune.argtypes = [P, P, P, P, C.POINTER(NdtParams), C.c_char_p, C.POINTER(C.c_double)]
         # learned warm start (ndt_learn.h)
         L.ndt_bevmap_build.argtypes = [P, P, C.c_int64, P, C.POINTER(P)]
         L.ndt_bevmap_destroy.argtypes = [P]
@@ -457,7 +458,7 @@ EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_
                     "ndt_ctx_device_info", "ndt_grid_build", "ndt_grid_info_get", "ndt_grid_destroy",
                     "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
                     "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing",
-                    "ndt_ctx_last_profile", "ndt_fitness_score")
+                    "ndt_ctx_last_profile", "ndt_fitness_score", "ndt_autotune")
 LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_model_create", "ndt_model_destroy",
                  "ndt_model_dims", "ndt_model_save", "ndt_model_load", "ndt_model_get", "ndt_model_set",
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
@@ -638,6 +639,16 @@ class Scans:
         out = (NdtResult * self.S)()
         _check(hip_lib().ndt_align_scans(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm), out))
         return [_result(out[i]) for i in range(self.S)]
+
+    def autotune(self, grid: Grid, guesses, prm: NdtParams | None = None, cache_path: str | None = None):
+        """Fastest (block size, chunk) for this batch; returns (params, kernel ms)."""
+        import copy
+        prm = copy.copy(prm or grid.params)
+        guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(self.S, 16)
+        ms = C.c_double()
+        _check(hip_lib().ndt_autotune(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm),
+                                      None if cache_path is None else os.fsencode(cache_path), C.byref(ms)))
+        return prm, ms.value
 
     def fitness(self, grid: Grid, poses, max_range: float = 0.0):
         """(fitness (S,), counts (S,)) at world poses (getFitnessScore)."""

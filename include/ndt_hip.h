@@ -188,6 +188,18 @@ int ndt_derivatives(ndt_ctx* ctx, const ndt_grid* grid, const float* scan_xyzw, 
 int ndt_fitness_score(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans, const double* poses,
                       double max_range, double* fitness, int64_t* counts);
 
+/* Autotuner of the batched align launch: times every candidate (block size 128,
+ * 256, 512 x chunk 1024, 2048, 4096 points) on the given device-resident batch
+ * and guesses, and writes the fastest into params->block_size and
+ * params->points_per_wg; best_ms (optional) receives its kernel time.  With a
+ * cache_path, the choice is stored per device, precision and batch shape (scan
+ * count, total points to 3 significant digits) and later calls with the same key
+ * read it back without timing.  The chunk size changes the rounding of the sums:
+ * results depend on it as they depend on any explicit points_per_wg.  No
+ * counterpart in the reference, which processes one testcase at a time. */
+int ndt_autotune(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans, const double* guesses,
+                 ndt_params* params, const char* cache_path, double* best_ms);
+
 /* Timing of the last ndt_align / ndt_align_batch / ndt_align_scans call on this
  * context: device milliseconds of the alignment kernel(s), measured with HIP
  * events on the context's stream, and the number of kernel launches. */

@@ -244,8 +244,12 @@ def test_daphne_harness_end_to_end(tmp_path):
     for f in ("ndt_input.dat", "ndt_output.dat"):
         with gzip.open(os.path.join(repo, "tests", "golden", "daphne_small", f + ".gz")) as src:
             (data / f).write_bytes(src.read())
-    p = subprocess.run([exe, "-p", "2"], cwd=run, capture_output=True, text=True, timeout=120)
-    assert p.returncode == 0 and "result ok" in p.stdout, p.stdout + p.stderr
+    # blocks of 1, 2 and 3 testcases; the testcases of a block run concurrently on
+    # up to NDT_HARNESS_STREAMS contexts
+    for blk, streams in (("1", "4"), ("2", "4"), ("3", "3"), ("3", "1")):
+        env = dict(os.environ, NDT_HARNESS_STREAMS=streams)
+        p = subprocess.run([exe, "-p", blk], cwd=run, capture_output=True, text=True, timeout=120, env=env)
+        assert p.returncode == 0 and "result ok" in p.stdout, (blk, streams, p.stdout + p.stderr)
 
 
 def test_fitness_score_matches_oracle(A, orc, ctx, gpu_grid, cfg0):
@@ -281,3 +285,18 @@ def test_captured_submission_replays(A, ctx, gpu_grid, cfg0):
             assert np.array_equal(r["final_f64"], e["final_f64"]) and r["passes"] == e["passes"], (k, which)
         # eager work between replays
         gpu_grid.align_batch(pts[:2], g2[:2], prm=prm)
+
+
+def test_autotune_chooses_and_caches(A, ctx, gpu_grid, cfg0, tmp_path):
+    scene, mp, _ = cfg0
+    made = [A.make_scan(scene, mp, A.scan_cfg(n), 140 + k) for k, n in enumerate([2500, 4000, 1200, 3300])]
+    sc = A.Scans(ctx, [m[0] for m in made])
+    gs = np.stack([m[2] for m in made])
+    cache = str(tmp_path / "tune.txt")
+    prm, ms = sc.autotune(gpu_grid, gs, A.params(A.NDT_MODE_NDT), cache)
+    assert prm.block_size in (128, 256, 512) and prm.points_per_wg in (1024, 2048, 4096) and ms > 0
+    again, ms2 = sc.autotune(gpu_grid, gs, A.params(A.NDT_MODE_NDT), cache)
+    assert (again.block_size, again.points_per_wg) == (prm.block_size, prm.points_per_wg)
+    assert abs(ms2 - ms) < 1e-3 and len(open(cache).read().splitlines()) == 1
+    res = sc.align(gpu_grid, gs, prm)
+    assert all(r["status"] == 0 for r in res)
