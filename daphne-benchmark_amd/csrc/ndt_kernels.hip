@@ -970,6 +970,33 @@ __device__ inline bool point_lookup(const NdtGridDev& g, const float4 p, const d
     return true;
 }
 
+/* The same lookup split in two for software pipelining over a thread's points:
+ * lookup_issue computes the cell and starts the dense-index load; the entry is
+ * consumed one iteration later, when its latency has passed under the previous
+ * point's work.  Only for grids with a dense index. */
+__device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const double* R, const double* t,
+                                    PointLook& L, uint2& e) {
+#pragma clang fp contract(fast)
+    const double X0 = p.x, X1 = p.y, X2 = p.z;
+    const double xd0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
+    const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
+    const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
+    e = make_uint2(0u, 0u);
+    if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return;
+    const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
+    const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
+    const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
+    const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
+    L.x0 = xd0;
+    L.x1 = xd1;
+    L.x2 = xd2;
+    L.r0 = (float)(xd0 - (double)a0 * g.resolution_d);
+    L.r1 = (float)(xd1 - (double)a1 * g.resolution_d);
+    L.r2 = (float)(xd2 - (double)a2 * g.resolution_d);
+    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= g.dims[0] || c1 >= g.dims[1] || c2 >= g.dims[2]) return;
+    e = g.dense[(c2 * g.dims[1] + c1) * g.dims[0] + c0];
+}
+
 /* range prefilter of candidate k of a point: bit k of the result */
 __device__ inline uint32_t prefilter_bit(const NdtGridDev& g, const PointLook& L, uint32_t k, float lim) {
     const float4 cd = g.head[L.begin + k];
@@ -1753,7 +1780,35 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const PassParams<T> u = load_uniform(pp);
             const int64_t pb = sd.offset + (int64_t)k * sd.chunk;
             const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * sd.chunk);
-            for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+            if (g.dense) {
+                /* software-pipelined: point i + BLOCK's load and index lookup are in
+                 * flight while point i's candidates and pairs are processed */
+                const float lim = prefilter_limit(u);
+                int64_t i = pb + tid;
+                float4 pc = (i < pe) ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+                PointLook lc;
+                uint2 ec;
+                lookup_issue(g, pc, u.R, u.t, lc, ec);
+                for (; i < pe; i += BLOCK) {
+                    const bool more = i + BLOCK < pe;
+                    float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (more) pn = pts[i + BLOCK];
+                    lc.begin = ec.x;
+                    lc.count = ec.y;
+                    uint32_t mask = 0;
+#pragma unroll 4
+                    for (uint32_t k = 0; k < lc.count; k++) mask |= prefilter_bit(g, lc, k, lim);
+                    PointLook ln;
+                    uint2 en = make_uint2(0u, 0u);
+                    if (more) lookup_issue(g, pn, u.R, u.t, ln, en);
+                    if (mask) pairs += point_pairs<T, SUM>(g, pc, lc, mask, u, acc);
+                    pc = pn;
+                    lc = ln;
+                    ec = en;
+                }
+            } else {
+                for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+            }
         }
         double v[NRED];
 #pragma unroll
