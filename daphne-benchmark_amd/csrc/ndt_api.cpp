@@ -30,7 +30,30 @@ static thread_local std::string g_last_error;
 
 int ndt::fail(int code, const std::string& msg) {
     g_last_error = msg;
+    /* the launch wrappers report hipGetLastError(): clear this failure's error so
+     * that it is not reported again by the thread's next launch */
+    (void)hipGetLastError();
     return code;
+}
+
+int ndt::check_poses(const double* poses, int64_t n, const char* what) {
+    for (int64_t i = 0; i < 16 * n; i++)
+        if (!std::isfinite(poses[i]))
+            return fail(NDT_ERR_INVALID, std::string(what) + " " + std::to_string(i / 16) + " has a non-finite entry");
+    return 0;
+}
+
+std::atomic<int64_t>& ndt::device_bytes_held() {
+    static std::atomic<int64_t> held{0};
+    return held;
+}
+
+int64_t ndt::device_memory_limit() {
+    const char* v = std::getenv("NDT_DEVICE_MEMORY_LIMIT");
+    if (!v || !*v) return -1;
+    char* end = nullptr;
+    const long long x = std::strtoll(v, &end, 10);
+    return (end && *end == 0 && x >= 0) ? (int64_t)x : -1;
 }
 
 extern "C" const char* ndt_last_error(void) { return g_last_error.c_str(); }
@@ -112,14 +135,22 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
     return 0;
 }
 
-extern "C" int ndt_ctx_destroy(ndt_ctx* c) {
-    if (!c) return 0;
+void ndt::ctx_retain(ndt_ctx* c) { c->refs.fetch_add(1); }
+
+void ndt::ctx_release(ndt_ctx* c) {
+    if (c->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
+    (void)hipGetLastError();
+}
+
+extern "C" int ndt_ctx_destroy(ndt_ctx* c) {
+    if (!c) return 0;
+    ctx_release(c);
     return 0;
 }
 
@@ -461,6 +492,7 @@ extern "C" int ndt_grid_destroy(ndt_grid* g) {
     if (!g) return 0;
     (void)hipSetDevice(g->ctx->device);
     delete g;
+    (void)hipGetLastError();
     return 0;
 }
 
@@ -503,6 +535,20 @@ extern "C" int ndt_grid_voxels(const ndt_grid* g, int64_t cap, int64_t* keys, in
 
 /* ------------------------------------------------------------------ ndt alignment */
 
+/* Bound of one device-side wait (grid barrier, work-queue claim), in ticks of the
+ * 100 MHz s_memrealtime counter: 2 s, or the environment variable
+ * NDT_WAIT_LIMIT_US (microseconds, for failure-path tests) when set.  An expired
+ * wait aborts the launch, which drains and reports NDT_ERR_TIMEOUT. */
+static uint64_t wait_limit_ticks() {
+    const char* v = std::getenv("NDT_WAIT_LIMIT_US");
+    if (v && *v) {
+        char* end = nullptr;
+        const unsigned long long us = std::strtoull(v, &end, 10);
+        if (end && *end == 0 && us <= 3600000000ull) return (uint64_t)us * 100ull;
+    }
+    return 200000000ull;
+}
+
 static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     AlignParams a;
     std::memset(&a, 0, sizeof(a));
@@ -515,6 +561,7 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     a.point_fp64 = p->point_precision == NDT_POINT_FP64;
     a.sum_mode = a.point_fp64 ? NDT_SUM_FP64 : p->sum_precision;
     a.nscans = S;
+    a.wait_ticks = wait_limit_ticks();
     return a;
 }
 
@@ -745,6 +792,8 @@ extern "C" int ndt_align_batch(ndt_ctx* c, const ndt_grid* g, const float* const
     int e = check_params(p);
     if (e) return e;
     if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    e = check_poses(guesses, S, "guess");
+    if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     std::vector<int64_t> offsets;
     int64_t total = 0;
@@ -789,6 +838,7 @@ extern "C" int ndt_scans_destroy(ndt_scans* sc) {
     if (!sc) return 0;
     (void)hipSetDevice(sc->ctx->device);
     delete sc;
+    (void)hipGetLastError();
     return 0;
 }
 
@@ -827,7 +877,8 @@ static int capture_body(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, GraphPlan&
  * the round scheduler. */
 static int graph_prepare(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, const ndt_params* p) {
     GraphPlan& gp = sc->plan;
-    if (gp.valid && gp.grid_serial == g->serial && gp.device == c->device && same_align_params(gp.params, *p))
+    if (gp.valid && gp.grid_serial == g->serial && gp.device == c->device && same_align_params(gp.params, *p) &&
+        gp.wait_ticks == wait_limit_ticks())
         return 0;
     gp.release();
     const int32_t S = sc->S;
@@ -871,6 +922,7 @@ static int graph_prepare(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, const ndt
     gp.graph = graph;
     HIP_TRY(hipGraphInstantiate(&gp.exec, gp.graph, nullptr, nullptr, 0));
     gp.valid = true;
+    gp.wait_ticks = prm.wait_ticks;
     gp.grid_serial = g->serial;
     gp.device = c->device;
     gp.params = *p;
@@ -901,6 +953,8 @@ extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* s
     int e = check_params(p);
     if (e) return e;
     if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    e = check_poses(guesses, sc->S, "guess");
+    if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
     ndt_scans* msc = const_cast<ndt_scans*>(sc);
@@ -923,6 +977,8 @@ int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
         off[(size_t)k] = sc->offsets[(size_t)idx[k]];
         cnt[(size_t)k] = sc->counts[(size_t)idx[k]];
     }
+    e = check_poses(guesses, n, "guess");
+    if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results);
 }
@@ -948,6 +1004,8 @@ extern "C" int ndt_autotune(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, 
     int e = check_params(params);
     if (e) return e;
     if (params->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "the autotuner tunes ndt-mode alignment");
+    e = check_poses(guesses, sc->S, "guess");
+    if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     const std::string key = tune_key(c, sc, params);
     if (cache_path) {
@@ -1012,6 +1070,8 @@ extern "C" int ndt_fitness_score(ndt_ctx* c, const ndt_grid* g, const ndt_scans*
     if (!c || !g || !sc || !poses || !fitness) return fail(NDT_ERR_INVALID, "null argument");
     if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "fitness needs an ndt-mode grid");
     if (max_range != max_range) return fail(NDT_ERR_INVALID, "max_range is NaN");
+    int e = check_poses(poses, sc->S, "pose");
+    if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     const int32_t S = sc->S;
     const double res = (double)g->params.resolution;

@@ -8,6 +8,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <atomic>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,9 @@ namespace ndt {
 
 /* records the message of a failure for ndt_last_error() and returns code */
 int fail(int code, const std::string& msg);
+
+/* NDT_ERR_INVALID unless every entry of the n row-major 4x4 poses is finite. */
+int check_poses(const double* poses, int64_t n, const char* what);
 
 #define HIP_TRY(expr)                                                                              \
     do {                                                                                           \
@@ -38,12 +43,27 @@ int fail(int code, const std::string& msg);
 
 /* ------------------------------------------------------------------ device buffers */
 
+/* Bytes of device memory held by all DevBufs of the process. */
+std::atomic<int64_t>& device_bytes_held();
+
+/* Injected device-memory limit for failure-path tests: with the environment
+ * variable NDT_DEVICE_MEMORY_LIMIT set to a byte count, a DevBuf allocation that
+ * would take device_bytes_held() above it fails with NDT_ERR_NOMEM exactly as an
+ * exhausted device would.  -1 when unset.  Read at every allocation. */
+int64_t device_memory_limit();
+
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            device_bytes_held() -= (int64_t)bytes;
+        }
         p = nullptr;
         bytes = 0;
     }
@@ -51,6 +71,10 @@ struct DevBuf {
         if (want <= bytes && p) return 0;
         release();
         size_t b = want ? want : 1;
+        const int64_t limit = device_memory_limit();
+        if (limit >= 0 && device_bytes_held().load() + (int64_t)b > limit)
+            return fail(NDT_ERR_NOMEM, std::string("hipMalloc(") + std::to_string(b) +
+                                           "): above NDT_DEVICE_MEMORY_LIMIT=" + std::to_string(limit));
         hipError_t e = hipMalloc(&p, b);
         if (e != hipSuccess) {
             p = nullptr;
@@ -58,6 +82,7 @@ struct DevBuf {
                         std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
         }
         bytes = b;
+        device_bytes_held() += (int64_t)b;
         return 0;
     }
     template <class T> T* as() const { return (T*)p; }
@@ -77,10 +102,37 @@ namespace ndt {
 /* aligns the scans idx[0..n) of a device-resident batch from guesses[0..n) (ndt mode) */
 int align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const int32_t* idx, int32_t n,
                       const double* guesses, const ndt_params* p, ndt_result* results);
+
+/* A context lives until ndt_ctx_destroy has been called and every handle made on it
+ * is destroyed: ndt_ctx_destroy drops the creator's reference, each handle holds
+ * one (CtxRef).  The last release synchronises and destroys the stream. */
+void ctx_retain(ndt_ctx* c);
+void ctx_release(ndt_ctx* c);
+
+/* A handle's reference to its context.  Declared as the handle's first member, it
+ * is released after the handle's device buffers are freed. */
+struct CtxRef {
+    ndt_ctx* p = nullptr;
+    CtxRef() = default;
+    CtxRef(const CtxRef&) = delete;
+    CtxRef& operator=(const CtxRef&) = delete;
+    CtxRef& operator=(ndt_ctx* c) {
+        if (c) ctx_retain(c);
+        if (p) ctx_release(p);
+        p = c;
+        return *this;
+    }
+    ~CtxRef() {
+        if (p) ctx_release(p);
+    }
+    operator ndt_ctx*() const { return p; }
+    ndt_ctx* operator->() const { return p; }
+};
 } // namespace ndt
 
 /* opaque handles of the C ABI */
 struct ndt_ctx {
+    std::atomic<int> refs{1};
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -95,7 +147,7 @@ struct ndt_ctx {
 };
 
 struct ndt_grid {
-    ndt_ctx* ctx = nullptr;
+    ndt::CtxRef ctx;
     uint64_t serial = 0; /* unique per built grid (captured submissions refer to it) */
     ndt_params params;
     ndt_grid_info info;
@@ -120,6 +172,7 @@ struct ndt_grid {
 struct GraphPlan {
     bool valid = false;
     uint64_t grid_serial = 0;
+    uint64_t wait_ticks = 0;
     int device = -1;
     ndt_params params{};
     hipGraph_t graph = nullptr;
@@ -145,7 +198,7 @@ struct GraphPlan {
 };
 
 struct ndt_scans {
-    ndt_ctx* ctx = nullptr;
+    ndt::CtxRef ctx;
     int32_t S = 0;
     std::vector<int64_t> counts, offsets;
     int64_t total = 0;

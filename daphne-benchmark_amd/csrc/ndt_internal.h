@@ -312,6 +312,7 @@ struct AlignParams {
     int32_t point_fp64;
     int32_t sum_mode;
     int32_t nscans;
+    uint64_t wait_ticks; /* bound of one device-side wait, s_memrealtime ticks (100 MHz) */
 };
 
 /* grid-barrier state of one cooperative align launch: arrivals so far (grows by

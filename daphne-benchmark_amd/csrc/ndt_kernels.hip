@@ -1414,8 +1414,6 @@ int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanSt
  * round.  Scans that finish drop out, so the last slow scans get the whole GPU.
  * All waits are bounded. */
 
-/* bound of one grid-barrier wait: 2 s of s_memrealtime (100 MHz) */
-constexpr uint64_t BARRIER_TIMEOUT_TICKS = 200000000ull;
 constexpr int MAX_BATCH = 2048; /* scans per launch (LDS tables) */
 /* waves per SIMD the register allocation must allow: float64 points or float64
  * sums need up to 256 VGPRs to keep the point loop free of spills, float32 points
@@ -1441,7 +1439,7 @@ constexpr int MAX_BATCH = 2048; /* scans per launch (LDS tables) */
  * workgroups are using.  `target` is nwg * (barrier number + 1).  The wait is
  * bounded in time; on expiry the workgroup raises the abort flag, which every
  * other workgroup sees in its own wait.  Returns false after an abort. */
-__device__ inline bool grid_barrier(RoundSync* rs, uint32_t target, int* s_flag) {
+__device__ inline bool grid_barrier(RoundSync* rs, uint32_t target, uint64_t wait_ticks, int* s_flag) {
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0);
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1458,7 +1456,7 @@ __device__ inline bool grid_barrier(RoundSync* rs, uint32_t target, int* s_flag)
                     ok = 0;
                     break;
                 }
-                if (__builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                if (__builtin_amdgcn_s_memrealtime() - t0 > wait_ticks) {
                     __hip_atomic_store(&rs->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     ok = 0;
                     break;
@@ -1592,7 +1590,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         }
         const uint64_t t2 = __builtin_amdgcn_s_memrealtime();
         target += (uint32_t)nwg;
-        ok = grid_barrier(rs, target, &s_flag);
+        ok = grid_barrier(rs, target, prm.wait_ticks, &s_flag);
         if (!ok) break;
         const uint64_t t3 = __builtin_amdgcn_s_memrealtime();
         /* reductions and controllers, scans spread over the workgroups */
@@ -1609,7 +1607,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         if (tid == 0) s_pp_scan = -1;
         const uint64_t t4 = __builtin_amdgcn_s_memrealtime();
         target += (uint32_t)nwg;
-        ok = grid_barrier(rs, target, &s_flag);
+        ok = grid_barrier(rs, target, prm.wait_ticks, &s_flag);
         if (!ok) break;
         const uint64_t t5 = __builtin_amdgcn_s_memrealtime();
         if (wg == 0 && tid == 0) {
@@ -1750,7 +1748,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
                         __hip_atomic_load(&q->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
                         break;
                     __builtin_amdgcn_s_sleep(4);
-                    if ((++spins & 63u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > BARRIER_TIMEOUT_TICKS) {
+                    if ((++spins & 63u) == 0 && __builtin_amdgcn_s_memrealtime() - t0 > prm.wait_ticks) {
                         __hip_atomic_store(&q->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         break;
                     }

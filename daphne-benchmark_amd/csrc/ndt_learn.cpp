@@ -95,7 +95,8 @@ int pick_splits(int M, int N, int K, int cus) {
 /* ------------------------------------------------------------------ handles */
 
 struct ndt_bevmap {
-    ndt_ctx* ctx = nullptr;
+    ndt::CtxRef ctx;
+    std::atomic<int> refs{1}; /* the creator's, plus one per dataset made on it */
     DevBuf pts, start, lut;
     TileGridDev dev{};
     double origin[3] = {0, 0, 0};
@@ -103,7 +104,7 @@ struct ndt_bevmap {
 };
 
 struct ndt_model {
-    ndt_ctx* ctx = nullptr;
+    ndt::CtxRef ctx;
     int f0 = 0, f1 = 0, f2 = 0;
     DevBuf w[6], mom[6], vel[6], g[6];
     DevBuf wb1, wb2, wb2t, wb3;
@@ -115,9 +116,20 @@ struct ndt_model {
     DevBuf sbev, idx, gl, off, cnt;
 };
 
+/* Drops one reference to a BEV map; the last frees it. */
+static void bevmap_release(ndt_bevmap* bm) {
+    if (bm->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(bm->ctx->device);
+    delete bm;
+    (void)hipGetLastError();
+}
+
 struct ndt_dataset {
-    ndt_ctx* ctx = nullptr;
-    const ndt_bevmap* bm = nullptr;
+    ndt::CtxRef ctx;
+    const ndt_bevmap* bm = nullptr; /* retained: training crops its map grids */
+    ~ndt_dataset() {
+        if (bm) bevmap_release(const_cast<ndt_bevmap*>(bm));
+    }
     int64_t n = 0;
     int64_t nscans = 0;
     DevBuf scan_bev; /* nscans grids; capacity grows by doubling */
@@ -213,8 +225,7 @@ extern "C" int ndt_bevmap_build(ndt_ctx* c, const float* map, int64_t m, const d
 
 extern "C" int ndt_bevmap_destroy(ndt_bevmap* bm) {
     if (!bm) return 0;
-    if (bm->ctx) (void)hipSetDevice(bm->ctx->device);
-    delete bm;
+    bevmap_release(bm);
     return 0;
 }
 
@@ -331,6 +342,7 @@ extern "C" int ndt_model_destroy(ndt_model* md) {
     if (!md) return 0;
     if (md->ctx) (void)hipSetDevice(md->ctx->device);
     delete md;
+    (void)hipGetLastError();
     return 0;
 }
 
@@ -585,6 +597,7 @@ extern "C" int ndt_features(ndt_ctx* c, const ndt_bevmap* bm, const ndt_scans* s
     if (B <= 0) return fail(NDT_ERR_INVALID, "B must be positive");
     for (int32_t b = 0; b < B; b++)
         if (scan_index[b] < 0 || scan_index[b] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
+    if (int e = check_poses(guesses, B, "guess")) return e;
     HIP_TRY(hipSetDevice(c->device));
     /* a throw-away model-less workspace: features only need X */
     DevBuf sbev, X, idx, gl, off, cnt;
@@ -646,8 +659,8 @@ static int dataset_add_grids(ndt_dataset* ds, const ndt_scans* sc) {
         std::swap(grown.bytes, ds->scan_bev.bytes);
     }
     ndt_model tmp;
-    tmp.ctx = ds->ctx;
-    DevBuf dst; /* a view of the store's tail */
+    tmp.ctx = (ndt_ctx*)ds->ctx;
+    DevBuf dst; /* staging for the new scans' grids */
     int e = scan_grids(&tmp, ds->bm, sc, dst);
     if (e) return e;
     HIP_TRY(hipMemcpyAsync((char*)ds->scan_bev.p + row * (size_t)ds->nscans, dst.p, row * (size_t)sc->S,
@@ -663,8 +676,11 @@ extern "C" int ndt_dataset_add(ndt_dataset* ds, const ndt_scans* sc, const int32
     if (ds->nscans + sc->S > 0x7fffffff) return fail(NDT_ERR_UNSUPPORTED, "too many scans in one dataset");
     for (int64_t i = 0; i < n; i++)
         if (scan_index[i] < 0 || scan_index[i] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
+    int e = check_poses(guesses, n, "guess");
+    if (!e) e = check_poses(targets, n, "target");
+    if (e) return e;
     HIP_TRY(hipSetDevice(ds->ctx->device));
-    int e = dataset_add_grids(ds, sc);
+    e = dataset_add_grids(ds, sc);
     if (e) return e;
     std::vector<double> gl;
     guesses_local(ds->bm, guesses, n, gl);
@@ -686,6 +702,7 @@ extern "C" int ndt_dataset_create(ndt_ctx* c, const ndt_bevmap* bm, const ndt_sc
     *out = nullptr;
     ndt_dataset* ds = new ndt_dataset();
     ds->ctx = c;
+    const_cast<ndt_bevmap*>(bm)->refs.fetch_add(1);
     ds->bm = bm;
     int e = ndt_dataset_add(ds, sc, scan_index, guesses, targets, n);
     if (e) {
@@ -707,6 +724,7 @@ extern "C" int ndt_dataset_destroy(ndt_dataset* ds) {
     if (!ds) return 0;
     if (ds->ctx) (void)hipSetDevice(ds->ctx->device);
     delete ds;
+    (void)hipGetLastError();
     return 0;
 }
 
@@ -820,6 +838,7 @@ static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const nd
 extern "C" int ndt_model_predict(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const ndt_scans* sc,
                                  const double* guesses, double* out_guesses) {
     if (!c || !md || !bm || !sc || !guesses || !out_guesses) return fail(NDT_ERR_INVALID, "null argument");
+    if (int e = check_poses(guesses, sc->S, "guess")) return e;
     HIP_TRY(hipSetDevice(c->device));
     return predict_dev(c, md, bm, sc, guesses, out_guesses);
 }
@@ -829,6 +848,7 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
                                        ndt_result* results, int32_t* used_learned) {
     if (!c || !g || !bm || !md || !sc || !guesses || !p || !results) return fail(NDT_ERR_INVALID, "null argument");
     if (policy != NDT_LEARNED_RACE && policy != NDT_LEARNED_FALLBACK) return fail(NDT_ERR_INVALID, "unknown policy");
+    if (int e = check_poses(guesses, sc->S, "guess")) return e;
     HIP_TRY(hipSetDevice(c->device));
     const int S = sc->S;
     double ms_total = 0.0;

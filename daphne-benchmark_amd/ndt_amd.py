@@ -274,19 +274,37 @@ def write_ndt_input(path, cases) -> None:
                 f.write(cloud.tobytes())
 
 
+class DataFormatError(ValueError):
+    """A DAPHNE data file that ends inside a record (or holds a negative point
+    count).  The message is the reference's for that record
+    (src/CPU/ndt_mapping/kernel.cpp:157-226)."""
+
+
+def _unpack(fmt, data, pos, what):
+    if pos + struct.calcsize(fmt) > len(data):
+        raise DataFormatError(what)
+    return struct.unpack_from(fmt, data, pos)
+
+
+def _floats(data, count, pos, what):
+    if count < 0 or pos + 4 * count > len(data):
+        raise DataFormatError(what)
+    return np.frombuffer(data, "<f4", count, pos).copy()
+
+
 def read_ndt_input(path) -> list:
     with _open(path, "rb") as f:
         data = f.read()
-    (t,) = struct.unpack_from("<i", data, 0)
+    (t,) = _unpack("<i", data, 0, "Error reading number of test cases")
     pos, cases = 4, []
     for _ in range(t):
-        guess = np.frombuffer(data, "<f4", 16, pos).reshape(4, 4).copy()
+        guess = _floats(data, 16, pos, "Error reading initial guess").reshape(4, 4)
         pos += 64
         clouds = []
         for _k in range(2):
-            (n,) = struct.unpack_from("<i", data, pos)
+            (n,) = _unpack("<i", data, pos, "Error reading filtered scan")
             pos += 4
-            clouds.append(np.frombuffer(data, "<f4", 4 * n, pos).reshape(n, 4).copy())
+            clouds.append(_floats(data, 4 * n, pos, "Error reading filtered scan").reshape(n, 4))
             pos += 16 * n
         cases.append(Testcase(guess, clouds[0], clouds[1]))
     return cases
@@ -303,6 +321,8 @@ def write_ndt_output(path, results) -> None:
 def read_ndt_output(path) -> list:
     with _open(path, "rb") as f:
         data = f.read()
+    if len(data) % 73:
+        raise DataFormatError("Error reading reference result")
     out = []
     for i in range(len(data) // 73):
         rec = data[73 * i: 73 * (i + 1)]

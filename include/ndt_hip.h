@@ -35,9 +35,11 @@ extern "C" {
 
 typedef enum ndt_status {
     NDT_OK = 0,
-    NDT_ERR_INVALID = 1,     /* bad argument (null pointer, empty cloud, bad parameter) */
+    NDT_ERR_INVALID = 1,     /* bad argument (null pointer, empty cloud, bad parameter, a
+                                guess or pose with a non-finite entry) */
     NDT_ERR_HIP = 2,         /* HIP runtime error */
-    NDT_ERR_NOMEM = 3,       /* device allocation failed */
+    NDT_ERR_NOMEM = 3,       /* device allocation failed (or would exceed the byte count in
+                                the environment variable NDT_DEVICE_MEMORY_LIMIT) */
     NDT_ERR_TIMEOUT = 4,     /* a device-side wait exceeded its bound */
     NDT_ERR_UNSUPPORTED = 5, /* map extent or sizes beyond what the grid encodes */
 } ndt_status;
@@ -118,6 +120,9 @@ const char* ndt_version(void);
  * Replaces the reference's per-process setup in ndt_mapping::init()
  * (src/CPU/ndt_mapping/kernel.cpp:345-377) for the compute side. */
 int ndt_ctx_create(int32_t device, ndt_ctx** out);
+/* Ends the caller's use of the context.  Grids, scan batches and learned-model
+ * handles made on it keep it alive until they are destroyed, so they may be
+ * destroyed in any order; it is not passed to other calls after this. */
 int ndt_ctx_destroy(ndt_ctx* ctx);
 /* Number of compute units and the device name. */
 int ndt_ctx_device_info(ndt_ctx* ctx, int32_t* compute_units, char* name, int32_t name_len);

@@ -103,7 +103,8 @@ void ndt_train_params_default(ndt_train_params* tp);
 
 /* Training triples: scan scan_index[i] of `scans`, the guess guesses[i] and the
  * target pose targets[i] (an NDT-converged pose), n samples.  Scan grids are
- * computed once and kept on the device with the guesses and scaled corrections. */
+ * computed once and kept on the device with the guesses and scaled corrections.
+ * The dataset holds `bm` (training crops its map grids) until it is destroyed. */
 int ndt_dataset_create(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
                        const double* guesses, const double* targets, int64_t n, ndt_dataset** out);
 /* Appends triples that refer to another scan batch (same map), so that datasets

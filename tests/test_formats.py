@@ -65,3 +65,72 @@ def test_small_daphne_fixture(A):
     assert ok and md == 0.0
     bad = [A.CallbackResult(o.converged, o.final_transformation + np.float32(0.01)) for o in outs]
     assert not A.check_outputs(bad, outs)[0]
+
+
+def _small_input_bytes():
+    import gzip
+    with gzip.open(os.path.join(REPO, "tests", "golden", "daphne_small", "ndt_input.dat.gz")) as f:
+        return f.read()
+
+
+def test_truncated_input_raises_reference_messages(A, tmp_path):
+    """A file cut inside a record raises DataFormatError with the reference's message
+    for that record (kernel.cpp:157-226); so does a negative point count."""
+    import struct
+    import pytest
+    data = _small_input_bytes()
+    (n_scan,) = struct.unpack_from("<i", data, 4 + 64)
+    cuts = {2: "Error reading number of test cases", 4 + 30: "Error reading initial guess",
+            4 + 64 + 2: "Error reading filtered scan", 4 + 64 + 4 + 16 * n_scan - 8: "Error reading filtered scan",
+            4 + 64 + 4 + 16 * n_scan + 4 + 40: "Error reading filtered scan", len(data) - 1: "Error reading filtered scan"}
+    for cut, msg in cuts.items():
+        p = tmp_path / f"cut{cut}.dat"
+        p.write_bytes(data[:cut])
+        with pytest.raises(A.DataFormatError, match=msg):
+            A.read_ndt_input(p)
+    neg = bytearray(data)
+    struct.pack_into("<i", neg, 4 + 64, -5)
+    (tmp_path / "neg.dat").write_bytes(bytes(neg))
+    with pytest.raises(A.DataFormatError, match="filtered scan"):
+        A.read_ndt_input(tmp_path / "neg.dat")
+    # an empty testcase list and an empty scan are well-formed
+    (tmp_path / "zero.dat").write_bytes(struct.pack("<i", 0))
+    assert A.read_ndt_input(tmp_path / "zero.dat") == []
+    empty = A.Testcase(np.eye(4, dtype=np.float32), np.zeros((0, 4), np.float32), np.ones((3, 4), np.float32))
+    A.write_ndt_input(tmp_path / "empty.dat", [empty])
+    back = A.read_ndt_input(tmp_path / "empty.dat")
+    assert back[0].scan.shape == (0, 4) and back[0].map.shape == (3, 4)
+
+
+def test_truncated_output_raises(A, tmp_path):
+    import pytest
+    A.write_ndt_output(tmp_path / "o.dat", [A.CallbackResult(True, np.eye(4, dtype=np.float32), 0.5)] * 2)
+    data = (tmp_path / "o.dat").read_bytes()
+    (tmp_path / "t.dat").write_bytes(data[:-10])
+    with pytest.raises(A.DataFormatError, match="reference result"):
+        A.read_ndt_output(tmp_path / "t.dat")
+    assert len(A.read_ndt_output(tmp_path / "o.dat")) == 2
+
+
+def test_harness_rejects_missing_or_truncated_files(tmp_path):
+    """The testcase binary exits with the reference's status -3 (kernel.cpp:211-214,
+    :345-366) and message when a data file is missing or its header is cut, before it
+    touches a GPU."""
+    import subprocess
+    import pytest
+    exe = os.path.join(REPO, "daphne-benchmark_amd", "ndt_mapping_hip")
+    if not os.path.exists(exe):
+        pytest.skip("harness not built")
+    data = tmp_path / "data"
+    run = tmp_path / "a" / "b" / "c"
+    data.mkdir()
+    run.mkdir(parents=True)
+    env = dict(os.environ, DAPHNE_DATA_DIR=str(data))
+    p = subprocess.run([exe], cwd=run, capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 253 and "Error opening the testcase file" in p.stderr, p.stderr
+    (data / "ndt_input.dat").write_bytes(b"\x01\x00")
+    (data / "ndt_output.dat").write_bytes(b"")
+    p = subprocess.run([exe], cwd=run, capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 253 and "Error reading number of test cases" in p.stderr, p.stderr
+    p = subprocess.run([exe, "-p", "0"], cwd=run, capture_output=True, text=True, timeout=60, env=env)
+    assert p.returncode == 4 and "Usage" in p.stdout

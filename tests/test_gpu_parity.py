@@ -155,6 +155,150 @@ def test_invalid_inputs(A, ctx, gpu_grid):
         A.Grid(ctx, np.full((10, 4), np.nan, np.float32), A.params(A.NDT_MODE_NDT))
 
 
+def test_non_finite_guess_rejected(A, ctx, gpu_grid, cfg0):
+    """A guess or pose with a NaN or infinite entry is a bad argument for every entry
+    point that takes one; the context stays usable."""
+    scene, mp, _ = cfg0
+    scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(800), 11)
+    sc = A.Scans(ctx, [scan, scan])
+    bm = A.BevMap(ctx, mp)
+    for bad in (np.nan, np.inf, -np.inf):
+        g = np.stack([guess, guess])
+        g[1, 0, 3 if bad != bad else 1] = bad
+        calls = [lambda: gpu_grid.align(scan, g[1]), lambda: sc.align(gpu_grid, g),
+                 lambda: gpu_grid.align_batch([scan, scan], g), lambda: sc.fitness(gpu_grid, g, 0.0),
+                 lambda: bm.features(sc, np.array([0, 1], np.int32), g)]
+        for k, call in enumerate(calls):
+            with pytest.raises(A.NdtError) as e:
+                call()
+            assert e.value.code == 1 and "non-finite" in str(e.value), (bad, k)
+    r = gpu_grid.align(scan, guess)
+    assert r["status"] == 0 and r["iterations"] > 0
+
+
+def test_degenerate_scans_match_oracle(A, orc, gpu_grid, cfg0):
+    """Scans whose Hessian is singular or rank-deficient (one point; points on a
+    line; points in one cell) take the same path as the oracle: same iteration count
+    and convergence, same pose."""
+    scene, mp, og = cfg0
+    scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(2000), 0)
+    line = np.zeros((50, 4), np.float32)
+    line[:, 0] = np.linspace(0, 5, 50)
+    line[:, 3] = 1
+    blob = np.tile(scan[:1], (64, 1))
+    for name, s in (("one", scan[:1]), ("two", scan[:2]), ("line", line), ("blob", blob)):
+        r = gpu_grid.align(s, guess)
+        o = og.align(s, guess)
+        dt, dr = orc.pose_error(r["final_f64"], o["final_f64"])
+        assert r["iterations"] == o["iterations"] and r["converged"] == o["converged"], name
+        assert dt <= 1e-6 and dr <= 1e-7 and np.isfinite(r["final_f64"]).all(), (name, dt, dr)
+        assert abs(r["score"] - o["score"]) <= 1e-9 * max(abs(o["score"]), 1.0), name
+
+
+def test_map_without_valid_cells(A, orc, ctx, cfg0):
+    """A map too sparse for any cell to hold the minimum point count builds a grid
+    with no valid cell; alignments then find no correspondences and stop at the guess
+    (as the oracle)."""
+    scene, mp, _ = cfg0
+    sparse = mp[:: len(mp) // 3][:3].copy()
+    g = A.Grid(ctx, sparse, A.params(A.NDT_MODE_NDT))
+    info = g.info()
+    assert info["voxels"] >= 1 and info["valid_voxels"] == 0
+    scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(600), 3)
+    r = g.align(scan, guess)
+    o = orc.NdtGrid(sparse).align(scan, guess)
+    assert r["iterations"] == o["iterations"] == 0 and r["converged"] == o["converged"]
+    assert r["score"] == o["score"] == 0.0 and np.allclose(r["final_f64"], guess, atol=1e-9)
+
+
+def test_handles_outlive_their_closed_context(A, ctx, gpu_grid, cfg0):
+    """Destroying a context before the grids, scan batches and BEV maps made on it is
+    safe: each handle keeps the context until it is destroyed itself, and nothing is
+    left behind for the next call on another context to report."""
+    scene, mp, _ = cfg0
+    scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(900), 12)
+    want = gpu_grid.align(scan, guess)
+    c2 = A.Context(0)
+    g2 = A.Grid(c2, mp, A.params(A.NDT_MODE_NDT))
+    s2 = A.Scans(c2, [scan])
+    b2 = A.BevMap(c2, mp)
+    got = s2.align(g2, guess[None])
+    c2.close()
+    g2.close()
+    s2.close()
+    b2.close()
+    r = gpu_grid.align(scan, guess)
+    assert r["status"] == 0 and np.array_equal(r["final_f64"], want["final_f64"])
+    assert np.array_equal(got[0]["final_f64"], want["final_f64"])
+
+
+def test_device_wait_timeout_is_reported(A, ctx, gpu_grid, cfg0, monkeypatch):
+    """NDT_WAIT_LIMIT_US=1 bounds every device-side wait (grid barrier, work-queue
+    claim) to 1 us.  Each scan is one 65536-point chunk, so idle workgroups wait far
+    longer than that: the wait expires, the launch drains, and the call returns
+    NDT_ERR_TIMEOUT (4) with both schedulers and the captured submission.  With the
+    limit lifted the same context and batch give the normal results again."""
+    scene, mp, _ = cfg0
+    made = [A.make_scan(scene, mp, A.scan_cfg(n), 140 + k) for k, n in enumerate([6000, 3000])]
+    scans = [m[0] for m in made]
+    guesses = np.stack([m[2] for m in made])
+    prm = {s: A.params(A.NDT_MODE_NDT, scheduler=s, points_per_wg=65536)
+           for s in (A.NDT_SCHED_QUEUE, A.NDT_SCHED_ROUNDS)}
+    want = gpu_grid.align_batch(scans, guesses, prm=prm[A.NDT_SCHED_QUEUE])
+    assert all(w["iterations"] >= 2 for w in want)
+    sc = A.Scans(ctx, scans)
+    try:
+        monkeypatch.setenv("NDT_WAIT_LIMIT_US", "1")
+        for sched, p in prm.items():
+            with pytest.raises(A.NdtError) as e:
+                gpu_grid.align_batch(scans, guesses, prm=p)
+            assert e.value.code == 4, sched
+        with pytest.raises(A.NdtError) as e:
+            sc.align(gpu_grid, guesses, prm[A.NDT_SCHED_QUEUE])
+        assert e.value.code == 4
+    finally:
+        monkeypatch.delenv("NDT_WAIT_LIMIT_US", raising=False)
+    runs = [sc.align(gpu_grid, guesses, prm[A.NDT_SCHED_QUEUE])]
+    runs += [gpu_grid.align_batch(scans, guesses, prm=p) for p in prm.values()]
+    for res in runs:
+        for r, w in zip(res, want):
+            assert r["status"] == 0 and np.array_equal(r["final_f64"], w["final_f64"])
+
+
+def test_injected_allocation_failure_recovers(A, ctx, gpu_grid, cfg0, monkeypatch):
+    """With NDT_DEVICE_MEMORY_LIMIT below what a call needs, grid building, scan
+    upload and alignment fail with NDT_ERR_NOMEM (3) and leave nothing half-built;
+    once the limit is lifted the same context gives the normal results."""
+    scene, mp, _ = cfg0
+    made = [A.make_scan(scene, mp, A.scan_cfg(n), 40 + k) for k, n in enumerate([1200, 700])]
+    scans = [m[0] for m in made]
+    guesses = np.stack([m[2] for m in made])
+    want = A.Scans(ctx, scans).align(gpu_grid, guesses)
+    fresh = A.Context(0)
+    try:
+        monkeypatch.setenv("NDT_DEVICE_MEMORY_LIMIT", "0")
+        with pytest.raises(A.NdtError) as e:
+            A.Grid(fresh, mp, A.params(A.NDT_MODE_NDT))
+        assert e.value.code == 3 and "NDT_DEVICE_MEMORY_LIMIT" in str(e.value)
+        with pytest.raises(A.NdtError) as e:
+            A.Scans(fresh, scans)
+        assert e.value.code == 3
+        monkeypatch.delenv("NDT_DEVICE_MEMORY_LIMIT")
+        grid = A.Grid(fresh, mp, A.params(A.NDT_MODE_NDT))
+        sc = A.Scans(fresh, scans)
+        monkeypatch.setenv("NDT_DEVICE_MEMORY_LIMIT", "0")
+        with pytest.raises(A.NdtError) as e:
+            sc.align(grid, guesses)
+        assert e.value.code == 3
+        monkeypatch.delenv("NDT_DEVICE_MEMORY_LIMIT")
+        got = sc.align(grid, guesses)
+        for r, w in zip(got, want):
+            assert np.array_equal(r["final_f64"], w["final_f64"]) and r["iterations"] == w["iterations"]
+    finally:
+        monkeypatch.delenv("NDT_DEVICE_MEMORY_LIMIT", raising=False)
+        fresh.close()
+
+
 def test_nan_points_are_ignored(A, orc, gpu_grid, cfg0):
     scene, mp, og = cfg0
     scan, gt, guess = A.make_scan(scene, mp, A.scan_cfg(1500), 5)
