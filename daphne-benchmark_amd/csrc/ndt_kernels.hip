@@ -974,15 +974,15 @@ __device__ inline bool point_lookup(const NdtGridDev& g, const float4 p, const d
  * lookup_issue computes the cell and starts the dense-index load; the entry is
  * consumed one iteration later, when its latency has passed under the previous
  * point's work.  Only for grids with a dense index. */
-__device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const double* R, const double* t,
-                                    PointLook& L, uint2& e) {
+__device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const bool valid, const double* R,
+                                    const double* t, PointLook& L, uint2& e) {
 #pragma clang fp contract(fast)
     const double X0 = p.x, X1 = p.y, X2 = p.z;
     const double xd0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
     const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
     const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
     e = make_uint2(0u, 0u);
-    if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return;
+    if (!valid || !(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return;
     const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
     const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
     const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
@@ -1011,121 +1011,202 @@ __device__ inline float prefilter_limit(const PassParams<T>& u) {
     return (sizeof(T) == 4) ? (float)u.res2 : (float)u.res2 * 1.0001f + 1e-5f;
 }
 
-/* Score / gradient / Hessian (Magnusson 2009 eqs. 6.9-6.13) of the candidates of
- * one looked-up point selected by mask whose mean lies within one resolution.
- * Returns the number of pairs. */
+/* Score / gradient / Hessian terms (Magnusson 2009 eqs. 6.9-6.13) of one
+ * (point, candidate) pair, added when the candidate's mean lies within one
+ * resolution of the point: xd is the transformed point (float64), r its position
+ * relative to its cell's lower corner (float32), p the point in the sensor frame.
+ * Returns 1 when the pair contributes. */
 template <typename T, int SUM>
-__device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, uint32_t mask,
-                                  const PassParams<T>& u, Acc<T, SUM>& acc) {
+__device__ inline int pair_add(const NdtGridDev& g, const double xd0, const double xd1, const double xd2,
+                               const float r0, const float r1, const float r2, const float4 p, const uint32_t cand,
+                               const PassParams<T>& u, Acc<T, SUM>& acc) {
 #pragma clang fp contract(fast)
-    const double xd0 = L.x0, xd1 = L.x1, xd2 = L.x2;
-    const float r0 = L.r0, r1 = L.r1, r2 = L.r2;
-    const uint32_t begin = L.begin;
-    const float4* heads = g.head + begin;
-    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
+    T d0, d1, d2;
+    T c00, c01, c02, c11, c12, c22;
+    if constexpr (sizeof(T) == 4) {
+        const float4 h = g.head[cand];
+        const float4* r = g.s32 + 2 * (int64_t)cand;
+        const float4 a = r[0], b = r[1];
+        d0 = r0 - h.x;
+        d1 = r1 - h.y;
+        d2 = r2 - h.z;
+        if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) return 0;
+        c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
+    } else {
+        const double2* r = g.s64 + 5 * (int64_t)cand;
+        const double2 m01 = r[0], m2c0 = r[1], c12v = r[2], c34 = r[3], c5p = r[4];
+        d0 = xd0 - m01.x;
+        d1 = xd1 - m01.y;
+        d2 = xd2 - m2c0.x;
+        if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) return 0;
+        c00 = m2c0.y; c01 = c12v.x; c02 = c12v.y; c11 = c34.x; c12 = c34.y; c22 = c5p.x;
+    }
     /* point Jacobian columns J3 = e_x x ya, J4 = a4 x ya, J5 = a5 x ya */
+    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
     const T y0 = u.Ra[0] * x0 + u.Ra[1] * x1 + u.Ra[2] * x2;
     const T y1 = u.Ra[3] * x0 + u.Ra[4] * x1 + u.Ra[5] * x2;
     const T y2 = u.Ra[6] * x0 + u.Ra[7] * x1 + u.Ra[8] * x2;
     const T J3y = -y2, J3z = y1;
     const T J4x = u.a4[1] * y2 - u.a4[2] * y1, J4y = u.a4[2] * y0 - u.a4[0] * y2, J4z = u.a4[0] * y1 - u.a4[1] * y0;
     const T J5x = u.a5[1] * y2 - u.a5[2] * y1, J5y = u.a5[2] * y0 - u.a5[0] * y2, J5z = u.a5[0] * y1 - u.a5[1] * y0;
+    /* q = C x' */
+    const T q0 = c00 * d0 + c01 * d1 + c02 * d2;
+    const T q1 = c01 * d0 + c11 * d1 + c12 * d2;
+    const T q2 = c02 * d0 + c12 * d1 + c22 * d2;
+    const T xCx = d0 * q0 + d1 * q1 + d2 * q2;
+    T e;
+    if constexpr (sizeof(T) == 4) e = expf(u.nhd2 * xCx);
+    else e = exp(u.nhd2 * xCx);
+    const T e2 = u.d2 * e;
+    /* invalid-value guard, kernel.cpp:456-457 */
+    if (e2 > (T)1 || e2 < (T)0 || e2 != e2) return 0;
+    const T f = e2 * u.d1;
+    const T m = -u.d2;
+    /* x'^T C J_k */
+    const T u3 = q1 * J3y + q2 * J3z;
+    const T u4 = q0 * J4x + q1 * J4y + q2 * J4z;
+    const T u5 = q0 * J5x + q1 * J5y + q2 * J5z;
+    /* C J_k */
+    const T k30 = c01 * J3y + c02 * J3z, k31 = c11 * J3y + c12 * J3z, k32 = c12 * J3y + c22 * J3z;
+    const T k40 = c00 * J4x + c01 * J4y + c02 * J4z;
+    const T k41 = c01 * J4x + c11 * J4y + c12 * J4z;
+    const T k42 = c02 * J4x + c12 * J4y + c22 * J4z;
+    const T k50 = c00 * J5x + c01 * J5y + c02 * J5z;
+    const T k51 = c01 * J5x + c11 * J5y + c12 * J5z;
+    const T k52 = c02 * J5x + c12 * J5y + c22 * J5z;
+    /* J_b^T C J_a */
+    const T j33 = J3y * k31 + J3z * k32;
+    const T j34 = J4x * k30 + J4y * k31 + J4z * k32;
+    const T j35 = J5x * k30 + J5y * k31 + J5z * k32;
+    const T j44 = J4x * k40 + J4y * k41 + J4z * k42;
+    const T j45 = J5x * k40 + J5y * k41 + J5z * k42;
+    const T j55 = J5x * k50 + J5y * k51 + J5z * k52;
+    /* x'^T C H_ab = J_b . (q x a_a) */
+    const T h33 = q2 * J3y - q1 * J3z;
+    const T h34 = q2 * J4y - q1 * J4z;
+    const T h35 = q2 * J5y - q1 * J5z;
+    const T qa40 = q1 * u.a4[2] - q2 * u.a4[1], qa41 = q2 * u.a4[0] - q0 * u.a4[2], qa42 = q0 * u.a4[1] - q1 * u.a4[0];
+    const T qa50 = q1 * u.a5[2] - q2 * u.a5[1], qa51 = q2 * u.a5[0] - q0 * u.a5[2], qa52 = q0 * u.a5[1] - q1 * u.a5[0];
+    const T h44 = qa40 * J4x + qa41 * J4y + qa42 * J4z;
+    const T h45 = qa40 * J5x + qa41 * J5y + qa42 * J5z;
+    const T h55 = qa50 * J5x + qa51 * J5y + qa52 * J5z;
+    acc.add(0, -u.d1 * e);
+    acc.add(1, f * q0);
+    acc.add(2, f * q1);
+    acc.add(3, f * q2);
+    acc.add(4, f * u3);
+    acc.add(5, f * u4);
+    acc.add(6, f * u5);
+    const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
+    acc.add(7, f * (mu0 * q0 + c00));
+    acc.add(8, f * (mu0 * q1 + c01));
+    acc.add(9, f * (mu0 * q2 + c02));
+    acc.add(10, f * (mu0 * u3 + k30));
+    acc.add(11, f * (mu0 * u4 + k40));
+    acc.add(12, f * (mu0 * u5 + k50));
+    acc.add(13, f * (mu1 * q1 + c11));
+    acc.add(14, f * (mu1 * q2 + c12));
+    acc.add(15, f * (mu1 * u3 + k31));
+    acc.add(16, f * (mu1 * u4 + k41));
+    acc.add(17, f * (mu1 * u5 + k51));
+    acc.add(18, f * (mu2 * q2 + c22));
+    acc.add(19, f * (mu2 * u3 + k32));
+    acc.add(20, f * (mu2 * u4 + k42));
+    acc.add(21, f * (mu2 * u5 + k52));
+    acc.add(22, f * (mu3 * u3 + h33 + j33));
+    acc.add(23, f * (mu3 * u4 + h34 + j34));
+    acc.add(24, f * (mu3 * u5 + h35 + j35));
+    acc.add(25, f * (mu4 * u4 + h44 + j44));
+    acc.add(26, f * (mu4 * u5 + h45 + j45));
+    acc.add(27, f * (m * u5 * u5 + h55 + j55));
+    return 1;
+}
+
+/* The candidates of one looked-up point selected by mask.  Returns the number of
+ * pairs. */
+template <typename T, int SUM>
+__device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, uint32_t mask,
+                                  const PassParams<T>& u, Acc<T, SUM>& acc) {
     int pairs = 0;
     while (mask) {
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1;
-        T d0, d1, d2;
-        T c00, c01, c02, c11, c12, c22;
-        if constexpr (sizeof(T) == 4) {
-            const float4 h = heads[k];
-            const float4* r = g.s32 + 2 * (int64_t)(begin + k);
-            const float4 a = r[0], b = r[1];
-            d0 = r0 - h.x;
-            d1 = r1 - h.y;
-            d2 = r2 - h.z;
-            if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) continue;
-            c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
-        } else {
-            const double2* r = g.s64 + 5 * (int64_t)(begin + k);
-            const double2 m01 = r[0], m2c0 = r[1], c12v = r[2], c34 = r[3], c5p = r[4];
-            d0 = xd0 - m01.x;
-            d1 = xd1 - m01.y;
-            d2 = xd2 - m2c0.x;
-            if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) continue;
-            c00 = m2c0.y; c01 = c12v.x; c02 = c12v.y; c11 = c34.x; c12 = c34.y; c22 = c5p.x;
-        }
-        /* q = C x' */
-        const T q0 = c00 * d0 + c01 * d1 + c02 * d2;
-        const T q1 = c01 * d0 + c11 * d1 + c12 * d2;
-        const T q2 = c02 * d0 + c12 * d1 + c22 * d2;
-        const T xCx = d0 * q0 + d1 * q1 + d2 * q2;
-        T e;
-        if constexpr (sizeof(T) == 4) e = expf(u.nhd2 * xCx);
-        else e = exp(u.nhd2 * xCx);
-        const T e2 = u.d2 * e;
-        /* invalid-value guard, kernel.cpp:456-457 */
-        if (e2 > (T)1 || e2 < (T)0 || e2 != e2) continue;
-        pairs++;
-        const T f = e2 * u.d1;
-        const T m = -u.d2;
-        /* x'^T C J_k */
-        const T u3 = q1 * J3y + q2 * J3z;
-        const T u4 = q0 * J4x + q1 * J4y + q2 * J4z;
-        const T u5 = q0 * J5x + q1 * J5y + q2 * J5z;
-        /* C J_k */
-        const T k30 = c01 * J3y + c02 * J3z, k31 = c11 * J3y + c12 * J3z, k32 = c12 * J3y + c22 * J3z;
-        const T k40 = c00 * J4x + c01 * J4y + c02 * J4z;
-        const T k41 = c01 * J4x + c11 * J4y + c12 * J4z;
-        const T k42 = c02 * J4x + c12 * J4y + c22 * J4z;
-        const T k50 = c00 * J5x + c01 * J5y + c02 * J5z;
-        const T k51 = c01 * J5x + c11 * J5y + c12 * J5z;
-        const T k52 = c02 * J5x + c12 * J5y + c22 * J5z;
-        /* J_b^T C J_a */
-        const T j33 = J3y * k31 + J3z * k32;
-        const T j34 = J4x * k30 + J4y * k31 + J4z * k32;
-        const T j35 = J5x * k30 + J5y * k31 + J5z * k32;
-        const T j44 = J4x * k40 + J4y * k41 + J4z * k42;
-        const T j45 = J5x * k40 + J5y * k41 + J5z * k42;
-        const T j55 = J5x * k50 + J5y * k51 + J5z * k52;
-        /* x'^T C H_ab = J_b . (q x a_a) */
-        const T h33 = q2 * J3y - q1 * J3z;
-        const T h34 = q2 * J4y - q1 * J4z;
-        const T h35 = q2 * J5y - q1 * J5z;
-        const T qa40 = q1 * u.a4[2] - q2 * u.a4[1], qa41 = q2 * u.a4[0] - q0 * u.a4[2], qa42 = q0 * u.a4[1] - q1 * u.a4[0];
-        const T qa50 = q1 * u.a5[2] - q2 * u.a5[1], qa51 = q2 * u.a5[0] - q0 * u.a5[2], qa52 = q0 * u.a5[1] - q1 * u.a5[0];
-        const T h44 = qa40 * J4x + qa41 * J4y + qa42 * J4z;
-        const T h45 = qa40 * J5x + qa41 * J5y + qa42 * J5z;
-        const T h55 = qa50 * J5x + qa51 * J5y + qa52 * J5z;
-        acc.add(0, -u.d1 * e);
-        acc.add(1, f * q0);
-        acc.add(2, f * q1);
-        acc.add(3, f * q2);
-        acc.add(4, f * u3);
-        acc.add(5, f * u4);
-        acc.add(6, f * u5);
-        const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
-        acc.add(7, f * (mu0 * q0 + c00));
-        acc.add(8, f * (mu0 * q1 + c01));
-        acc.add(9, f * (mu0 * q2 + c02));
-        acc.add(10, f * (mu0 * u3 + k30));
-        acc.add(11, f * (mu0 * u4 + k40));
-        acc.add(12, f * (mu0 * u5 + k50));
-        acc.add(13, f * (mu1 * q1 + c11));
-        acc.add(14, f * (mu1 * q2 + c12));
-        acc.add(15, f * (mu1 * u3 + k31));
-        acc.add(16, f * (mu1 * u4 + k41));
-        acc.add(17, f * (mu1 * u5 + k51));
-        acc.add(18, f * (mu2 * q2 + c22));
-        acc.add(19, f * (mu2 * u3 + k32));
-        acc.add(20, f * (mu2 * u4 + k42));
-        acc.add(21, f * (mu2 * u5 + k52));
-        acc.add(22, f * (mu3 * u3 + h33 + j33));
-        acc.add(23, f * (mu3 * u4 + h34 + j34));
-        acc.add(24, f * (mu3 * u5 + h35 + j35));
-        acc.add(25, f * (mu4 * u4 + h44 + j44));
-        acc.add(26, f * (mu4 * u5 + h45 + j45));
-        acc.add(27, f * (m * u5 * u5 + h55 + j55));
+        pairs += pair_add<T, SUM>(g, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, L.begin + k, u, acc);
     }
+    return pairs;
+}
+
+/* (lane, candidate) entries of one wave's pair table: a query cell has at most 27
+ * candidates, so a wave has at most 64 * 27 pairs per point round */
+constexpr int WAVE_PAIR_SLOTS = 2048;
+
+__device__ inline uint32_t bperm_u32(int src_byte, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src_byte, (int)v);
+}
+__device__ inline float bperm_f32(int src_byte, float v) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_byte, __float_as_int(v)));
+}
+__device__ inline double bperm_f64(int src_byte, double v) {
+    const uint64_t b = (uint64_t)__double_as_longlong(v);
+    const uint32_t lo = bperm_u32(src_byte, (uint32_t)b), hi = bperm_u32(src_byte, (uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+/* The pairs of one point per lane, spread evenly over the wave.  Lane l holds its
+ * point's lookup L and prefilter mask (zero without a point).  The wave's pairs are
+ * numbered lane by lane, candidates in index order within a lane, and pair n is
+ * evaluated by lane n mod 64 in round n / 64.  A wave thus spends ceil(pairs / 64)
+ * rounds on the pair terms, where a per-point loop spends as many as the busiest
+ * lane has pairs.  Each lane adds the terms of the pairs it evaluates to its own
+ * accumulators; the assignment depends only on the wave's points, so results are
+ * reproducible.  slots: this wave's LDS table of WAVE_PAIR_SLOTS entries.  Every
+ * lane of the wave must call this. */
+template <typename T, int SUM>
+__device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, const uint32_t mask,
+                                 const PassParams<T>& u, Acc<T, SUM>& acc, uint16_t* slots) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t cnt = (uint32_t)__builtin_popcount(mask);
+    /* exclusive prefix of the per-lane counts (< 32) from five bit-plane ballots */
+    uint32_t excl = 0, total = 0;
+#pragma unroll
+    for (int b = 0; b < 5; b++) {
+        const uint64_t bal = __ballot((cnt >> b) & 1u);
+        excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << b;
+        total += (uint32_t)__popcll(bal) << b;
+    }
+    uint32_t m = mask, j = excl;
+    while (m) {
+        const uint32_t k = (uint32_t)__builtin_ctz(m);
+        m &= m - 1;
+        slots[j++] = (uint16_t)((lane << 5) | k);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int pairs = 0;
+    for (uint32_t base = 0; base < total; base += WAVE) {
+        const uint32_t n = base + lane;
+        const uint32_t e = (n < total) ? (uint32_t)slots[n] : (lane << 5);
+        const int src = (int)(e >> 5) << 2;
+        const uint32_t cand = bperm_u32(src, L.begin) + (e & 31u);
+        const float4 ps = make_float4(bperm_f32(src, p.x), bperm_f32(src, p.y), bperm_f32(src, p.z), 0.f);
+        double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+        if constexpr (sizeof(T) == 4) {
+            r0 = bperm_f32(src, L.r0);
+            r1 = bperm_f32(src, L.r1);
+            r2 = bperm_f32(src, L.r2);
+        } else {
+            x0 = bperm_f64(src, L.x0);
+            x1 = bperm_f64(src, L.x1);
+            x2 = bperm_f64(src, L.x2);
+        }
+        if (n < total) pairs += pair_add<T, SUM>(g, x0, x1, x2, r0, r1, r2, ps, cand, u, acc);
+    }
+    /* the next call rewrites the table: its reads stay before those writes */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     return pairs;
 }
 
@@ -1140,6 +1221,50 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
 #pragma unroll 4
     for (uint32_t k = 0; k < L.count; k++) mask |= prefilter_bit(g, L, k, lim);
     return point_pairs<T, SUM>(g, p, L, mask, u, acc);
+}
+
+/* Points [pb, pe) of one chunk at one trial pose, added to this thread's
+ * accumulators; returns this thread's pair count.  With a dense index each wave
+ * takes 64 consecutive points per round (wave-uniform trip count), the next
+ * round's point load and index lookup are in flight during this round's pairs, and
+ * the pairs are spread over the wave (wave_pairs).  With the hashed index each
+ * thread handles its own points.  Both schedulers run their chunks through this,
+ * so their partial sums are bit-identical. */
+template <typename T, int SUM, int BLOCK>
+__device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __restrict__ pts, const int64_t pb,
+                                       const int64_t pe, const PassParams<T>& u, Acc<T, SUM>& acc,
+                                       uint16_t* wave_slots) {
+    int pairs = 0;
+    if (g.dense) {
+        const int lane = threadIdx.x % WAVE;
+        const int64_t wb = pb + (threadIdx.x - lane);
+        const float lim = prefilter_limit(u);
+        bool vc = wb + lane < pe;
+        float4 pc = vc ? pts[wb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        PointLook lc;
+        uint2 ec;
+        lookup_issue(g, pc, vc, u.R, u.t, lc, ec);
+        for (int64_t ib = wb; ib < pe; ib += BLOCK) {
+            const bool vn = ib + BLOCK + lane < pe;
+            float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (vn) pn = pts[ib + BLOCK + lane];
+            lc.begin = ec.x;
+            lc.count = ec.y;
+            uint32_t mask = 0;
+#pragma unroll 4
+            for (uint32_t k = 0; k < lc.count; k++) mask |= prefilter_bit(g, lc, k, lim);
+            PointLook ln;
+            uint2 en;
+            lookup_issue(g, pn, vn, u.R, u.t, ln, en);
+            pairs += wave_pairs<T, SUM>(g, pc, lc, mask, u, acc, wave_slots);
+            pc = pn;
+            lc = ln;
+            ec = en;
+        }
+    } else {
+        for (int64_t i = pb + threadIdx.x; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+    }
+    return pairs;
 }
 
 
@@ -1520,6 +1645,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double red[NRED];
     __shared__ uint64_t sstate[STATE_WORDS];
     __shared__ double gred[SLOT_GROUPS][32];
+    __shared__ uint16_t wslots[BLOCK / WAVE][WAVE_PAIR_SLOTS];
     __shared__ uint32_t s_active[MAX_BATCH];
     __shared__ uint32_t s_prefix[MAX_BATCH + 1];
     __shared__ uint32_t lw[BLOCK / WAVE];
@@ -1579,7 +1705,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
                 const PassParams<T> u = load_uniform(pp);
                 const int64_t pb = sd.offset + k * sd.chunk;
                 const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * (int64_t)sd.chunk);
-                for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+                pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
             }
             double v[NRED];
 #pragma unroll
@@ -1727,6 +1853,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double red[NRED];
     __shared__ double gred[SLOT_GROUPS][32];
     __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ uint16_t wslots[BLOCK / WAVE][WAVE_PAIR_SLOTS];
     __shared__ uint64_t s_item, s_ppkey;
     __shared__ uint32_t s_base;
     __shared__ int s_last, s_done;
@@ -1778,35 +1905,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const PassParams<T> u = load_uniform(pp);
             const int64_t pb = sd.offset + (int64_t)k * sd.chunk;
             const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * sd.chunk);
-            if (g.dense) {
-                /* software-pipelined: point i + BLOCK's load and index lookup are in
-                 * flight while point i's candidates and pairs are processed */
-                const float lim = prefilter_limit(u);
-                int64_t i = pb + tid;
-                float4 pc = (i < pe) ? pts[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-                PointLook lc;
-                uint2 ec;
-                lookup_issue(g, pc, u.R, u.t, lc, ec);
-                for (; i < pe; i += BLOCK) {
-                    const bool more = i + BLOCK < pe;
-                    float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (more) pn = pts[i + BLOCK];
-                    lc.begin = ec.x;
-                    lc.count = ec.y;
-                    uint32_t mask = 0;
-#pragma unroll 4
-                    for (uint32_t k = 0; k < lc.count; k++) mask |= prefilter_bit(g, lc, k, lim);
-                    PointLook ln;
-                    uint2 en = make_uint2(0u, 0u);
-                    if (more) lookup_issue(g, pn, u.R, u.t, ln, en);
-                    if (mask) pairs += point_pairs<T, SUM>(g, pc, lc, mask, u, acc);
-                    pc = pn;
-                    lc = ln;
-                    ec = en;
-                }
-            } else {
-                for (int64_t i = pb + tid; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
-            }
+            pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
         }
         double v[NRED];
 #pragma unroll
