@@ -335,7 +335,8 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
     BUF_TRY(g->cand_head, sizeof(float4) * (ne + 1));
     BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
-    BUF_TRY(g->cand_s64, 5 * sizeof(double2) * (ne + 1));
+    BUF_TRY(g->vs64, 5 * sizeof(double2) * (nv + 1));
+    LAUNCH_TRY(launch_voxel_s64(g->mean_c.as<double>(), g->icov_c.as<double>(), nv, g->vs64.as<double2>(), c->stream));
     /* dense query-cell index when the map's cell box is at most 2^28 cells (2 GiB) */
     const double box = (double)g->dims[0] * (double)g->dims[1] * (double)g->dims[2];
     const bool dense = box <= (double)(1 << 28);
@@ -352,8 +353,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
                                       log2cap, c->stream));
         LAUNCH_TRY(launch_cand_fill(keys.as<uint64_t>(), vals.as<uint32_t>(), ne, g->mean_c.as<double>(),
                                     g->icov_c.as<double>(), g->bits, g->origin_cell, (double)g->params.resolution,
-                                    g->cand_head.as<float4>(), g->cand_s32.as<float4>(), g->cand_s64.as<double2>(),
-                                    c->stream));
+                                    g->cand_head.as<float4>(), g->cand_s32.as<float4>(), c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     g->info.query_cells = nq;
@@ -365,7 +365,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     d.slots = g->slots.as<HashSlot>();
     d.head = g->cand_head.as<float4>();
     d.s32 = g->cand_s32.as<float4>();
-    d.s64 = g->cand_s64.as<double2>();
+    d.vs64 = g->vs64.as<double2>();
     d.log2cap = log2cap;
     for (int k = 0; k < 3; k++) {
         d.bits[k] = g->bits[k];

@@ -153,7 +153,7 @@ struct ndt_grid {
     ndt_grid_info info;
     /* ndt */
     ndt::NdtGridDev dev;
-    DevBuf slots, dense, cand_head, cand_s32, cand_s64, icov_f, mean_c, icov_c;
+    DevBuf slots, dense, cand_head, cand_s32, vs64, icov_f, mean_c, icov_c;
     int32_t dims[3] = {0, 0, 0};
     /* map points sorted by voxel + occupied-voxel hash (fitness score) */
     ndt::FitGridDev fdev;

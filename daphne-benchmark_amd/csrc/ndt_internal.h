@@ -223,20 +223,21 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
 
 /* Device view of an ndt-mode grid. */
 /* Candidates: one entry per (query cell, valid voxel within reach), the entries
- * of a query cell contiguous in probe order, in three parallel arrays:
+ * of a query cell contiguous in probe order, in two parallel arrays:
  *   head  float4 (mean - query cell corner as float32, voxel id bits): the range
  *         prefilter reads only these, 16 bytes per candidate;
- *   s32   2 x float4 (c00 c01 c02 c11)(c12 c22 0 0): float32 inverse covariance;
- *   s64   5 x double2 (mx my)(mz c00)(c01 c02)(c11 c12)(c22 0): float64 mean and
- *         inverse covariance.
- * A scan point reaches everything it needs from its hash slot alone. */
+ *   s32   2 x float4 (c00 c01 c02 c11)(c12 c22 0 0): float32 inverse covariance.
+ * The float64 statistics are stored once per valid voxel, indexed by voxel id:
+ *   vs64  5 x double2 (mx my)(mz c00)(c01 c02)(c11 c12)(c22 0): float64 mean and
+ *         inverse covariance, 80 bytes per voxel (27 times less than per
+ *         candidate, so that they stay in the caches). */
 struct NdtGridDev {
     const uint2* dense;      /* query cell -> (begin, count), dense over the map's cell box, or null */
     int32_t dims[3];         /* cells per axis of the dense index */
     const HashSlot* slots;   /* query cell -> candidate range (when dense is null) */
     const float4* head;      /* candidates: relative mean, voxel id */
     const float4* s32;       /* candidates: float32 statistics (2 per entry) */
-    const double2* s64;      /* candidates: float64 statistics (5 per entry) */
+    const double2* vs64;     /* valid voxels: float64 statistics (5 per voxel) */
     int log2cap;
     int bits[3];             /* key bit widths x, y, z */
     int64_t origin_cell[3];  /* cell coordinates of key 0 */
@@ -393,7 +394,8 @@ int launch_hash_insert(const uint64_t* sorted_keys, const uint32_t* starts, int6
 int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, int log2cap, hipStream_t stream);
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
                      const double* icov, const int* bits, const int64_t* origin_cell, double res, float4* head,
-                     float4* s32, double2* s64, hipStream_t stream);
+                     float4* s32, hipStream_t stream);
+int launch_voxel_s64(const double* mean, const double* icov, int64_t nvalid, double2* vs64, hipStream_t stream);
 int launch_hash_clear(HashSlot* slots, int64_t cap, hipStream_t stream);
 int launch_voxel_stats_compat(const float4* pts, const uint32_t* sorted_idx, const uint64_t* sorted_keys,
                               const uint32_t* seg_start, int64_t nseg, int64_t npts, double ncells,

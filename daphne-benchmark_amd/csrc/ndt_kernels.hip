@@ -668,11 +668,10 @@ int launch_hash_insert_ids(const uint64_t* vkey, int64_t nvox, HashSlot* slots, 
 /* Candidate i of the sorted dilation keys (query cell << 5 | probe): the voxel's
  * mean relative to the lower corner of the query cell in float32 (values in
  * [-res, 2 res), kept to about 1e-7 m at any map extent) with the voxel id, and
- * the voxel's statistics in float32 and float64. */
+ * the voxel's statistics in float32. */
 __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* __restrict__ svals, int64_t n,
                             const double* __restrict__ mean, const double* __restrict__ icov, Bits3 B, int64_t o0,
-                            int64_t o1, int64_t o2, double res, float4* __restrict__ head, float4* __restrict__ s32,
-                            double2* __restrict__ s64) {
+                            int64_t o1, int64_t o2, double res, float4* __restrict__ head, float4* __restrict__ s32) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t v = svals[i];
@@ -686,20 +685,36 @@ __global__ void k_cand_fill(const uint64_t* __restrict__ skeys, const uint32_t* 
                           (float)(m[2] - (double)c2 * res), __uint_as_float(v));
     s32[2 * i] = make_float4((float)C[0], (float)C[1], (float)C[2], (float)C[3]);
     s32[2 * i + 1] = make_float4((float)C[4], (float)C[5], 0.0f, 0.0f);
-    s64[5 * i] = make_double2(m[0], m[1]);
-    s64[5 * i + 1] = make_double2(m[2], C[0]);
-    s64[5 * i + 2] = make_double2(C[1], C[2]);
-    s64[5 * i + 3] = make_double2(C[3], C[4]);
-    s64[5 * i + 4] = make_double2(C[5], 0.0);
 }
 
 int launch_cand_fill(const uint64_t* sorted_keys, const uint32_t* sorted_vals, int64_t nentries, const double* mean,
                      const double* icov, const int* bits, const int64_t* origin_cell, double res, float4* head,
-                     float4* s32, double2* s64, hipStream_t stream) {
+                     float4* s32, hipStream_t stream) {
     Bits3 B;
     for (int k = 0; k < 3; k++) B.b[k] = bits[k];
     hipLaunchKernelGGL(k_cand_fill, dim3(grid_for(nentries, 256)), dim3(256), 0, stream, sorted_keys, sorted_vals,
-                       nentries, mean, icov, B, origin_cell[0], origin_cell[1], origin_cell[2], res, head, s32, s64);
+                       nentries, mean, icov, B, origin_cell[0], origin_cell[1], origin_cell[2], res, head, s32);
+    return (int)hipGetLastError();
+}
+
+/* Float64 statistics of valid voxel v: mean and inverse covariance (upper
+ * triangle), 80 bytes. */
+__global__ void k_voxel_s64(const double* __restrict__ mean, const double* __restrict__ icov, int64_t n,
+                            double2* __restrict__ vs64) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const double* m = mean + 3 * v;
+    const double* C = icov + 6 * v;
+    vs64[5 * v] = make_double2(m[0], m[1]);
+    vs64[5 * v + 1] = make_double2(m[2], C[0]);
+    vs64[5 * v + 2] = make_double2(C[1], C[2]);
+    vs64[5 * v + 3] = make_double2(C[3], C[4]);
+    vs64[5 * v + 4] = make_double2(C[5], 0.0);
+}
+
+int launch_voxel_s64(const double* mean, const double* icov, int64_t nvalid, double2* vs64, hipStream_t stream) {
+    if (nvalid <= 0) return 0;
+    hipLaunchKernelGGL(k_voxel_s64, dim3(grid_for(nvalid, 256)), dim3(256), 0, stream, mean, icov, nvalid, vs64);
     return (int)hipGetLastError();
 }
 
@@ -981,11 +996,11 @@ __device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const b
     const double xd0 = R[0] * X0 + R[1] * X1 + R[2] * X2 + t[0];
     const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
     const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
-    e = make_uint2(0u, 0u);
-    if (!valid || !(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return;
-    const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
-    const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
-    const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
+    const bool finite = (fabs(xd0) < 1e15) && (fabs(xd1) < 1e15) && (fabs(xd2) < 1e15);
+    /* cells of non-finite points are never used (ok is false for them) */
+    const int64_t a0 = finite ? (int64_t)floor(xd0 / g.resolution_d) : 0;
+    const int64_t a1 = finite ? (int64_t)floor(xd1 / g.resolution_d) : 0;
+    const int64_t a2 = finite ? (int64_t)floor(xd2 / g.resolution_d) : 0;
     const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
     L.x0 = xd0;
     L.x1 = xd1;
@@ -993,8 +1008,12 @@ __device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const b
     L.r0 = (float)(xd0 - (double)a0 * g.resolution_d);
     L.r1 = (float)(xd1 - (double)a1 * g.resolution_d);
     L.r2 = (float)(xd2 - (double)a2 * g.resolution_d);
-    if (c0 < 0 || c1 < 0 || c2 < 0 || c0 >= g.dims[0] || c1 >= g.dims[1] || c2 >= g.dims[2]) return;
-    e = g.dense[(c2 * g.dims[1] + c1) * g.dims[0] + c0];
+    const bool ok = valid && finite && c0 >= 0 && c1 >= 0 && c2 >= 0 && c0 < g.dims[0] && c1 < g.dims[1] &&
+                    c2 < g.dims[2];
+    /* the load is unconditional (entry 0 stands in for points without a cell), so
+     * that its wait is counted exactly where the entry is used */
+    const uint2 v = g.dense[ok ? (c2 * g.dims[1] + c1) * g.dims[0] + c0 : 0];
+    e = ok ? v : make_uint2(0u, 0u);
 }
 
 /* range prefilter of candidate k of a point: bit k of the result */
@@ -1015,17 +1034,18 @@ __device__ inline float prefilter_limit(const PassParams<T>& u) {
  * (point, candidate) pair, added when the candidate's mean lies within one
  * resolution of the point: xd is the transformed point (float64), r its position
  * relative to its cell's lower corner (float32), p the point in the sensor frame.
+ * idx is the candidate entry (float32 points) or the voxel id (float64 points).
  * Returns 1 when the pair contributes. */
 template <typename T, int SUM>
 __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const double xd1, const double xd2,
-                               const float r0, const float r1, const float r2, const float4 p, const uint32_t cand,
+                               const float r0, const float r1, const float r2, const float4 p, const uint32_t idx,
                                const PassParams<T>& u, Acc<T, SUM>& acc) {
 #pragma clang fp contract(fast)
     T d0, d1, d2;
     T c00, c01, c02, c11, c12, c22;
     if constexpr (sizeof(T) == 4) {
-        const float4 h = g.head[cand];
-        const float4* r = g.s32 + 2 * (int64_t)cand;
+        const float4 h = g.head[idx];
+        const float4* r = g.s32 + 2 * (int64_t)idx;
         const float4 a = r[0], b = r[1];
         d0 = r0 - h.x;
         d1 = r1 - h.y;
@@ -1033,7 +1053,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
         if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) return 0;
         c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
     } else {
-        const double2* r = g.s64 + 5 * (int64_t)cand;
+        const double2* r = g.vs64 + 5 * (int64_t)idx;
         const double2 m01 = r[0], m2c0 = r[1], c12v = r[2], c34 = r[3], c5p = r[4];
         d0 = xd0 - m01.x;
         d1 = xd1 - m01.y;
@@ -1131,14 +1151,24 @@ __device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const Poi
     while (mask) {
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1;
-        pairs += pair_add<T, SUM>(g, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, L.begin + k, u, acc);
+        const uint32_t idx = (sizeof(T) == 4) ? L.begin + k : __float_as_uint(g.head[L.begin + k].w);
+        pairs += pair_add<T, SUM>(g, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, idx, u, acc);
     }
     return pairs;
 }
 
-/* (lane, candidate) entries of one wave's pair table: a query cell has at most 27
- * candidates, so a wave has at most 64 * 27 pairs per point round */
-constexpr int WAVE_PAIR_SLOTS = 2048;
+/* A query cell has at most 27 candidates (its 3 x 3 x 3 neighbourhood). */
+constexpr int MAX_CANDIDATES = 27;
+
+/* One wave's pair table in LDS: lane l's candidates that pass the prefilter at
+ * pick[32 l ...] (their pair_add index; a row has room for the 27 candidates plus
+ * the branch-free prefilter's writes past the last one), and the wave's pairs in
+ * evaluation order as positions in pick. */
+constexpr int PICK_ROW = 32;
+struct WavePairs {
+    uint32_t pick[WAVE * PICK_ROW];
+    uint16_t order[WAVE * MAX_CANDIDATES];
+};
 
 __device__ inline uint32_t bperm_u32(int src_byte, uint32_t v) {
     return (uint32_t)__builtin_amdgcn_ds_bpermute(src_byte, (int)v);
@@ -1153,19 +1183,18 @@ __device__ inline double bperm_f64(int src_byte, double v) {
 }
 
 /* The pairs of one point per lane, spread evenly over the wave.  Lane l holds its
- * point's lookup L and prefilter mask (zero without a point).  The wave's pairs are
- * numbered lane by lane, candidates in index order within a lane, and pair n is
- * evaluated by lane n mod 64 in round n / 64.  A wave thus spends ceil(pairs / 64)
- * rounds on the pair terms, where a per-point loop spends as many as the busiest
- * lane has pairs.  Each lane adds the terms of the pairs it evaluates to its own
- * accumulators; the assignment depends only on the wave's points, so results are
- * reproducible.  slots: this wave's LDS table of WAVE_PAIR_SLOTS entries.  Every
- * lane of the wave must call this. */
+ * point's lookup L and has written its cnt prefiltered candidates to its row of
+ * wp.pick (cnt <= 27; 0 without a point).  The wave's pairs are numbered lane by lane,
+ * in prefilter order within a lane, and pair n is evaluated by lane n mod 64 in
+ * round n / 64.  A wave thus spends ceil(pairs / 64) rounds on the pair terms,
+ * where a per-point loop spends as many as the busiest lane has pairs.  Each lane
+ * adds the terms of the pairs it evaluates to its own accumulators; the assignment
+ * depends only on the wave's points, so results are reproducible.  Every lane of
+ * the wave must call this. */
 template <typename T, int SUM>
-__device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, const uint32_t mask,
-                                 const PassParams<T>& u, Acc<T, SUM>& acc, uint16_t* slots) {
+__device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, const uint32_t cnt,
+                                 const PassParams<T>& u, Acc<T, SUM>& acc, WavePairs& wp) {
     const uint32_t lane = threadIdx.x % WAVE;
-    const uint32_t cnt = (uint32_t)__builtin_popcount(mask);
     /* exclusive prefix of the per-lane counts (< 32) from five bit-plane ballots */
     uint32_t excl = 0, total = 0;
 #pragma unroll
@@ -1174,21 +1203,16 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
         excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << b;
         total += (uint32_t)__popcll(bal) << b;
     }
-    uint32_t m = mask, j = excl;
-    while (m) {
-        const uint32_t k = (uint32_t)__builtin_ctz(m);
-        m &= m - 1;
-        slots[j++] = (uint16_t)((lane << 5) | k);
-    }
+    for (uint32_t j = 0; j < cnt; j++) wp.order[excl + j] = (uint16_t)(PICK_ROW * lane + j);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int pairs = 0;
     for (uint32_t base = 0; base < total; base += WAVE) {
         const uint32_t n = base + lane;
-        const uint32_t e = (n < total) ? (uint32_t)slots[n] : (lane << 5);
-        const int src = (int)(e >> 5) << 2;
-        const uint32_t cand = bperm_u32(src, L.begin) + (e & 31u);
+        const uint32_t d = (n < total) ? (uint32_t)wp.order[n] : PICK_ROW * lane;
+        const uint32_t idx = wp.pick[d];
+        const int src = (int)(d / PICK_ROW) << 2;
         const float4 ps = make_float4(bperm_f32(src, p.x), bperm_f32(src, p.y), bperm_f32(src, p.z), 0.f);
         double x0 = 0.0, x1 = 0.0, x2 = 0.0;
         float r0 = 0.f, r1 = 0.f, r2 = 0.f;
@@ -1201,9 +1225,9 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
             x1 = bperm_f64(src, L.x1);
             x2 = bperm_f64(src, L.x2);
         }
-        if (n < total) pairs += pair_add<T, SUM>(g, x0, x1, x2, r0, r1, r2, ps, cand, u, acc);
+        if (n < total) pairs += pair_add<T, SUM>(g, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
     }
-    /* the next call rewrites the table: its reads stay before those writes */
+    /* the next round rewrites the table: its reads stay before those writes */
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1233,30 +1257,50 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
 template <typename T, int SUM, int BLOCK>
 __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __restrict__ pts, const int64_t pb,
                                        const int64_t pe, const PassParams<T>& u, Acc<T, SUM>& acc,
-                                       uint16_t* wave_slots) {
+                                       WavePairs& wp) {
     int pairs = 0;
     if (g.dense) {
         const int lane = threadIdx.x % WAVE;
         const int64_t wb = pb + (threadIdx.x - lane);
         const float lim = prefilter_limit(u);
-        bool vc = wb + lane < pe;
-        float4 pc = vc ? pts[wb + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        /* point loads are unconditional, clamped to the chunk's last point, so
+         * that their waits are counted exactly */
+        const bool vc = wb + lane < pe;
+        float4 pc = pts[min(wb + lane, pe - 1)];
         PointLook lc;
         uint2 ec;
         lookup_issue(g, pc, vc, u.R, u.t, lc, ec);
         for (int64_t ib = wb; ib < pe; ib += BLOCK) {
             const bool vn = ib + BLOCK + lane < pe;
-            float4 pn = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (vn) pn = pts[ib + BLOCK + lane];
+            const float4 pn = pts[min(ib + BLOCK + lane, pe - 1)];
             lc.begin = ec.x;
             lc.count = ec.y;
-            uint32_t mask = 0;
-#pragma unroll 4
-            for (uint32_t k = 0; k < lc.count; k++) mask |= prefilter_bit(g, lc, k, lim);
+            /* prefilter: the passing candidates go to this lane's row of the table.
+             * Heads are loaded four at a time, all four loads issued before the
+             * first is used (indices past the run repeat its last entry and are
+             * not counted), so a point waits for one round trip per four
+             * candidates.  Branch-free: every candidate is written at the next
+             * free position, which a failing one leaves free. */
+            uint32_t* mine = wp.pick + PICK_ROW * lane;
+            uint32_t cnt = 0;
+            for (uint32_t k0 = 0; k0 < lc.count; k0 += 4) {
+                const uint32_t last = lc.begin + lc.count - 1;
+                const uint32_t i0 = lc.begin + k0;
+                float4 h[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) h[q] = g.head[min(i0 + q, last)];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const float e0 = lc.r0 - h[q].x, e1 = lc.r1 - h[q].y, e2 = lc.r2 - h[q].z;
+                    const bool in = (k0 + q < lc.count) && (e0 * e0 + e1 * e1 + e2 * e2 < lim);
+                    mine[cnt] = (sizeof(T) == 4) ? i0 + q : __float_as_uint(h[q].w);
+                    cnt += in ? 1u : 0u;
+                }
+            }
             PointLook ln;
             uint2 en;
             lookup_issue(g, pn, vn, u.R, u.t, ln, en);
-            pairs += wave_pairs<T, SUM>(g, pc, lc, mask, u, acc, wave_slots);
+            pairs += wave_pairs<T, SUM>(g, pc, lc, cnt, u, acc, wp);
             pc = pn;
             lc = ln;
             ec = en;
@@ -1645,7 +1689,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double red[NRED];
     __shared__ uint64_t sstate[STATE_WORDS];
     __shared__ double gred[SLOT_GROUPS][32];
-    __shared__ uint16_t wslots[BLOCK / WAVE][WAVE_PAIR_SLOTS];
+    __shared__ WavePairs wslots[BLOCK / WAVE];
     __shared__ uint32_t s_active[MAX_BATCH];
     __shared__ uint32_t s_prefix[MAX_BATCH + 1];
     __shared__ uint32_t lw[BLOCK / WAVE];
@@ -1853,7 +1897,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double red[NRED];
     __shared__ double gred[SLOT_GROUPS][32];
     __shared__ uint64_t sstate[STATE_WORDS];
-    __shared__ uint16_t wslots[BLOCK / WAVE][WAVE_PAIR_SLOTS];
+    __shared__ WavePairs wslots[BLOCK / WAVE];
     __shared__ uint64_t s_item, s_ppkey;
     __shared__ uint32_t s_base;
     __shared__ int s_last, s_done;
