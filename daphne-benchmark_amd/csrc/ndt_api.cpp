@@ -374,6 +374,9 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     d.resolution = g->params.resolution;
     d.inv_resolution = 1.0f / g->params.resolution;
     d.resolution_d = res;
+    int ex = 0;
+    d.res_pow2 = std::frexp(res, &ex) == 0.5 ? 1 : 0;
+    d.inv_resolution_d = 1.0 / res;
     return 0;
 }
 

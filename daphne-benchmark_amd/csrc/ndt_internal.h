@@ -244,6 +244,8 @@ struct NdtGridDev {
     float resolution;
     float inv_resolution;
     double resolution_d;
+    double inv_resolution_d; /* 1 / resolution, exact when res_pow2 */
+    int32_t res_pow2;        /* the resolution is a power of two */
 };
 
 /* Device view of the map points of an ndt grid, sorted by voxel, for nearest-point

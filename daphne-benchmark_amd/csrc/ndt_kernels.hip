@@ -946,6 +946,13 @@ struct PointLook {
     uint32_t begin, count;
 };
 
+/* x / resolution in float64, as the oracle computes a cell coordinate.  With a
+ * power-of-two resolution the reciprocal is exact and one multiplication gives the
+ * same value as the division. */
+__device__ inline double cell_quot(const NdtGridDev& g, const double x) {
+    return g.res_pow2 ? x * g.inv_resolution_d : x / g.resolution_d;
+}
+
 /* Transform, cell and index lookup; false when the point has no candidates. */
 __device__ inline bool point_lookup(const NdtGridDev& g, const float4 p, const double* R, const double* t,
                                     PointLook& L) {
@@ -955,9 +962,9 @@ __device__ inline bool point_lookup(const NdtGridDev& g, const float4 p, const d
     const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
     const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
     if (!(fabs(xd0) < 1e15) || !(fabs(xd1) < 1e15) || !(fabs(xd2) < 1e15)) return false;
-    const int64_t a0 = (int64_t)floor(xd0 / g.resolution_d);
-    const int64_t a1 = (int64_t)floor(xd1 / g.resolution_d);
-    const int64_t a2 = (int64_t)floor(xd2 / g.resolution_d);
+    const int64_t a0 = (int64_t)floor(cell_quot(g, xd0));
+    const int64_t a1 = (int64_t)floor(cell_quot(g, xd1));
+    const int64_t a2 = (int64_t)floor(cell_quot(g, xd2));
     const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
     uint32_t begin, count;
     if (g.dense) {
@@ -998,9 +1005,9 @@ __device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const b
     const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
     const bool finite = (fabs(xd0) < 1e15) && (fabs(xd1) < 1e15) && (fabs(xd2) < 1e15);
     /* cells of non-finite points are never used (ok is false for them) */
-    const int64_t a0 = finite ? (int64_t)floor(xd0 / g.resolution_d) : 0;
-    const int64_t a1 = finite ? (int64_t)floor(xd1 / g.resolution_d) : 0;
-    const int64_t a2 = finite ? (int64_t)floor(xd2 / g.resolution_d) : 0;
+    const int64_t a0 = finite ? (int64_t)floor(cell_quot(g, xd0)) : 0;
+    const int64_t a1 = finite ? (int64_t)floor(cell_quot(g, xd1)) : 0;
+    const int64_t a2 = finite ? (int64_t)floor(cell_quot(g, xd2)) : 0;
     const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
     L.x0 = xd0;
     L.x1 = xd1;
@@ -1035,14 +1042,19 @@ __device__ inline float prefilter_limit(const PassParams<T>& u) {
  * resolution of the point: xd is the transformed point (float64), r its position
  * relative to its cell's lower corner (float32), p the point in the sensor frame.
  * idx is the candidate entry (float32 points) or the voxel id (float64 points).
- * Returns 1 when the pair contributes. */
+ * Branch-free: a pair that is not valid, not in range or fails the value guard
+ * still runs every instruction, on zeroed inputs and with zero weights, and adds
+ * +-0 to every sum (the sums are unchanged), so the accumulators need no copies at
+ * control-flow joins.  Returns 1 when the pair contributes. */
 template <typename T, int SUM>
-__device__ inline int pair_add(const NdtGridDev& g, const double xd0, const double xd1, const double xd2,
-                               const float r0, const float r1, const float r2, const float4 p, const uint32_t idx,
-                               const PassParams<T>& u, Acc<T, SUM>& acc) {
+__device__ inline int pair_add(const NdtGridDev& g, const bool valid, const double xd0, const double xd1,
+                               const double xd2, const float r0, const float r1, const float r2, const float4 p,
+                               uint32_t idx, const PassParams<T>& u, Acc<T, SUM>& acc) {
 #pragma clang fp contract(fast)
+    idx = valid ? idx : 0u; /* entry / voxel 0 always exists */
     T d0, d1, d2;
     T c00, c01, c02, c11, c12, c22;
+    bool in;
     if constexpr (sizeof(T) == 4) {
         const float4 h = g.head[idx];
         const float4* r = g.s32 + 2 * (int64_t)idx;
@@ -1050,7 +1062,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
         d0 = r0 - h.x;
         d1 = r1 - h.y;
         d2 = r2 - h.z;
-        if (!(d0 * d0 + d1 * d1 + d2 * d2 < u.res2)) return 0;
+        in = valid && (d0 * d0 + d1 * d1 + d2 * d2 < u.res2);
         c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
     } else {
         const double2* r = g.vs64 + 5 * (int64_t)idx;
@@ -1058,11 +1070,14 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
         d0 = xd0 - m01.x;
         d1 = xd1 - m01.y;
         d2 = xd2 - m2c0.x;
-        if (!(sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res)) return 0;
+        in = valid && (sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res);
         c00 = m2c0.y; c01 = c12v.x; c02 = c12v.y; c11 = c34.x; c12 = c34.y; c22 = c5p.x;
     }
+    d0 = in ? d0 : (T)0;
+    d1 = in ? d1 : (T)0;
+    d2 = in ? d2 : (T)0;
     /* point Jacobian columns J3 = e_x x ya, J4 = a4 x ya, J5 = a5 x ya */
-    const T x0 = (T)p.x, x1 = (T)p.y, x2 = (T)p.z;
+    const T x0 = in ? (T)p.x : (T)0, x1 = in ? (T)p.y : (T)0, x2 = in ? (T)p.z : (T)0;
     const T y0 = u.Ra[0] * x0 + u.Ra[1] * x1 + u.Ra[2] * x2;
     const T y1 = u.Ra[3] * x0 + u.Ra[4] * x1 + u.Ra[5] * x2;
     const T y2 = u.Ra[6] * x0 + u.Ra[7] * x1 + u.Ra[8] * x2;
@@ -1079,8 +1094,8 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
     else e = exp(u.nhd2 * xCx);
     const T e2 = u.d2 * e;
     /* invalid-value guard, kernel.cpp:456-457 */
-    if (e2 > (T)1 || e2 < (T)0 || e2 != e2) return 0;
-    const T f = e2 * u.d1;
+    const bool ok = in && !(e2 > (T)1 || e2 < (T)0 || e2 != e2);
+    const T f = ok ? e2 * u.d1 : (T)0;
     const T m = -u.d2;
     /* x'^T C J_k */
     const T u3 = q1 * J3y + q2 * J3z;
@@ -1110,7 +1125,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
     const T h44 = qa40 * J4x + qa41 * J4y + qa42 * J4z;
     const T h45 = qa40 * J5x + qa41 * J5y + qa42 * J5z;
     const T h55 = qa50 * J5x + qa51 * J5y + qa52 * J5z;
-    acc.add(0, -u.d1 * e);
+    acc.add(0, ok ? -u.d1 * e : (T)0);
     acc.add(1, f * q0);
     acc.add(2, f * q1);
     acc.add(3, f * q2);
@@ -1139,7 +1154,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const double xd0, const doub
     acc.add(25, f * (mu4 * u4 + h44 + j44));
     acc.add(26, f * (mu4 * u5 + h45 + j45));
     acc.add(27, f * (m * u5 * u5 + h55 + j55));
-    return 1;
+    return ok ? 1 : 0;
 }
 
 /* The candidates of one looked-up point selected by mask.  Returns the number of
@@ -1152,7 +1167,7 @@ __device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const Poi
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1;
         const uint32_t idx = (sizeof(T) == 4) ? L.begin + k : __float_as_uint(g.head[L.begin + k].w);
-        pairs += pair_add<T, SUM>(g, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, idx, u, acc);
+        pairs += pair_add<T, SUM>(g, true, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, idx, u, acc);
     }
     return pairs;
 }
@@ -1225,7 +1240,7 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
             x1 = bperm_f64(src, L.x1);
             x2 = bperm_f64(src, L.x2);
         }
-        if (n < total) pairs += pair_add<T, SUM>(g, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
+        pairs += pair_add<T, SUM>(g, n < total, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
     }
     /* the next round rewrites the table: its reads stay before those writes */
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
