@@ -795,6 +795,9 @@ struct PassParams {
     double R[9], t[3];         /* the transform, float64 in every precision */
     T Ra[9], a4[3], a5[3];
     T res, res2, d1, d2, nhd2; /* nhd2 = -d2 / 2 */
+    /* res2 (1 -+ 2^-48): outside [r2lo, r2hi) the float64 range test
+     * sqrt(|d|^2) < res is decided by |d|^2 alone (set by load_uniform) */
+    T r2lo, r2hi;
     /* parameter k of the 32-value layout written by pass_params_f64 */
     __device__ void set(int k, double v) {
         if (k < 9) R[k] = v;
@@ -876,6 +879,8 @@ __device__ inline PassParams<T> load_uniform(const PassParams<T>& lds) {
     u.d1 = uniform_value(lds.d1);
     u.d2 = uniform_value(lds.d2);
     u.nhd2 = uniform_value(lds.nhd2);
+    u.r2lo = u.res2 * (T)(1.0 - 0x1p-48);
+    u.r2hi = u.res2 * (T)(1.0 + 0x1p-48);
     return u;
 }
 
@@ -890,6 +895,8 @@ struct Acc<T, NDT_SUM_FP64> {
         for (int k = 0; k < NACC; k++) s[k] = 0.0;
     }
     __device__ void add(int k, T v) { s[k] += (double)v; }
+    /* s[k] += a * b with one rounding */
+    __device__ void fma(int k, double a, double b) { s[k] = ::fma(a, b, s[k]); }
     __device__ double get(int k) const { return s[k]; }
 };
 
@@ -1070,7 +1077,13 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
         d0 = xd0 - m01.x;
         d1 = xd1 - m01.y;
         d2 = xd2 - m2c0.x;
-        in = valid && (sqrt(d0 * d0 + d1 * d1 + d2 * d2) < u.res);
+        /* sqrt(dd) < res as the oracle tests it: a correctly rounded square root
+         * is below res for every dd < r2lo and not below it for dd >= r2hi, so it
+         * is computed only for the rare dd in between */
+        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+        in = dd < u.r2lo;
+        if (!(dd < u.r2lo) && dd < u.r2hi) in = sqrt(dd) < u.res;
+        in = valid && in;
         c00 = m2c0.y; c01 = c12v.x; c02 = c12v.y; c11 = c34.x; c12 = c34.y; c22 = c5p.x;
     }
     d0 = in ? d0 : (T)0;
@@ -1126,34 +1139,68 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
     const T h45 = qa40 * J5x + qa41 * J5y + qa42 * J5z;
     const T h55 = qa50 * J5x + qa51 * J5y + qa52 * J5z;
     acc.add(0, ok ? -u.d1 * e : (T)0);
-    acc.add(1, f * q0);
-    acc.add(2, f * q1);
-    acc.add(3, f * q2);
-    acc.add(4, f * u3);
-    acc.add(5, f * u4);
-    acc.add(6, f * u5);
-    const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
-    acc.add(7, f * (mu0 * q0 + c00));
-    acc.add(8, f * (mu0 * q1 + c01));
-    acc.add(9, f * (mu0 * q2 + c02));
-    acc.add(10, f * (mu0 * u3 + k30));
-    acc.add(11, f * (mu0 * u4 + k40));
-    acc.add(12, f * (mu0 * u5 + k50));
-    acc.add(13, f * (mu1 * q1 + c11));
-    acc.add(14, f * (mu1 * q2 + c12));
-    acc.add(15, f * (mu1 * u3 + k31));
-    acc.add(16, f * (mu1 * u4 + k41));
-    acc.add(17, f * (mu1 * u5 + k51));
-    acc.add(18, f * (mu2 * q2 + c22));
-    acc.add(19, f * (mu2 * u3 + k32));
-    acc.add(20, f * (mu2 * u4 + k42));
-    acc.add(21, f * (mu2 * u5 + k52));
-    acc.add(22, f * (mu3 * u3 + h33 + j33));
-    acc.add(23, f * (mu3 * u4 + h34 + j34));
-    acc.add(24, f * (mu3 * u5 + h35 + j35));
-    acc.add(25, f * (mu4 * u4 + h44 + j44));
-    acc.add(26, f * (mu4 * u5 + h45 + j45));
-    acc.add(27, f * (m * u5 * u5 + h55 + j55));
+    if constexpr (sizeof(T) == 8 && SUM == NDT_SUM_FP64) {
+        /* H_ij += f m u_i u_j + f K_ij and g_i += f u_i as fused multiply-adds
+         * into the sums: (f m u_i) u_j and f K_ij, one rounding each */
+        acc.fma(1, f, q0);
+        acc.fma(2, f, q1);
+        acc.fma(3, f, q2);
+        acc.fma(4, f, u3);
+        acc.fma(5, f, u4);
+        acc.fma(6, f, u5);
+        const T fm = f * m;
+        const T w0 = fm * q0, w1 = fm * q1, w2 = fm * q2, w3 = fm * u3, w4 = fm * u4, w5 = fm * u5;
+        acc.fma(7, w0, q0);   acc.fma(7, f, c00);
+        acc.fma(8, w0, q1);   acc.fma(8, f, c01);
+        acc.fma(9, w0, q2);   acc.fma(9, f, c02);
+        acc.fma(10, w0, u3);  acc.fma(10, f, k30);
+        acc.fma(11, w0, u4);  acc.fma(11, f, k40);
+        acc.fma(12, w0, u5);  acc.fma(12, f, k50);
+        acc.fma(13, w1, q1);  acc.fma(13, f, c11);
+        acc.fma(14, w1, q2);  acc.fma(14, f, c12);
+        acc.fma(15, w1, u3);  acc.fma(15, f, k31);
+        acc.fma(16, w1, u4);  acc.fma(16, f, k41);
+        acc.fma(17, w1, u5);  acc.fma(17, f, k51);
+        acc.fma(18, w2, q2);  acc.fma(18, f, c22);
+        acc.fma(19, w2, u3);  acc.fma(19, f, k32);
+        acc.fma(20, w2, u4);  acc.fma(20, f, k42);
+        acc.fma(21, w2, u5);  acc.fma(21, f, k52);
+        acc.fma(22, w3, u3);  acc.fma(22, f, h33 + j33);
+        acc.fma(23, w3, u4);  acc.fma(23, f, h34 + j34);
+        acc.fma(24, w3, u5);  acc.fma(24, f, h35 + j35);
+        acc.fma(25, w4, u4);  acc.fma(25, f, h44 + j44);
+        acc.fma(26, w4, u5);  acc.fma(26, f, h45 + j45);
+        acc.fma(27, w5, u5);  acc.fma(27, f, h55 + j55);
+    } else {
+        acc.add(1, f * q0);
+        acc.add(2, f * q1);
+        acc.add(3, f * q2);
+        acc.add(4, f * u3);
+        acc.add(5, f * u4);
+        acc.add(6, f * u5);
+        const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
+        acc.add(7, f * (mu0 * q0 + c00));
+        acc.add(8, f * (mu0 * q1 + c01));
+        acc.add(9, f * (mu0 * q2 + c02));
+        acc.add(10, f * (mu0 * u3 + k30));
+        acc.add(11, f * (mu0 * u4 + k40));
+        acc.add(12, f * (mu0 * u5 + k50));
+        acc.add(13, f * (mu1 * q1 + c11));
+        acc.add(14, f * (mu1 * q2 + c12));
+        acc.add(15, f * (mu1 * u3 + k31));
+        acc.add(16, f * (mu1 * u4 + k41));
+        acc.add(17, f * (mu1 * u5 + k51));
+        acc.add(18, f * (mu2 * q2 + c22));
+        acc.add(19, f * (mu2 * u3 + k32));
+        acc.add(20, f * (mu2 * u4 + k42));
+        acc.add(21, f * (mu2 * u5 + k52));
+        acc.add(22, f * (mu3 * u3 + h33 + j33));
+        acc.add(23, f * (mu3 * u4 + h34 + j34));
+        acc.add(24, f * (mu3 * u5 + h35 + j35));
+        acc.add(25, f * (mu4 * u4 + h44 + j44));
+        acc.add(26, f * (mu4 * u5 + h45 + j45));
+        acc.add(27, f * (m * u5 * u5 + h55 + j55));
+    }
     return ok ? 1 : 0;
 }
 
