@@ -370,6 +370,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     for (int k = 0; k < 3; k++) {
         d.bits[k] = g->bits[k];
         d.origin_cell[k] = g->origin_cell[k];
+        d.origin_cell_d[k] = (double)g->origin_cell[k];
     }
     d.resolution = g->params.resolution;
     d.inv_resolution = 1.0f / g->params.resolution;

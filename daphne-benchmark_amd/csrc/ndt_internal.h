@@ -241,6 +241,7 @@ struct NdtGridDev {
     int log2cap;
     int bits[3];             /* key bit widths x, y, z */
     int64_t origin_cell[3];  /* cell coordinates of key 0 */
+    double origin_cell_d[3]; /* the same as float64 (exact: |cell| < 2^53) */
     float resolution;
     float inv_resolution;
     double resolution_d;

@@ -1011,22 +1011,24 @@ __device__ inline void lookup_issue(const NdtGridDev& g, const float4 p, const b
     const double xd1 = R[3] * X0 + R[4] * X1 + R[5] * X2 + t[1];
     const double xd2 = R[6] * X0 + R[7] * X1 + R[8] * X2 + t[2];
     const bool finite = (fabs(xd0) < 1e15) && (fabs(xd1) < 1e15) && (fabs(xd2) < 1e15);
-    /* cells of non-finite points are never used (ok is false for them) */
-    const int64_t a0 = finite ? (int64_t)floor(cell_quot(g, xd0)) : 0;
-    const int64_t a1 = finite ? (int64_t)floor(cell_quot(g, xd1)) : 0;
-    const int64_t a2 = finite ? (int64_t)floor(cell_quot(g, xd2)) : 0;
-    const int64_t c0 = a0 - g.origin_cell[0], c1 = a1 - g.origin_cell[1], c2 = a2 - g.origin_cell[2];
+    /* Cell coordinates in float64: floor() of the quotient and its difference from
+     * the index origin are exact integers there, so no 64-bit integer arithmetic
+     * is needed; inside the index box they convert exactly to int32, and the box
+     * (at most 2^28 cells) keeps the linear index in int32. */
+    const double a0 = floor(cell_quot(g, xd0)), a1 = floor(cell_quot(g, xd1)), a2 = floor(cell_quot(g, xd2));
+    const double c0 = a0 - g.origin_cell_d[0], c1 = a1 - g.origin_cell_d[1], c2 = a2 - g.origin_cell_d[2];
     L.x0 = xd0;
     L.x1 = xd1;
     L.x2 = xd2;
-    L.r0 = (float)(xd0 - (double)a0 * g.resolution_d);
-    L.r1 = (float)(xd1 - (double)a1 * g.resolution_d);
-    L.r2 = (float)(xd2 - (double)a2 * g.resolution_d);
-    const bool ok = valid && finite && c0 >= 0 && c1 >= 0 && c2 >= 0 && c0 < g.dims[0] && c1 < g.dims[1] &&
-                    c2 < g.dims[2];
+    L.r0 = (float)(xd0 - a0 * g.resolution_d);
+    L.r1 = (float)(xd1 - a1 * g.resolution_d);
+    L.r2 = (float)(xd2 - a2 * g.resolution_d);
+    const bool ok = valid && finite && c0 >= 0.0 && c1 >= 0.0 && c2 >= 0.0 && c0 < (double)g.dims[0] &&
+                    c1 < (double)g.dims[1] && c2 < (double)g.dims[2];
+    const int i0 = ok ? (int)c0 : 0, i1 = ok ? (int)c1 : 0, i2 = ok ? (int)c2 : 0;
     /* the load is unconditional (entry 0 stands in for points without a cell), so
      * that its wait is counted exactly where the entry is used */
-    const uint2 v = g.dense[ok ? (c2 * g.dims[1] + c1) * g.dims[0] + c0 : 0];
+    const uint2 v = g.dense[(i2 * g.dims[1] + i1) * g.dims[0] + i0];
     e = ok ? v : make_uint2(0u, 0u);
 }
 
