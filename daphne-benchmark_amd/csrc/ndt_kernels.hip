@@ -1224,9 +1224,10 @@ __device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const Poi
 /* A query cell has at most 27 candidates (its 3 x 3 x 3 neighbourhood). */
 constexpr int MAX_CANDIDATES = 27;
 
-/* One wave's pair table in LDS: lane l's candidates that pass the prefilter at
- * pick[32 l ...] (their pair_add index; a row has room for the 27 candidates plus
- * the branch-free prefilter's writes past the last one), and the wave's pairs in
+/* One wave's pair table in LDS: candidate j of lane l that passes the prefilter
+ * at pick[64 j + l] (its pair_add index; lanes interleaved so that the lanes of
+ * one store hit distinct banks; 32 columns hold the 27 candidates plus the
+ * branch-free prefilter's writes past the last one), and the wave's pairs in
  * evaluation order as positions in pick. */
 constexpr int PICK_ROW = 32;
 struct WavePairs {
@@ -1247,8 +1248,8 @@ __device__ inline double bperm_f64(int src_byte, double v) {
 }
 
 /* The pairs of one point per lane, spread evenly over the wave.  Lane l holds its
- * point's lookup L and has written its cnt prefiltered candidates to its row of
- * wp.pick (cnt <= 27; 0 without a point).  The wave's pairs are numbered lane by lane,
+ * point's lookup L and has written its cnt prefiltered candidates to its column
+ * of wp.pick (cnt <= 27; 0 without a point).  The wave's pairs are numbered lane by lane,
  * in prefilter order within a lane, and pair n is evaluated by lane n mod 64 in
  * round n / 64.  A wave thus spends ceil(pairs / 64) rounds on the pair terms,
  * where a per-point loop spends as many as the busiest lane has pairs.  Each lane
@@ -1267,16 +1268,16 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
         excl += __builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u)) << b;
         total += (uint32_t)__popcll(bal) << b;
     }
-    for (uint32_t j = 0; j < cnt; j++) wp.order[excl + j] = (uint16_t)(PICK_ROW * lane + j);
+    for (uint32_t j = 0; j < cnt; j++) wp.order[excl + j] = (uint16_t)(WAVE * j + lane);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     int pairs = 0;
     for (uint32_t base = 0; base < total; base += WAVE) {
         const uint32_t n = base + lane;
-        const uint32_t d = (n < total) ? (uint32_t)wp.order[n] : PICK_ROW * lane;
+        const uint32_t d = (n < total) ? (uint32_t)wp.order[n] : lane;
         const uint32_t idx = wp.pick[d];
-        const int src = (int)(d / PICK_ROW) << 2;
+        const int src = (int)(d % WAVE) << 2;
         const float4 ps = make_float4(bperm_f32(src, p.x), bperm_f32(src, p.y), bperm_f32(src, p.z), 0.f);
         double x0 = 0.0, x1 = 0.0, x2 = 0.0;
         float r0 = 0.f, r1 = 0.f, r2 = 0.f;
@@ -1345,7 +1346,7 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
              * not counted), so a point waits for one round trip per four
              * candidates.  Branch-free: every candidate is written at the next
              * free position, which a failing one leaves free. */
-            uint32_t* mine = wp.pick + PICK_ROW * lane;
+            uint32_t* mine = wp.pick + lane;
             uint32_t cnt = 0;
             for (uint32_t k0 = 0; k0 < lc.count; k0 += 4) {
                 const uint32_t last = lc.begin + lc.count - 1;
@@ -1357,7 +1358,7 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
                 for (int q = 0; q < 4; q++) {
                     const float e0 = lc.r0 - h[q].x, e1 = lc.r1 - h[q].y, e2 = lc.r2 - h[q].z;
                     const bool in = (k0 + q < lc.count) && (e0 * e0 + e1 * e1 + e2 * e2 < lim);
-                    mine[cnt] = (sizeof(T) == 4) ? i0 + q : __float_as_uint(h[q].w);
+                    mine[WAVE * cnt] = (sizeof(T) == 4) ? i0 + q : __float_as_uint(h[q].w);
                     cnt += in ? 1u : 0u;
                 }
             }
