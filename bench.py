@@ -111,6 +111,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--policy", choices=["race", "fallback"], default="race")
     ap.add_argument("--no-autotune", action="store_true", help="default block size and chunk instead of the tuned ones")
+    ap.add_argument("--block-size", type=int, default=0, help="fixed launch shape (with --points-per-wg): no tuning")
+    ap.add_argument("--points-per-wg", type=int, default=0, help="fixed chunk size (with --block-size): no tuning")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every box here has one GPU", file=sys.stderr)
@@ -131,7 +133,10 @@ def main():
     scans = A.Scans(ctx, w.scans)
     log(f"workload {gen_s:.1f}s, grid {ginfo['build_ms']:.1f} ms")
     tuned_ms = None
-    if not args.no_autotune:
+    if args.block_size or args.points_per_wg:
+        prm.block_size, prm.points_per_wg = args.block_size, args.points_per_wg
+        log(f"fixed launch shape: block {prm.block_size or 256}, chunk {prm.points_per_wg or 2048}")
+    elif not args.no_autotune:
         prm, tuned_ms = scans.autotune(grid, w.guesses, prm)
         log(f"autotuned: block {prm.block_size}, chunk {prm.points_per_wg} ({tuned_ms:.2f} ms)")
 

@@ -504,6 +504,28 @@ def params(mode=NDT_MODE_NDT, **kw) -> NdtParams:
     return p
 
 
+_RESULT_DTYPE = np.dtype({"names": ["final_transformation", "final_transformation_f64", "fitness_score", "score",
+                                     "trans_probability", "iterations", "converged", "passes", "status"],
+                           "formats": [("<f4", (16,)), ("<f8", (16,)), "<f8", "<f8", "<f8", "<i4", "<i4", "<i4", "<i4"],
+                           "offsets": [NdtResult.final_transformation.offset, NdtResult.final_transformation_f64.offset,
+                                       NdtResult.fitness_score.offset, NdtResult.score.offset,
+                                       NdtResult.trans_probability.offset, NdtResult.iterations.offset,
+                                       NdtResult.converged.offset, NdtResult.passes.offset, NdtResult.status.offset],
+                           "itemsize": C.sizeof(NdtResult)})
+
+
+def _results(out, n: int) -> list:
+    """Result dicts of the first n entries of a ctypes NdtResult array, converted
+    with one structured view instead of field by field."""
+    a = np.frombuffer(out, dtype=_RESULT_DTYPE, count=n)
+    f32 = a["final_transformation"].reshape(n, 4, 4).copy()
+    f64 = a["final_transformation_f64"].reshape(n, 4, 4).copy()
+    score, tp = a["score"].tolist(), a["trans_probability"].tolist()
+    it, cv, ps, st = a["iterations"].tolist(), a["converged"].tolist(), a["passes"].tolist(), a["status"].tolist()
+    return [dict(final_f32=f32[i], final_f64=f64[i], score=score[i], trans_probability=tp[i], iterations=it[i],
+                 converged=bool(cv[i]), passes=ps[i], status=st[i]) for i in range(n)]
+
+
 def _result(r: NdtResult) -> dict:
     return dict(final_f32=np.ctypeslib.as_array(r.final_transformation).reshape(4, 4).copy(),
                 final_f64=np.ctypeslib.as_array(r.final_transformation_f64).reshape(4, 4).copy(),
@@ -626,7 +648,7 @@ class Grid:
         out = (NdtResult * S)()
         _check(hip_lib().ndt_align_batch(self.ctx.h, self.h, ptrs, counts.ctypes.data, S, guesses.ctypes.data,
                                          C.byref(prm), out))
-        return [_result(out[i]) for i in range(S)]
+        return _results(out, S)
 
 
 class Scans:
@@ -658,7 +680,7 @@ class Scans:
         guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(self.S, 16)
         out = (NdtResult * self.S)()
         _check(hip_lib().ndt_align_scans(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm), out))
-        return [_result(out[i]) for i in range(self.S)]
+        return _results(out, self.S)
 
     def autotune(self, grid: Grid, guesses, prm: NdtParams | None = None, cache_path: str | None = None):
         """Fastest (block size, chunk) for this batch; returns (params, kernel ms)."""
@@ -820,7 +842,7 @@ class Model(_Handle):
         used = np.zeros(scans.S, np.int32)
         _check(hip_lib().ndt_align_scans_learned(self.ctx.h, grid.h, bevmap.h, self.h, scans.h, g.ctypes.data,
                                                  C.byref(prm), policy, out, used.ctypes.data))
-        return [_result(out[i]) for i in range(scans.S)], used
+        return _results(out, scans.S), used
 
 
 class Dataset(_Handle):

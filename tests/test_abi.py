@@ -5,6 +5,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -57,3 +58,21 @@ def test_no_device_fails_loudly(A):
         pytest.skip("a GPU is present")
     with pytest.raises(A.NdtError):
         A.Context(0)
+
+
+def test_result_conversion_matches_fields(A):
+    """The structured-view conversion of result arrays gives the same dicts as
+    reading the ctypes fields one by one."""
+    import ctypes as C
+    out = (A.NdtResult * 7)()
+    for i in range(7):
+        out[i].iterations, out[i].converged, out[i].passes, out[i].status = i, i % 2, 3 * i, i % 3
+        out[i].score, out[i].trans_probability = -1.5 * i, 0.25 * i
+        for k in range(16):
+            out[i].final_transformation[k] = i + 0.5 * k
+            out[i].final_transformation_f64[k] = i * 1e7 + k / 3.0
+    assert A._RESULT_DTYPE.itemsize == C.sizeof(A.NdtResult)
+    for x, y in zip([A._result(out[i]) for i in range(7)], A._results(out, 7)):
+        assert x.keys() == y.keys()
+        for k in x:
+            assert np.array_equal(x[k], y[k]) and type(x[k]) is type(y[k]), k
