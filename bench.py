@@ -38,6 +38,29 @@ sys.path.insert(0, os.path.join(REPO, "daphne-benchmark_amd"))
 import ndt_amd as A  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+
+TRAFFIC_PROFILE = "profiles/k_align_wq_cfg2_batch512_fp64.md"
+
+
+def profiled_traffic(args, prm):
+    """HBM bytes per batch launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE,
+    calibrated) and recorded in the committed profile, when that profile covers this
+    workload, precision and launch shape; None otherwise."""
+    import re
+    try:
+        with open(os.path.join(REPO, TRAFFIC_PROFILE)) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch: (\d+) \(workload (\w+), (\d+) scans, (\w+), block (\d+), chunk (\d+)\)",
+                  text)
+    if not m:
+        return None
+    shape = (prm.block_size or 256, prm.points_per_wg or 2048)
+    if (m.group(2), int(m.group(3)), m.group(4), (int(m.group(5)), int(m.group(6)))) != (
+            args.workload, args.scans, args.precision, shape):
+        return None
+    return int(m.group(1))
 FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
 
@@ -247,7 +270,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(args, prm),
+            "traffic_source": TRAFFIC_PROFILE if profiled_traffic(args, prm) else None,
             "valu_frac": valu_frac,
             "kernel": "ndt::k_align_wq (work-queue align launch of the 512-scan batch, no proposer)",
         },
