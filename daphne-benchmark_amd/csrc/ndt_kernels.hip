@@ -1247,6 +1247,13 @@ __device__ inline double bperm_f64(int src_byte, double v) {
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
 }
 
+/* candidate heads a lane loads at once in the prefilter: all eight loads are in
+ * flight together, so most points wait for one round trip (a query cell has
+ * 3.7 candidates on average on the 1 M-point map) */
+#ifndef HEAD_BATCH
+#define HEAD_BATCH 8
+#endif
+
 /* The pairs of one point per lane, spread evenly over the wave.  Lane l holds its
  * point's lookup L and has written its cnt prefiltered candidates to its column
  * of wp.pick (cnt <= 27; 0 without a point).  The wave's pairs are numbered lane by lane,
@@ -1340,22 +1347,22 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
             const float4 pn = pts[min(ib + BLOCK + lane, pe - 1)];
             lc.begin = ec.x;
             lc.count = ec.y;
-            /* prefilter: the passing candidates go to this lane's row of the table.
-             * Heads are loaded four at a time, all four loads issued before the
-             * first is used (indices past the run repeat its last entry and are
-             * not counted), so a point waits for one round trip per four
+            /* prefilter: the passing candidates go to this lane's column of the
+             * table.  Heads are loaded HEAD_BATCH at a time, all loads issued before
+             * the first is used (indices past the run repeat its last entry and are
+             * not counted), so a point waits for one round trip per HEAD_BATCH
              * candidates.  Branch-free: every candidate is written at the next
              * free position, which a failing one leaves free. */
             uint32_t* mine = wp.pick + lane;
             uint32_t cnt = 0;
-            for (uint32_t k0 = 0; k0 < lc.count; k0 += 4) {
+            for (uint32_t k0 = 0; k0 < lc.count; k0 += HEAD_BATCH) {
                 const uint32_t last = lc.begin + lc.count - 1;
                 const uint32_t i0 = lc.begin + k0;
-                float4 h[4];
+                float4 h[HEAD_BATCH];
 #pragma unroll
-                for (int q = 0; q < 4; q++) h[q] = g.head[min(i0 + q, last)];
+                for (int q = 0; q < HEAD_BATCH; q++) h[q] = g.head[min(i0 + q, last)];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < HEAD_BATCH; q++) {
                     const float e0 = lc.r0 - h[q].x, e1 = lc.r1 - h[q].y, e2 = lc.r2 - h[q].z;
                     const bool in = (k0 + q < lc.count) && (e0 * e0 + e1 * e1 + e2 * e2 < lim);
                     mine[WAVE * cnt] = (sizeof(T) == 4) ? i0 + q : __float_as_uint(h[q].w);
