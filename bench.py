@@ -297,6 +297,10 @@ def main():
         tc = time.perf_counter()
         ref = og.align_many([w.scans[i] for i in idx], local, threads=threads)
         cpu_s = time.perf_counter() - tc
+        # single-thread latency on the first 4 of them
+        tc1 = time.perf_counter()
+        og.align_many([w.scans[i] for i in idx[:4]], local[:4], threads=1)
+        single_s = (time.perf_counter() - tc1) / min(4, len(idx))
         shifted = local.copy()
         shifted[:, 0, 3] += 1e-12
         ref2 = og.align_many([w.scans[i] for i in idx], shifted, threads=threads)
@@ -314,6 +318,7 @@ def main():
             "value": len(idx) / cpu_s, "unit": "alignments/s", "cores": threads, "kind": "port",
             "sample": f"{len(idx)} scans of the same batch ({int(sum(counts[i] for i in idx))} points) from the "
                       f"same starting guesses, float64 oracle (oracle/ndt_oracle.c), one scan per OpenMP thread",
+            "single_thread_ms_per_alignment": single_s * 1e3,
         }
         out["accuracy_vs_fp64_oracle"] = {
             "scans": len(idx),
