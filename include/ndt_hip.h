@@ -35,8 +35,8 @@ extern "C" {
 
 typedef enum ndt_status {
     NDT_OK = 0,
-    NDT_ERR_INVALID = 1,     /* bad argument (null pointer, empty cloud, bad parameter, a
-                                guess or pose with a non-finite entry) */
+    NDT_ERR_INVALID = 1,     /* bad argument (null pointer, empty cloud, bad parameter, an
+                                ndt-mode guess or pose with a non-finite entry) */
     NDT_ERR_HIP = 2,         /* HIP runtime error */
     NDT_ERR_NOMEM = 3,       /* device allocation failed (or would exceed the byte count in
                                 the environment variable NDT_DEVICE_MEMORY_LIMIT) */
