@@ -635,7 +635,8 @@ constexpr uint64_t MAX_QUEUE_ITEMS = 1ull << 28;
  * The chunk size (points per partial sum) depends only on the scan's own size
  * unless the caller sets points_per_wg, and partial sums are reduced in a fixed
  * order, so a scan's result does not depend on the other scans of the batch, on
- * the number of workgroups the device runs, nor on the scheduler.  The work-queue scheduler runs the whole
+ * the number of workgroups the device runs, nor on the scheduler (it does depend
+ * on the block size, which groups the points into waves).  The work-queue scheduler runs the whole
  * batch in one launch; the round scheduler runs groups of up to MAX_BATCH_SCANS
  * scans and is also used when the queue would exceed MAX_QUEUE_ITEMS. */
 static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
