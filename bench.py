@@ -133,6 +133,8 @@ def main():
     ap.add_argument("--train-per-scan", type=int, default=8)
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--policy", choices=["race", "fallback"], default="race")
+    ap.add_argument("--full-pose", action="store_true",
+                    help="refined guesses take the model's rotation too (default: its translation only)")
     ap.add_argument("--no-autotune", action="store_true", help="default block size and chunk instead of the tuned ones")
     ap.add_argument("--block-size", type=int, default=0, help="fixed launch shape (with --points-per-wg): no tuning")
     ap.add_argument("--points-per-wg", type=int, default=0, help="fixed chunk size (with --block-size): no tuning")
@@ -182,6 +184,8 @@ def main():
         model, tinfo = train_proposer(ctx, w, grid, bm, prm, args.train_scans, args.train_per_scan, args.epochs)
         log(f"proposer trained: {tinfo}")
         policy = A.LEARNED_RACE if args.policy == "race" else A.LEARNED_FALLBACK
+        if not args.full_pose:
+            policy |= A.LEARNED_KEEP_ROTATION
         for _ in range(args.warmup):
             model.align(grid, bm, scans, w.guesses, prm, policy)
         t_l = time.perf_counter()
@@ -190,8 +194,11 @@ def main():
         learned_s = (time.perf_counter() - t_l) / args.steps
         learned_ms = ctx.last_timing()[0]
         refined = model.predict(bm, scans, w.guesses)
+        if not args.full_pose:
+            refined[:, :3, :3] = w.guesses[:, :3, :3]
+        policy_name = args.policy + ("" if args.full_pose else ", translation from the model")
         learned = dict(model=model, bm=bm, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
-                       refined=refined, train=tinfo, policy=args.policy)
+                       refined=refined, train=tinfo, policy=policy_name)
         log(f"batched+learned {S / learned_s:.0f}/s")
 
     # ---- plain single-scan GPU NDT on the same scans: one submission and host sync
@@ -242,7 +249,7 @@ def main():
                         f"{S} scans of {min(counts)}-{max(counts)} points (log-uniform), guesses "
                         f"+-{ws['perturb_xyz']} m / +-{ws['perturb_yaw_deg']} deg yaw / "
                         f"+-{ws['perturb_rollpitch_deg']} deg roll-pitch; "
-                        + (f"batched + learned warm start ({args.policy})" if learned else "batched, no proposer"),
+                        + (f"batched + learned warm start ({learned['policy']})" if learned else "batched, no proposer"),
             "scans_per_step": S,
             "scan_points_total": int(sum(counts)),
             "map_points": int(w.map_xyzw.shape[0]),

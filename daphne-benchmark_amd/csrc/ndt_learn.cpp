@@ -847,6 +847,8 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
                                        const ndt_scans* sc, const double* guesses, const ndt_params* p, int32_t policy,
                                        ndt_result* results, int32_t* used_learned) {
     if (!c || !g || !bm || !md || !sc || !guesses || !p || !results) return fail(NDT_ERR_INVALID, "null argument");
+    const bool keep_rotation = (policy & NDT_LEARNED_KEEP_ROTATION) != 0;
+    policy &= ~NDT_LEARNED_KEEP_ROTATION;
     if (policy != NDT_LEARNED_RACE && policy != NDT_LEARNED_FALLBACK) return fail(NDT_ERR_INVALID, "unknown policy");
     if (int e = check_poses(guesses, sc->S, "guess")) return e;
     HIP_TRY(hipSetDevice(c->device));
@@ -864,6 +866,10 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
     std::vector<double> refined((size_t)(16 * S));
     int e = predict_dev(c, md, bm, sc, guesses, refined.data());
     if (e) return e;
+    if (keep_rotation)
+        for (int s = 0; s < S; s++)
+            for (int r = 0; r < 3; r++)
+                for (int k = 0; k < 3; k++) refined[16 * (size_t)s + 4 * r + k] = guesses[16 * (size_t)s + 4 * r + k];
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     float fms = 0.f;

@@ -714,7 +714,7 @@ BEV_CELLS = 32768
 FEATURES = 65536
 W1, B1, W2, B2, W3, B3 = range(6)
 CORR_SCALE = np.array([3.0, 3.0, 1.0, 0.05, 0.05, 0.2])
-LEARNED_RACE, LEARNED_FALLBACK = 0, 1
+LEARNED_RACE, LEARNED_FALLBACK, LEARNED_KEEP_ROTATION = 0, 1, 0x10
 
 
 def train_params(**kw) -> TrainParams:

@@ -62,8 +62,11 @@ typedef struct ndt_train_params {
  *                        result is never worse, by the NDT score, than plain NDT;
  *   NDT_LEARNED_FALLBACK NDT runs from the refined guess; plain NDT from the given
  *                        guess replaces it where it did not converge or ended with
- *                        a lower score than the given guess itself has. */
-enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1 };
+ *                        a lower score than the given guess itself has.
+ * Either may be combined (bitwise or) with
+ *   NDT_LEARNED_KEEP_ROTATION  the refined guess takes only the model's translation
+ *                        and keeps the given guess's rotation. */
+enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1, NDT_LEARNED_KEEP_ROTATION = 0x10 };
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m
  * tiles for the map grids; origin = world position of the map-local frame. */

@@ -166,18 +166,26 @@ def test_learned_warm_start_gains(A, trained):
 
 
 def test_policies_return_plain_or_learned_runs(A, trained):
-    """Every result of either policy is bit for bit the NDT run from the given guess
-    or the one from the refined guess, as used_learned says; the race keeps the
-    better final score."""
+    """Every result of either policy, with the model's full pose or only its
+    translation, is bit for bit the NDT run from the given guess or the one from the
+    refined guess, as used_learned says; the race keeps the better final score; an
+    unknown policy is rejected."""
     t = trained
     ev, eg = t["evS"], t["eg"]
     pred = t["model"].predict(t["bm"], ev, eg)
     plain = ev.align(t["grid"], eg, t["prm"])
     from_pred = ev.align(t["grid"], pred, t["prm"])
-    for policy in (A.LEARNED_RACE, A.LEARNED_FALLBACK):
+    pred_t = pred.copy()
+    pred_t[:, :3, :3] = eg[:, :3, :3]
+    from_pred_t = ev.align(t["grid"], pred_t, t["prm"])
+    for policy in (A.LEARNED_RACE, A.LEARNED_FALLBACK, A.LEARNED_RACE | A.LEARNED_KEEP_ROTATION,
+                   A.LEARNED_FALLBACK | A.LEARNED_KEEP_ROTATION):
+        learned = from_pred_t if policy & A.LEARNED_KEEP_ROTATION else from_pred
         res, used = t["model"].align(t["grid"], t["bm"], ev, eg, t["prm"], policy)
         for s, r in enumerate(res):
-            ref = from_pred[s] if used[s] else plain[s]
+            ref = learned[s] if used[s] else plain[s]
             assert np.array_equal(r["final_f64"], ref["final_f64"]) and r["iterations"] == ref["iterations"], (policy, s)
-            if policy == A.LEARNED_RACE and plain[s]["converged"] == from_pred[s]["converged"]:
-                assert r["score"] == max(plain[s]["score"], from_pred[s]["score"])
+            if (policy & 1) == A.LEARNED_RACE and plain[s]["converged"] == learned[s]["converged"]:
+                assert r["score"] == max(plain[s]["score"], learned[s]["score"])
+    with pytest.raises(A.NdtError):
+        t["model"].align(t["grid"], t["bm"], ev, eg, t["prm"], 7)
