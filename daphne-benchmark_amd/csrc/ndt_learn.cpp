@@ -111,7 +111,7 @@ struct ndt_model {
     int64_t step = 0;
     /* batch workspace */
     int wsB = 0, splits1 = 1, splits2 = 1;
-    DevBuf X, XT, P1, H1, H1T, P2, H2, Y, dY, lossb, dZ2, dZ2T, dH1, dZ1T, target, weights, lossacc;
+    DevBuf X, XT, P1, H1, H1T, P2, H2, Y, dY, lossb, dZ2, dZ2T, dH1, dZ1T, target, weights, lossacc, bpart;
     /* prediction inputs */
     DevBuf sbev, idx, gl, off, cnt;
 };
@@ -253,6 +253,7 @@ static int ensure_workspace(ndt_model* md, int B) {
     BUF_TRY(md->dH1, 4 * (size_t)B * f1);
     BUF_TRY(md->dZ1T, 2 * (size_t)B * f1);
     BUF_TRY(md->lossacc, 4 * 64);
+    BUF_TRY(md->bpart, 4 * (size_t)bwd_tiles(B) * (size_t)std::max<int64_t>(f1, (NOUT + 1) * f2 + NOUT));
     md->wsB = B;
     return 0;
 }
@@ -466,12 +467,12 @@ static int backward_dev(ndt_model* md, int B) {
     const int f0 = md->f0, f1 = md->f1, f2 = md->f2;
     LAUNCH_TRY(launch_head_bwd(md->dY.as<float>(), md->H2.as<uint16_t>(), md->wb3.as<uint16_t>(), B, f2,
                                md->g[NDT_W3].as<float>(), md->g[NDT_B3].as<float>(), md->dZ2.as<uint16_t>(),
-                               md->dZ2T.as<uint16_t>(), md->g[NDT_B2].as<float>(), st));
+                               md->dZ2T.as<uint16_t>(), md->g[NDT_B2].as<float>(), md->bpart.as<float>(), st));
     /* dH1 [B x f1] = dZ2 [B x f2] W2 = dZ2 (W2^T)^T */
     LAUNCH_TRY(launch_gemm_nt(md->dZ2.as<uint16_t>(), f2, md->wb2t.as<uint16_t>(), f2, md->dH1.as<float>(), f1, B, f1,
                               f2, 1, st));
     LAUNCH_TRY(launch_relu_bwd(md->dH1.as<float>(), md->H1.as<uint16_t>(), B, f1, md->dZ1T.as<uint16_t>(),
-                               md->g[NDT_B1].as<float>(), st));
+                               md->g[NDT_B1].as<float>(), md->bpart.as<float>(), st));
     /* dW2 [f2 x f1] = dZ2^T H1 */
     LAUNCH_TRY(launch_gemm_nt(md->dZ2T.as<uint16_t>(), B, md->H1T.as<uint16_t>(), B, md->g[NDT_W2].as<float>(), f1, f2,
                               f1, B, 1, st));

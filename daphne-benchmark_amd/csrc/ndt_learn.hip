@@ -41,9 +41,10 @@ __device__ inline float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b <<
 __device__ inline int bev_index(double qx, double qy, double qz) {
     const double fx = floor((qx - BEV_X0) * 2.0);
     const double fy = floor((qy - BEV_Y0) * 2.0);
+    if (!(fx >= 0.0 && fx < (double)BEV_NX && fy >= 0.0 && fy < (double)BEV_NY)) return -1;
+    /* the division only for points inside the footprint */
     const double fz = floor((qz - BEV_Z0) / BEV_DZ);
-    if (!(fx >= 0.0 && fx < (double)BEV_NX && fy >= 0.0 && fy < (double)BEV_NY && fz >= 0.0 && fz < (double)BEV_NZ))
-        return -1;
+    if (!(fz >= 0.0 && fz < (double)BEV_NZ)) return -1;
     return ((int)fz * BEV_NY + (int)fy) * BEV_NX + (int)fx;
 }
 
@@ -60,40 +61,38 @@ __device__ inline void bev_emit(const uint32_t* hist, int half, const uint16_t* 
     }
 }
 
-/* One workgroup per scan: histogram of its points (sensor frame), two passes of
- * four z planes each so that exact 32-bit counters fit in 64 KiB of LDS. */
+/* Two workgroups per scan, one per half of the cells (z planes 0-3, then 4-7),
+ * so that exact 32-bit counters fit in 64 KiB of LDS: histogram of the scan's
+ * points (sensor frame) falling in that half. */
 __global__ __launch_bounds__(BEV_BLOCK) void k_bev_scan(const float4* __restrict__ pts,
                                                         const int64_t* __restrict__ offsets,
                                                         const int32_t* __restrict__ counts,
                                                         const uint16_t* __restrict__ lut, uint16_t* __restrict__ out) {
     __shared__ uint32_t hist[BEV_HALF];
-    const int s = blockIdx.x;
+    const int s = blockIdx.x, half = blockIdx.y;
     const int64_t off = offsets[s];
     const int n = counts[s];
     uint16_t* o = out + (int64_t)s * BEV_CELLS;
-    for (int half = 0; half < 2; half++) {
-        bev_clear(hist);
-        __syncthreads();
-        for (int i = threadIdx.x; i < n; i += BEV_BLOCK) {
-            const float4 p = pts[off + i];
-            const int c = bev_index((double)p.x, (double)p.y, (double)p.z) - half * BEV_HALF;
-            if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
-        }
-        __syncthreads();
-        bev_emit(hist, half, lut, o);
-        __syncthreads();
+    bev_clear(hist);
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += BEV_BLOCK) {
+        const float4 p = pts[off + i];
+        const int c = bev_index((double)p.x, (double)p.y, (double)p.z) - half * BEV_HALF;
+        if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
     }
+    __syncthreads();
+    bev_emit(hist, half, lut, o);
 }
 
 int launch_bev_scan(const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S, const uint16_t* lut,
                     uint16_t* out, hipStream_t stream) {
     if (S <= 0) return 0;
-    hipLaunchKernelGGL(k_bev_scan, dim3(S), dim3(BEV_BLOCK), 0, stream, pts, offsets, counts, lut, out);
+    hipLaunchKernelGGL(k_bev_scan, dim3(S, 2), dim3(BEV_BLOCK), 0, stream, pts, offsets, counts, lut, out);
     return (int)hipGetLastError();
 }
 
-/* One workgroup per guess: histogram of the map points around the guess, in the
- * guess frame q = R^T (p - t).  Only the tiles under the grid's footprint are
+/* Two workgroups per guess, one per half of the cells: histogram of the map
+ * points around the guess, in the guess frame q = R^T (p - t).  Only the tiles under the grid's footprint are
  * visited: the footprint's corners at both z limits, mapped to the map frame,
  * bound the tile rows and columns. */
 __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const double* __restrict__ guesses,
@@ -101,7 +100,7 @@ __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const do
                                                         int64_t ldx, int64_t col_off) {
     __shared__ uint32_t hist[BEV_HALF];
     __shared__ int s_rng[4];
-    const int b = blockIdx.x;
+    const int b = blockIdx.x, half = blockIdx.y;
     const double* G = guesses + 16 * (int64_t)b;
     const double r00 = G[0], r01 = G[1], r02 = G[2], t0 = G[3];
     const double r10 = G[4], r11 = G[5], r12 = G[6], t1 = G[7];
@@ -137,32 +136,29 @@ __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const do
     __syncthreads();
     const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
     uint16_t* o = X + (int64_t)b * ldx + col_off;
-    for (int half = 0; half < 2; half++) {
-        bev_clear(hist);
-        __syncthreads();
-        for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
-            const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
-            const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
-            for (uint32_t i = pb + threadIdx.x; i < pe; i += BEV_BLOCK) {
-                const float4 p = tg.pts[i];
-                const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
-                const double qx = r00 * dx + r10 * dy + r20 * dz;
-                const double qy = r01 * dx + r11 * dy + r21 * dz;
-                const double qz = r02 * dx + r12 * dy + r22 * dz;
-                const int c = bev_index(qx, qy, qz) - half * BEV_HALF;
-                if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
-            }
+    bev_clear(hist);
+    __syncthreads();
+    for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
+        const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
+        const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
+        for (uint32_t i = pb + threadIdx.x; i < pe; i += BEV_BLOCK) {
+            const float4 p = tg.pts[i];
+            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+            const double qx = r00 * dx + r10 * dy + r20 * dz;
+            const double qy = r01 * dx + r11 * dy + r21 * dz;
+            const double qz = r02 * dx + r12 * dy + r22 * dz;
+            const int c = bev_index(qx, qy, qz) - half * BEV_HALF;
+            if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
         }
-        __syncthreads();
-        bev_emit(hist, half, lut, o);
-        __syncthreads();
     }
+    __syncthreads();
+    bev_emit(hist, half, lut, o);
 }
 
 int launch_bev_crop(TileGridDev tg, const double* guesses, int32_t B, const uint16_t* lut, uint16_t* X, int64_t ldx,
                     int64_t col_off, hipStream_t stream) {
     if (B <= 0) return 0;
-    hipLaunchKernelGGL(k_bev_crop, dim3(B), dim3(BEV_BLOCK), 0, stream, tg, guesses, lut, X, ldx, col_off);
+    hipLaunchKernelGGL(k_bev_crop, dim3(B, 2), dim3(BEV_BLOCK), 0, stream, tg, guesses, lut, X, ldx, col_off);
     return (int)hipGetLastError();
 }
 
@@ -368,72 +364,165 @@ int launch_head_fwd(const uint16_t* H2, const uint16_t* W3, const float* b3, int
     return (int)hipGetLastError();
 }
 
-/* Backward of the output layer and the second ReLU, one thread per hidden unit n
- * of layer 2, samples in order: dW3[o][n] = sum_b dY[b][o] H2[b][n],
- * dZ2[b][n] = [H2[b][n] > 0] sum_o dY[b][o] W3[o][n] (bf16, and transposed),
- * db2[n] = sum_b dZ2[b][n]; thread 0 also forms db3[o] = sum_b dY[b][o]. */
-__global__ void k_head_bwd(const float* __restrict__ dY, const uint16_t* __restrict__ H2,
-                           const uint16_t* __restrict__ W3, int B, int F2, float* __restrict__ dW3,
-                           float* __restrict__ db3, uint16_t* __restrict__ dZ2, uint16_t* __restrict__ dZ2T,
-                           float* __restrict__ db2) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= F2) return;
+/* The two backward kernels below cut the (sample, unit) plane into tiles of
+ * BWD_ROWS samples x 64 units, one tile per 256-thread workgroup: each of the four
+ * waves takes BWD_ROWS / 4 consecutive samples of the tile, in order.  Sums over
+ * samples are formed per wave, added over the waves in wave order and left per
+ * tile in `part`; a second kernel adds the tiles in tile order.  Every sum
+ * therefore has one fixed order.  The transposed bf16 outputs go through an LDS
+ * tile so that both layouts are written in contiguous runs. */
+constexpr int BWD_ROWS = 32;
+constexpr int BWD_WAVES = 4;
+constexpr int BWD_PER_WAVE = BWD_ROWS / BWD_WAVES;
+
+int bwd_tiles(int B) { return (B + BWD_ROWS - 1) / BWD_ROWS; }
+
+/* tile [64 units][BWD_ROWS samples] -> dst[unit][sample], row length B */
+__device__ inline void store_tile_t(const uint16_t (*tile)[BWD_ROWS + 2], int n0, int b0, int F, int B,
+                                    uint16_t* __restrict__ dst) {
+    for (int e = threadIdx.x; e < 64 * BWD_ROWS; e += 256) {
+        const int u = e / BWD_ROWS, r = e % BWD_ROWS;
+        const int n = n0 + u, b = b0 + r;
+        if (n < F && b < B) dst[(int64_t)n * B + b] = tile[u][r];
+    }
+}
+
+/* Backward of the output layer and the second ReLU over one tile:
+ * dZ2[b][n] = [H2[b][n] > 0] sum_o dY[b][o] W3[o][n] (bf16, also transposed into
+ * dZ2T), and per tile the partial sums over its samples of dY[b][o] H2[b][n]
+ * (-> dW3[o][n]), of dZ2[b][n] (-> db2[n]) and, in the first column of tiles,
+ * of dY[b][o] (-> db3[o]).  part holds tiles x (NOUT + 1) x F2 values, then
+ * tiles x NOUT. */
+__global__ __launch_bounds__(256) void k_head_bwd(const float* __restrict__ dY, const uint16_t* __restrict__ H2,
+                                                  const uint16_t* __restrict__ W3, int B, int F2,
+                                                  uint16_t* __restrict__ dZ2, uint16_t* __restrict__ dZ2T,
+                                                  float* __restrict__ part) {
+    __shared__ uint16_t tile[64][BWD_ROWS + 2];
+    __shared__ float sy[BWD_ROWS][NOUT];
+    __shared__ float psum[BWD_WAVES][NOUT + 1][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n0 = blockIdx.x * 64, n = n0 + lane;
+    const int b0 = blockIdx.y * BWD_ROWS;
+    for (int e = threadIdx.x; e < BWD_ROWS * NOUT; e += 256) {
+        const int b = b0 + e / NOUT;
+        sy[e / NOUT][e % NOUT] = b < B ? dY[(int64_t)b * NOUT + e % NOUT] : 0.f;
+    }
     float w3[NOUT], gw[NOUT];
+#pragma unroll
     for (int o = 0; o < NOUT; o++) {
-        w3[o] = bf2f(W3[(int64_t)o * F2 + n]);
+        w3[o] = n < F2 ? bf2f(W3[(int64_t)o * F2 + n]) : 0.f;
         gw[o] = 0.f;
     }
+    __syncthreads();
     float gb = 0.f;
-    for (int b = 0; b < B; b++) {
-        const float h = bf2f(H2[(int64_t)b * F2 + n]);
+#pragma unroll
+    for (int i = 0; i < BWD_PER_WAVE; i++) {
+        const int r = wv * BWD_PER_WAVE + i, b = b0 + r;
+        const bool in = n < F2 && b < B;
+        const float h = in ? bf2f(H2[(int64_t)b * F2 + n]) : 0.f;
         float dz = 0.f;
+#pragma unroll
         for (int o = 0; o < NOUT; o++) {
-            const float d = dY[(int64_t)b * NOUT + o];
+            const float d = sy[r][o];
             dz += d * w3[o];
             gw[o] += d * h;
         }
         if (!(h > 0.f)) dz = 0.f;
         const uint16_t zb = f2bf(dz);
-        dZ2[(int64_t)b * F2 + n] = zb;
-        dZ2T[(int64_t)n * B + b] = zb;
+        if (in) dZ2[(int64_t)b * F2 + n] = zb;
+        tile[lane][r] = zb;
         gb += dz;
     }
-    for (int o = 0; o < NOUT; o++) dW3[(int64_t)o * F2 + n] = gw[o];
-    db2[n] = gb;
-    if (n == 0) {
-        for (int o = 0; o < NOUT; o++) {
-            float s = 0.f;
-            for (int b = 0; b < B; b++) s += dY[(int64_t)b * NOUT + o];
-            db3[o] = s;
-        }
+#pragma unroll
+    for (int o = 0; o < NOUT; o++) psum[wv][o][lane] = gw[o];
+    psum[wv][NOUT][lane] = gb;
+    __syncthreads();
+    if (wv == 0 && n < F2) {
+#pragma unroll
+        for (int q = 0; q <= NOUT; q++)
+            part[((int64_t)blockIdx.y * (NOUT + 1) + q) * F2 + n] =
+                ((psum[0][q][lane] + psum[1][q][lane]) + psum[2][q][lane]) + psum[3][q][lane];
+    }
+    if (blockIdx.x == 0 && threadIdx.x < NOUT) {
+        float s = 0.f;
+        for (int r = 0; r < BWD_ROWS; r++) s += sy[r][threadIdx.x];
+        part[(int64_t)gridDim.y * (NOUT + 1) * F2 + blockIdx.y * NOUT + threadIdx.x] = s;
+    }
+    store_tile_t(tile, n0, b0, F2, B, dZ2T);
+}
+
+/* dW3, db2 and db3 from the per-tile partials of k_head_bwd, tiles in order;
+ * blockIdx.y picks the output (dW3 row q < NOUT, or db2) */
+__global__ void k_head_bwd_sum(const float* __restrict__ part, int tiles, int F2, float* __restrict__ dW3,
+                               float* __restrict__ db2, float* __restrict__ db3) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    const int q = blockIdx.y;
+    if (n < F2) {
+        float s = 0.f;
+        for (int t = 0; t < tiles; t++) s += part[((int64_t)t * (NOUT + 1) + q) * F2 + n];
+        if (q < NOUT) dW3[(int64_t)q * F2 + n] = s;
+        else db2[n] = s;
+    }
+    if (blockIdx.x == 0 && q == 0 && threadIdx.x < NOUT) {
+        const float* p3 = part + (int64_t)tiles * (NOUT + 1) * F2;
+        float s = 0.f;
+        for (int t = 0; t < tiles; t++) s += p3[t * NOUT + threadIdx.x];
+        db3[threadIdx.x] = s;
     }
 }
 
 int launch_head_bwd(const float* dY, const uint16_t* H2, const uint16_t* W3, int B, int F2, float* dW3, float* db3,
-                    uint16_t* dZ2, uint16_t* dZ2T, float* db2, hipStream_t stream) {
-    hipLaunchKernelGGL(k_head_bwd, dim3(blocks_for(F2, 64)), dim3(64), 0, stream, dY, H2, W3, B, F2, dW3, db3, dZ2, dZ2T,
-                       db2);
+                    uint16_t* dZ2, uint16_t* dZ2T, float* db2, float* part, hipStream_t stream) {
+    const int tiles = bwd_tiles(B);
+    hipLaunchKernelGGL(k_head_bwd, dim3(blocks_for(F2, 64), tiles), dim3(256), 0, stream, dY, H2, W3, B, F2, dZ2, dZ2T,
+                       part);
+    hipLaunchKernelGGL(k_head_bwd_sum, dim3(blocks_for(F2, 64), NOUT + 1), dim3(64), 0, stream, part, tiles, F2, dW3,
+                       db2, db3);
     return (int)hipGetLastError();
 }
 
-/* Backward of a ReLU layer, one thread per unit n, samples in order:
- * dZ[b][n] = [H[b][n] > 0] dH[b][n], written transposed in bf16; db[n] = sum_b dZ. */
-__global__ void k_relu_bwd(const float* __restrict__ dH, const uint16_t* __restrict__ H, int B, int F,
-                           uint16_t* __restrict__ dZT, float* __restrict__ db) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= F) return;
+/* Backward of a ReLU layer over one tile: dZ[b][n] = [H[b][n] > 0] dH[b][n],
+ * written transposed in bf16, and per tile the partial sums over its samples of
+ * dZ (-> db[n]); part holds tiles x F values. */
+__global__ __launch_bounds__(256) void k_relu_bwd(const float* __restrict__ dH, const uint16_t* __restrict__ H, int B,
+                                                  int F, uint16_t* __restrict__ dZT, float* __restrict__ part) {
+    __shared__ uint16_t tile[64][BWD_ROWS + 2];
+    __shared__ float psum[BWD_WAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int n0 = blockIdx.x * 64, n = n0 + lane;
+    const int b0 = blockIdx.y * BWD_ROWS;
     float gb = 0.f;
-    for (int b = 0; b < B; b++) {
-        const float h = bf2f(H[(int64_t)b * F + n]);
-        const float dz = (h > 0.f) ? dH[(int64_t)b * F + n] : 0.f;
-        dZT[(int64_t)n * B + b] = f2bf(dz);
+#pragma unroll
+    for (int i = 0; i < BWD_PER_WAVE; i++) {
+        const int r = wv * BWD_PER_WAVE + i, b = b0 + r;
+        float dz = 0.f;
+        if (n < F && b < B) {
+            const float h = bf2f(H[(int64_t)b * F + n]);
+            dz = (h > 0.f) ? dH[(int64_t)b * F + n] : 0.f;
+        }
+        tile[lane][r] = f2bf(dz);
         gb += dz;
     }
-    db[n] = gb;
+    psum[wv][lane] = gb;
+    __syncthreads();
+    if (wv == 0 && n < F) part[(int64_t)blockIdx.y * F + n] = ((psum[0][lane] + psum[1][lane]) + psum[2][lane]) + psum[3][lane];
+    store_tile_t(tile, n0, b0, F, B, dZT);
 }
 
-int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, hipStream_t stream) {
-    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(F, 64)), dim3(64), 0, stream, dH, H, B, F, dZT, db);
+/* out[n] = sum over tiles t, in order, of part[t][n] */
+__global__ void k_sum_tiles(const float* __restrict__ part, int tiles, int F, float* __restrict__ out) {
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= F) return;
+    float s = 0.f;
+    for (int t = 0; t < tiles; t++) s += part[(int64_t)t * F + n];
+    out[n] = s;
+}
+
+int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, float* part,
+                    hipStream_t stream) {
+    const int tiles = bwd_tiles(B);
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(F, 64), tiles), dim3(256), 0, stream, dH, H, B, F, dZT, part);
+    hipLaunchKernelGGL(k_sum_tiles, dim3(blocks_for(F, 64)), dim3(64), 0, stream, part, tiles, F, db);
     return (int)hipGetLastError();
 }
 
@@ -472,12 +561,20 @@ int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream)
     return (int)hipGetLastError();
 }
 
-/* out[0] += x[0] + ... + x[n-1], in index order (one thread) */
-__global__ void k_sum_ordered(const float* __restrict__ x, int n, float* __restrict__ out) {
-    if (threadIdx.x != 0) return;
+/* out[0] += x[0] + ... + x[n-1] in one fixed order: lane l of a 64-thread
+ * workgroup sums x[l], x[l + 64], ... in turn, then the 64 lane sums are added
+ * by a pairwise tree in LDS. */
+__global__ __launch_bounds__(64) void k_sum_ordered(const float* __restrict__ x, int n, float* __restrict__ out) {
+    __shared__ float t[64];
     float s = 0.f;
-    for (int i = 0; i < n; i++) s += x[i];
-    out[0] += s;
+    for (int i = threadIdx.x; i < n; i += 64) s += x[i];
+    t[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 32; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) t[threadIdx.x] = t[threadIdx.x] + t[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] += t[0];
 }
 
 int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream) {

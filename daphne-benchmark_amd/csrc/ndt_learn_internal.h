@@ -56,9 +56,13 @@ int launch_bias_act(const float* P, int splits, int M, int N, const float* bias,
                     uint16_t* outT, hipStream_t stream);
 int launch_head_fwd(const uint16_t* H2, const uint16_t* W3, const float* b3, int B, int F2, float* Y,
                     const float* target, const float* weights, float* dY, float* loss_b, hipStream_t stream);
+/* sample tiles of the backward kernels; their partial-sum buffer `part` holds
+ * bwd_tiles(B) * ((NOUT + 1) * F2 + NOUT) floats (head) or bwd_tiles(B) * F (ReLU) */
+int bwd_tiles(int B);
 int launch_head_bwd(const float* dY, const uint16_t* H2, const uint16_t* W3, int B, int F2, float* dW3, float* db3,
-                    uint16_t* dZ2, uint16_t* dZ2T, float* db2, hipStream_t stream);
-int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, hipStream_t stream);
+                    uint16_t* dZ2, uint16_t* dZ2T, float* db2, float* part, hipStream_t stream);
+int launch_relu_bwd(const float* dH, const uint16_t* H, int B, int F, uint16_t* dZT, float* db, float* part,
+                    hipStream_t stream);
 int launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float lr, float beta1, float beta2, float eps,
                 float bc1, float bc2, float wd, uint16_t* wb, hipStream_t stream);
 int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream);
