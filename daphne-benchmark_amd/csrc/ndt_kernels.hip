@@ -1227,12 +1227,18 @@ constexpr int MAX_CANDIDATES = 27;
 /* One wave's pair table in LDS: candidate j of lane l that passes the prefilter
  * at pick[64 j + l] (its pair_add index; lanes interleaved so that the lanes of
  * one store hit distinct banks; 32 columns hold the 27 candidates plus the
- * branch-free prefilter's writes past the last one), and the wave's pairs in
- * evaluation order as positions in pick. */
+ * branch-free prefilter's writes past the last one), the wave's pairs in
+ * evaluation order as positions in pick, and the pairs carried over to the next
+ * round (fewer than 64: the point's transformed position, or its cell-relative
+ * position with float32 points, its sensor-frame position and the pair_add
+ * index). */
 constexpr int PICK_ROW = 32;
 struct WavePairs {
     uint32_t pick[WAVE * PICK_ROW];
     uint16_t order[WAVE * MAX_CANDIDATES];
+    double cx[3][WAVE];
+    float cp[3][WAVE];
+    uint32_t cidx[WAVE];
 };
 
 __device__ inline uint32_t bperm_u32(int src_byte, uint32_t v) {
@@ -1254,18 +1260,68 @@ __device__ inline double bperm_f64(int src_byte, double v) {
 #define HEAD_BATCH 8
 #endif
 
-/* The pairs of one point per lane, spread evenly over the wave.  Lane l holds its
- * point's lookup L and has written its cnt prefiltered candidates to its column
- * of wp.pick (cnt <= 27; 0 without a point).  The wave's pairs are numbered lane by lane,
- * in prefilter order within a lane, and pair n is evaluated by lane n mod 64 in
- * round n / 64.  A wave thus spends ceil(pairs / 64) rounds on the pair terms,
- * where a per-point loop spends as many as the busiest lane has pairs.  Each lane
- * adds the terms of the pairs it evaluates to its own accumulators; the assignment
- * depends only on the wave's points, so results are reproducible.  Every lane of
- * the wave must call this. */
+/* Inputs of pair n of the wave's current round.  Pairs 0 .. carry-1 are those
+ * carried over from the previous round (slot n of the carry table, read only when
+ * `read_carry`), pairs carry .. all-1 are this round's, in order, with their
+ * point data taken from the lane that looked the point up. */
+template <typename T>
+__device__ inline void pair_source(const WavePairs& wp, const uint32_t n, const uint32_t carry, const uint32_t all,
+                                   const bool read_carry, const float4 p, const PointLook& L, bool& valid,
+                                   double& x0, double& x1, double& x2, float& r0, float& r1, float& r2, float4& ps,
+                                   uint32_t& idx) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const bool fresh = n >= carry && n < all;
+    const uint32_t d = fresh ? (uint32_t)wp.order[n - carry] : lane;
+    idx = wp.pick[d];
+    const int src = (int)(d % WAVE) << 2;
+    ps = make_float4(bperm_f32(src, p.x), bperm_f32(src, p.y), bperm_f32(src, p.z), 0.f);
+    x0 = x1 = x2 = 0.0;
+    r0 = r1 = r2 = 0.f;
+    if constexpr (sizeof(T) == 4) {
+        r0 = bperm_f32(src, L.r0);
+        r1 = bperm_f32(src, L.r1);
+        r2 = bperm_f32(src, L.r2);
+    } else {
+        x0 = bperm_f64(src, L.x0);
+        x1 = bperm_f64(src, L.x1);
+        x2 = bperm_f64(src, L.x2);
+    }
+    if (read_carry) {
+        const uint32_t c = n % WAVE;
+        const bool held = n < carry;
+        ps.x = held ? wp.cp[0][c] : ps.x;
+        ps.y = held ? wp.cp[1][c] : ps.y;
+        ps.z = held ? wp.cp[2][c] : ps.z;
+        idx = held ? wp.cidx[c] : idx;
+        if constexpr (sizeof(T) == 4) {
+            r0 = held ? (float)wp.cx[0][c] : r0;
+            r1 = held ? (float)wp.cx[1][c] : r1;
+            r2 = held ? (float)wp.cx[2][c] : r2;
+        } else {
+            x0 = held ? wp.cx[0][c] : x0;
+            x1 = held ? wp.cx[1][c] : x1;
+            x2 = held ? wp.cx[2][c] : x2;
+        }
+    }
+    valid = n < all;
+}
+
+/* The pairs of one point per lane, spread evenly over the wave in full batches of
+ * 64.  Lane l holds its point's lookup L and has written its cnt prefiltered
+ * candidates to its column of wp.pick (cnt <= 27; 0 without a point).  The
+ * round's pairs are numbered after the `carry` pairs left from the previous
+ * round: lane by lane, in prefilter order within a lane.  Pair n is evaluated by
+ * lane n mod 64 in the round's batch n / 64.  Only full batches are evaluated;
+ * the rest (fewer than 64) is copied to the carry table and leads the next round,
+ * except in the wave's last round of the chunk (`last`), which evaluates
+ * everything.  A wave thus spends about pairs / 64 batches on the pair terms.
+ * Each lane adds the terms of the pairs it evaluates to its own accumulators; the
+ * assignment depends only on the wave's points, so results are reproducible.
+ * Every lane of the wave must call this. */
 template <typename T, int SUM>
 __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, const uint32_t cnt,
-                                 const PassParams<T>& u, Acc<T, SUM>& acc, WavePairs& wp) {
+                                 const PassParams<T>& u, Acc<T, SUM>& acc, WavePairs& wp, uint32_t& carry,
+                                 const bool last) {
     const uint32_t lane = threadIdx.x % WAVE;
     /* exclusive prefix of the per-lane counts (< 32) from five bit-plane ballots */
     uint32_t excl = 0, total = 0;
@@ -1279,27 +1335,49 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const uint32_t all = carry + total;
+    const uint32_t full = last ? all : all - all % WAVE;
     int pairs = 0;
-    for (uint32_t base = 0; base < total; base += WAVE) {
+    for (uint32_t base = 0; base < full; base += WAVE) {
         const uint32_t n = base + lane;
-        const uint32_t d = (n < total) ? (uint32_t)wp.order[n] : lane;
-        const uint32_t idx = wp.pick[d];
-        const int src = (int)(d % WAVE) << 2;
-        const float4 ps = make_float4(bperm_f32(src, p.x), bperm_f32(src, p.y), bperm_f32(src, p.z), 0.f);
-        double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-        float r0 = 0.f, r1 = 0.f, r2 = 0.f;
-        if constexpr (sizeof(T) == 4) {
-            r0 = bperm_f32(src, L.r0);
-            r1 = bperm_f32(src, L.r1);
-            r2 = bperm_f32(src, L.r2);
-        } else {
-            x0 = bperm_f64(src, L.x0);
-            x1 = bperm_f64(src, L.x1);
-            x2 = bperm_f64(src, L.x2);
-        }
-        pairs += pair_add<T, SUM>(g, n < total, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
+        bool valid;
+        double x0, x1, x2;
+        float r0, r1, r2;
+        float4 ps;
+        uint32_t idx;
+        pair_source<T>(wp, n, carry, all, base < carry, p, L, valid, x0, x1, x2, r0, r1, r2, ps, idx);
+        pairs += pair_add<T, SUM>(g, valid, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
     }
-    /* the next round rewrites the table: its reads stay before those writes */
+    if (full < all) {
+        /* the remainder moves to carry slots 0 ..: pair full + j to slot j.  When no
+         * batch ran (full = 0) the carried pairs stay in their slots; otherwise
+         * full >= 64 > carry and every remaining pair is new.  Slot j is only ever
+         * written and read by lane j. */
+        const uint32_t n = full + lane;
+        bool valid;
+        double x0, x1, x2;
+        float r0, r1, r2;
+        float4 ps;
+        uint32_t idx;
+        pair_source<T>(wp, n, carry, all, false, p, L, valid, x0, x1, x2, r0, r1, r2, ps, idx);
+        if (valid && n >= carry) {
+            if constexpr (sizeof(T) == 4) {
+                wp.cx[0][lane] = (double)r0;
+                wp.cx[1][lane] = (double)r1;
+                wp.cx[2][lane] = (double)r2;
+            } else {
+                wp.cx[0][lane] = x0;
+                wp.cx[1][lane] = x1;
+                wp.cx[2][lane] = x2;
+            }
+            wp.cp[0][lane] = ps.x;
+            wp.cp[1][lane] = ps.y;
+            wp.cp[2][lane] = ps.z;
+            wp.cidx[lane] = idx;
+        }
+    }
+    carry = all - full;
+    /* the next round rewrites the tables: its writes stay after these reads */
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -1323,7 +1401,8 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
  * accumulators; returns this thread's pair count.  With a dense index each wave
  * takes 64 consecutive points per round (wave-uniform trip count), the next
  * round's point load and index lookup are in flight during this round's pairs, and
- * the pairs are spread over the wave (wave_pairs).  With the hashed index each
+ * the pairs are spread over the wave in full batches of 64, a round's remainder
+ * leading the next round (wave_pairs).  With the hashed index each
  * thread handles its own points.  Both schedulers run their chunks through this,
  * so their partial sums are bit-identical. */
 template <typename T, int SUM, int BLOCK>
@@ -1342,6 +1421,7 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
         PointLook lc;
         uint2 ec;
         lookup_issue(g, pc, vc, u.R, u.t, lc, ec);
+        uint32_t carry = 0;
         for (int64_t ib = wb; ib < pe; ib += BLOCK) {
             const bool vn = ib + BLOCK + lane < pe;
             const float4 pn = pts[min(ib + BLOCK + lane, pe - 1)];
@@ -1372,7 +1452,7 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
             PointLook ln;
             uint2 en;
             lookup_issue(g, pn, vn, u.R, u.t, ln, en);
-            pairs += wave_pairs<T, SUM>(g, pc, lc, cnt, u, acc, wp);
+            pairs += wave_pairs<T, SUM>(g, pc, lc, cnt, u, acc, wp, carry, ib + BLOCK >= pe);
             pc = pn;
             lc = ln;
             ec = en;
