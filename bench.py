@@ -117,6 +117,73 @@ def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, seed=5):
                        train_s=t_train, loss_first=float(losses[0]), loss_last=float(losses[-1]))
 
 
+def _harness_time(exe, cwd, args=(), env=None):
+    """Runs a DAPHNE testcase binary; returns (seconds per testcase, result ok)."""
+    import subprocess
+    p = subprocess.run([exe, *args], cwd=cwd, capture_output=True, text=True, timeout=600, env=env)
+    per = None
+    for line in p.stdout.splitlines():
+        if line.startswith("elapsed time:") and "):" in line:
+            per = float(line.rsplit("):", 1)[1].split()[0])
+    return per, (p.returncode == 0 and "result ok" in p.stdout)
+
+
+def daphne_harness_leg(cases_n: int) -> dict:
+    """The reference's own benchmark harness and the drop-in on one DAPHNE testcase
+    file (cfg0-sized testcases: 100 k-point map, 2 k-point scan, guess within
+    0.5 m / 2 deg).  The compiled reference (oracle/_ref: its single-threaded CPU
+    build and its OpenMP build) and daphne-benchmark_amd/ndt_mapping_hip read the
+    same ndt_input.dat and check the same ndt_output.dat, written by the reference's
+    datagen build; each prints its kernel time per testcase (grid build and
+    alignment, compat semantics)."""
+    import shutil
+    import subprocess
+    import tempfile
+    ref = os.path.join(REPO, "oracle", "_ref")
+    exe = {k: os.path.join(ref, f) for k, f in (("datagen", "ndt_ref_datagen"), ("cpu", "ndt_ref_cpu"),
+                                                 ("omp", "ndt_ref_omp"))}
+    hip = os.path.join(REPO, "daphne-benchmark_amd", "ndt_mapping_hip")
+    missing = [v for v in list(exe.values()) + [hip] if not os.path.exists(v)]
+    if missing:
+        return {"skipped": "not built: " + ", ".join(os.path.relpath(m, REPO) for m in missing)}
+    scene = A.scene_cfg(3, extent=100.0)
+    mp = A.make_map(scene, 100_000)
+    cases = []
+    for k in range(cases_n):
+        scan, _gt, guess = A.make_scan(scene, mp, A.scan_cfg(2000, 0.5, 2.0), 500 + k)
+        cases.append(A.Testcase(guess.astype(np.float32), scan, mp))
+    d = tempfile.mkdtemp()
+    try:
+        data, run = os.path.join(d, "data"), os.path.join(d, "a", "b", "c")
+        os.makedirs(data)
+        os.makedirs(run)
+        A.write_ndt_input(os.path.join(data, "ndt_input.dat"), cases)
+        # the datagen build compares against ndt_output.dat while writing dategen.dat
+        A.write_ndt_output(os.path.join(data, "ndt_output.dat"),
+                           [A.CallbackResult(True, np.eye(4, dtype=np.float32))] * cases_n)
+        subprocess.run([exe["datagen"]], cwd=run, capture_output=True, timeout=600)
+        res = A.read_ndt_output(os.path.join(run, "dategen.dat"))
+        for r in res:
+            r.fitness_score = 0.0  # never set by the reference
+        A.write_ndt_output(os.path.join(data, "ndt_output.dat"), res)
+        threads = os.cpu_count() or 1
+        out = {"testcases": cases_n, "map_points": int(mp.shape[0]), "scan_points": 2000,
+               "unit": "seconds per testcase (the harness's own timer)"}
+        out["reference_cpu"], ok_cpu = _harness_time(exe["cpu"], run)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        out["reference_openmp"], ok_omp = _harness_time(exe["omp"], run, env=env)
+        out["reference_openmp_threads"] = threads
+        _harness_time(hip, run)  # warm-up: device initialisation and code loading
+        out["hip_p1"], ok_h1 = _harness_time(hip, run)
+        out["hip_p8"], ok_h8 = _harness_time(hip, run, ("-p", "8"))
+        out["result_ok"] = {"reference_cpu": ok_cpu, "reference_openmp": ok_omp, "hip_p1": ok_h1, "hip_p8": ok_h8}
+        if out["hip_p8"] and out["reference_openmp"]:
+            out["speedup_p8_vs_openmp"] = out["reference_openmp"] / out["hip_p8"]
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +194,8 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--plain-scans", type=int, default=128, help="scans timed one at a time (subset of the batch)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="scans aligned by the CPU oracle")
+    ap.add_argument("--harness-cases", type=int, default=24,
+                    help="DAPHNE testcases for the harness comparison with the compiled reference (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="batched NDT only (no pose proposer)")
     ap.add_argument("--train-scans", type=int, default=25000)
@@ -337,6 +406,11 @@ def main():
             "oracle_reproducible_scans": int(errs[:, 3].sum()),
             "within_1cm_0.05deg_on_reproducible": int(np.sum(ok & (errs[:, 3] > 0))),
         }
+        if args.harness_cases > 0:
+            try:
+                out["daphne_harness"] = daphne_harness_leg(args.harness_cases)
+            except Exception as e:  # reported, never fatal to the bench line
+                out["daphne_harness"] = {"error": f"{type(e).__name__}: {e}"}
     print(json.dumps(out))
 
 

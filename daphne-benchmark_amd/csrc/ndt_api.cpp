@@ -127,7 +127,8 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
         return fail(NDT_ERR_UNSUPPORTED, "device lacks cooperative launch");
     }
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
-        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+        (e = hipHostMalloc(&c->hsmall, HSMALL_BYTES, hipHostMallocDefault)) != hipSuccess) {
         delete c;
         return fail(NDT_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
     }
@@ -144,12 +145,20 @@ void ndt::ctx_release(ndt_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->pin) (void)hipHostFree(c->pin);
+    if (c->hsmall) (void)hipHostFree(c->hsmall);
     delete c;
     (void)hipGetLastError();
 }
 
 extern "C" int ndt_ctx_destroy(ndt_ctx* c) {
     if (!c) return 0;
+    if (c->compat_grid) {
+        /* the cached grid holds a reference to its context */
+        ndt_grid* g = c->compat_grid;
+        c->compat_grid = nullptr;
+        ndt_grid_destroy(g);
+    }
     ctx_release(c);
     return 0;
 }
@@ -197,9 +206,9 @@ static int sorted_runs(ndt_ctx* c, const uint64_t* keys, int64_t n, int shift, u
     LAUNCH_TRY(launch_run_heads(keys, n, shift, limit, flags.as<uint32_t>(), c->stream));
     uint32_t* total = (uint32_t*)((char*)scratch.p + ((scan_scratch_bytes(n) + 255) & ~(size_t)255));
     LAUNCH_TRY(exclusive_scan_u32(flags.as<uint32_t>(), ranks.as<uint32_t>(), n, total, scratch.p, c->stream));
-    uint32_t h_total = 0;
-    HIP_TRY(hipMemcpyAsync(&h_total, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->hsmall, total, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    const uint32_t h_total = *(const uint32_t*)c->hsmall;
     BUF_TRY(starts, sizeof(uint32_t) * (h_total + 1));
     LAUNCH_TRY(launch_run_starts(flags.as<uint32_t>(), ranks.as<uint32_t>(), n, starts.as<uint32_t>(), c->stream));
     *nruns = h_total;
@@ -381,6 +390,42 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     return 0;
 }
 
+/* Host-to-device copy through the context's pinned staging buffer (grown in
+ * powers of two, at least 4 MiB): a pageable hipMemcpyAsync is staged by the
+ * runtime through buffers shared by every stream, which serialises contexts that
+ * upload concurrently.  The staging buffer is refilled only after the previous
+ * copy from it has completed. */
+static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (c->pin_pending) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->pin_pending = false;
+    }
+    if (!c->pin || c->pin_bytes < bytes) {
+        if (c->pin) {
+            (void)hipHostFree(c->pin);
+            c->pin = nullptr;
+            c->pin_bytes = 0;
+        }
+        size_t want = (size_t)4 << 20;
+        while (want < bytes) want <<= 1;
+        hipError_t e = hipHostMalloc(&c->pin, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->pin = nullptr;
+            return fail(NDT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        c->pin_bytes = want;
+    }
+    std::memcpy(c->pin, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, c->pin, bytes, hipMemcpyHostToDevice, c->stream));
+    c->pin_pending = true;
+    return 0;
+}
+
+static uint64_t next_grid_serial() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
 static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
     /* initCompute, src/CPU/ndt_mapping/kernel.cpp:1302-1373 */
     for (int64_t i = 0; i < m; i++)
@@ -388,15 +433,19 @@ static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* 
             return fail(NDT_ERR_INVALID, "compat mode needs finite map points");
     const float res = g->params.resolution;
     BUF_TRY(c->pts, sizeof(float4) * m);
-    HIP_TRY(hipMemcpyAsync(c->pts.p, map, sizeof(float4) * m, hipMemcpyHostToDevice, c->stream));
-    DevBuf mm;
+    int e = upload_staged(c, c->pts.p, map, sizeof(float4) * m);
+    if (e) return e;
+    DevBuf& mm = c->bk_mm;
     BUF_TRY(mm, 64);
-    uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    HIP_TRY(hipMemcpyAsync(mm.p, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
+    uint32_t* hs = (uint32_t*)c->hsmall;
+    const uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+    std::memcpy(hs, init, sizeof(init));
+    HIP_TRY(hipMemcpyAsync(mm.p, hs, sizeof(init), hipMemcpyHostToDevice, c->stream));
     LAUNCH_TRY(launch_bbox(c->pts.as<float4>(), m, mm.as<uint32_t>(), c->stream));
-    uint32_t h_mm[6];
-    HIP_TRY(hipMemcpyAsync(h_mm, mm.p, sizeof(h_mm), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(hs + 16, mm.p, 6 * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    uint32_t h_mm[6];
+    std::memcpy(h_mm, hs + 16, sizeof(h_mm));
     CompatGridDev& d = g->cdev;
     int64_t ncells = 1;
     for (int k = 0; k < 3; k++) {
@@ -416,7 +465,9 @@ static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* 
         g->info.min_corner[k] = d.minv[k];
         g->info.max_corner[k] = d.maxv[k];
     }
-    DevBuf keys, vals, keys_t, vals_t, scratch, flags, ranks, starts, sc2;
+    DevBuf &keys = c->bk_keys, &vals = c->bk_vals, &keys_t = c->bk_keys_t, &vals_t = c->bk_vals_t;
+    DevBuf &scratch = c->bk_scratch, &flags = c->bk_flags, &ranks = c->bk_ranks, &starts = c->bk_starts;
+    DevBuf& sc2 = c->bk_sc2;
     BUF_TRY(keys, sizeof(uint64_t) * m);
     BUF_TRY(vals, sizeof(uint32_t) * m);
     BUF_TRY(keys_t, sizeof(uint64_t) * m);
@@ -427,7 +478,7 @@ static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* 
     LAUNCH_TRY(radix_sort_pairs(keys.as<uint64_t>(), vals.as<uint32_t>(), keys_t.as<uint64_t>(), vals_t.as<uint32_t>(), m,
                                 std::max(1, ceil_log2(ncells + 1)), scratch.p, c->stream));
     int64_t nvox = 0;
-    int e = sorted_runs(c, keys.as<uint64_t>(), m, 0, (uint64_t)ncells, flags, ranks, starts, sc2, &nvox);
+    e = sorted_runs(c, keys.as<uint64_t>(), m, 0, (uint64_t)ncells, flags, ranks, starts, sc2, &nvox);
     if (e) return e;
     BUF_TRY(g->all_mean, sizeof(double) * 3 * nvox);
     BUF_TRY(g->all_icov, sizeof(double) * 9 * nvox);
@@ -463,8 +514,7 @@ extern "C" int ndt_grid_build(ndt_ctx* c, const float* map, int64_t m, const ndt
     if (e) return e;
     HIP_TRY(hipSetDevice(c->device));
     ndt_grid* g = new (std::nothrow) ndt_grid();
-    static std::atomic<uint64_t> next_serial{1};
-    if (g) g->serial = next_serial.fetch_add(1);
+    if (g) g->serial = next_grid_serial();
     if (!g) return fail(NDT_ERR_NOMEM, "host allocation failed");
     g->ctx = c;
     g->params = *params;
@@ -1189,13 +1239,16 @@ static int compat_pass(ndt_ctx* c, const ndt_grid* g, const float4* scan, int64_
         for (int q = 0; q < 3; q++) jh[k * 3 + q] = jd[k][q];
     for (int k = 0; k < 15; k++)
         for (int q = 0; q < 3; q++) jh[(8 + k) * 3 + q] = hd[k][q];
-    const int nblocks = (int)std::min<int64_t>(512, (n + 255) / 256);
+    /* sixteen threads per point (k_deriv_compat) */
+    const int nblocks = (int)std::min<int64_t>(512, (16 * n + 255) / 256);
     BUF_TRY(c->partials2, sizeof(double) * NRED_COMPAT * (nblocks + 1));
     double* red_dev = c->partials2.as<double>() + (size_t)NRED_COMPAT * nblocks;
     LAUNCH_TRY(launch_deriv_compat(g->cdev, scan, n, M, jh, d1, d2, c->partials2.as<double>(), nblocks, c->stream));
     LAUNCH_TRY(launch_reduce_partials(c->partials2.as<double>(), nblocks, NRED_COMPAT, red_dev, c->stream));
-    HIP_TRY(hipMemcpyAsync(red, red_dev, sizeof(double) * NRED_COMPAT, hipMemcpyDeviceToHost, c->stream));
+    static_assert(sizeof(double) * NRED_COMPAT <= HSMALL_BYTES, "pinned readback buffer too small");
+    HIP_TRY(hipMemcpyAsync(c->hsmall, red_dev, sizeof(double) * NRED_COMPAT, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    std::memcpy(red, c->hsmall, sizeof(double) * NRED_COMPAT);
     return 0;
 }
 
@@ -1267,16 +1320,26 @@ extern "C" int ndt_compat_callback(ndt_ctx* c, const float* scan, int64_t n, con
     HIP_TRY(hipSetDevice(c->device));
     ndt_params prm;
     ndt_params_default(&prm, NDT_MODE_COMPAT);
-    ndt_grid* g = nullptr;
-    int e = ndt_grid_build(c, map, m, &prm, &g);
+    if (m > 0x7fffffffll) return fail(NDT_ERR_UNSUPPORTED, "map larger than 2^31 points");
+    /* the context's compat grid is rebuilt in place: its buffers only grow */
+    ndt_grid* g = c->compat_grid;
+    if (!g) {
+        g = new (std::nothrow) ndt_grid();
+        if (!g) return fail(NDT_ERR_NOMEM, "host allocation failed");
+        g->ctx = c;
+        c->compat_grid = g;
+    }
+    g->serial = next_grid_serial();
+    g->params = prm;
+    std::memset(&g->info, 0, sizeof(g->info));
+    std::memset(&g->dev, 0, sizeof(g->dev));
+    std::memset(&g->cdev, 0, sizeof(g->cdev));
+    int e = grid_build_compat(c, map, m, g);
     if (e) return e;
-    struct GridGuard {
-        ndt_grid* g;
-        ~GridGuard() { ndt_grid_destroy(g); }
-    } guard{g};
-    DevBuf scan_dev;
+    DevBuf& scan_dev = c->compat_scan;
     BUF_TRY(scan_dev, sizeof(float4) * n);
-    HIP_TRY(hipMemcpyAsync(scan_dev.p, scan, sizeof(float4) * n, hipMemcpyHostToDevice, c->stream));
+    e = upload_staged(c, scan_dev.p, scan, sizeof(float4) * n);
+    if (e) return e;
     double d1, d2;
     gauss_consts((double)prm.resolution, prm.outlier_ratio, &d1, &d2);
     float final_m[16];

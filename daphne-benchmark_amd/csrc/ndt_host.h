@@ -130,6 +130,9 @@ struct CtxRef {
 };
 } // namespace ndt
 
+/* bytes of a context's small pinned readback buffer */
+constexpr size_t HSMALL_BYTES = 4096;
+
 /* opaque handles of the C ABI */
 struct ndt_ctx {
     std::atomic<int> refs{1};
@@ -140,6 +143,21 @@ struct ndt_ctx {
     char name[256] = {0};
     /* workspaces */
     DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2, queue, items, done;
+    /* grid-build temporaries, and the compat callback's grid and scan, kept for
+     * reuse: freeing device memory synchronises the device, which would serialise
+     * contexts that work concurrently (one per host thread) */
+    DevBuf bk_keys, bk_vals, bk_keys_t, bk_vals_t, bk_scratch, bk_flags, bk_ranks, bk_starts, bk_sc2, bk_mm;
+    DevBuf compat_scan;
+    ndt_grid* compat_grid = nullptr;
+    /* pinned staging buffer for host-to-device uploads; pin_pending while a copy
+     * from it may still be in flight on the stream */
+    void* pin = nullptr;
+    size_t pin_bytes = 0;
+    bool pin_pending = false;
+    /* small pinned buffer for scalars read back by host-side control (bounding
+     * box, run counts, a compat pass's sums); each use ends with a stream
+     * synchronisation before the next */
+    void* hsmall = nullptr;
     double last_ms = 0.0;
     int32_t last_launches = 0;
     int64_t last_pairs = 0;

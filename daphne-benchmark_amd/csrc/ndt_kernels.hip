@@ -54,17 +54,36 @@ static inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) 
 
 /* ------------------------------------------------------------------ bbox */
 
-__global__ void k_bbox(const float4* __restrict__ pts, int64_t n, uint32_t* __restrict__ mm) {
+/* Bounding box of the finite points as order-preserving keys (mm[0..2] minima,
+ * mm[3..5] maxima, merged into what mm holds).  Each thread keeps four loads in
+ * flight; a workgroup reduces over its waves in LDS and merges once, so the
+ * atomics on the six words number six per workgroup (at most 256 workgroups). */
+constexpr int BBOX_BLOCK = 512;
+
+__global__ __launch_bounds__(BBOX_BLOCK) void k_bbox(const float4* __restrict__ pts, int64_t n,
+                                                      uint32_t* __restrict__ mm) {
+    __shared__ uint32_t red[BBOX_BLOCK / WAVE][6];
     uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float4 p = pts[i];
-        if (!isfinite(p.x) || !isfinite(p.y) || !isfinite(p.z)) continue;
-        uint32_t o[3] = {f2ord(p.x), f2ord(p.y), f2ord(p.z)};
-        for (int k = 0; k < 3; k++) {
-            lo[k] = o[k] < lo[k] ? o[k] : lo[k];
-            hi[k] = o[k] > hi[k] ? o[k] : hi[k];
+    const int64_t stride = (int64_t)gridDim.x * BBOX_BLOCK;
+    for (int64_t i0 = blockIdx.x * (int64_t)BBOX_BLOCK + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        float4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t i = i0 + u * stride;
+            p[u] = i < n ? pts[i] : make_float4(NAN, NAN, NAN, NAN);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (!isfinite(p[u].x) || !isfinite(p[u].y) || !isfinite(p[u].z)) continue;
+            const uint32_t o[3] = {f2ord(p[u].x), f2ord(p[u].y), f2ord(p[u].z)};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                lo[k] = o[k] < lo[k] ? o[k] : lo[k];
+                hi[k] = o[k] > hi[k] ? o[k] : hi[k];
+            }
         }
     }
+#pragma unroll
     for (int k = 0; k < 3; k++) {
         for (int off = WAVE / 2; off > 0; off >>= 1) {
             uint32_t a = __shfl_xor(lo[k], off, WAVE), b = __shfl_xor(hi[k], off, WAVE);
@@ -72,19 +91,28 @@ __global__ void k_bbox(const float4* __restrict__ pts, int64_t n, uint32_t* __re
             hi[k] = b > hi[k] ? b : hi[k];
         }
     }
+    const int wave = threadIdx.x / WAVE;
     if ((threadIdx.x & (WAVE - 1)) == 0) {
         for (int k = 0; k < 3; k++) {
-            atomicMin(&mm[k], lo[k]);
-            atomicMax(&mm[3 + k], hi[k]);
+            red[wave][k] = lo[k];
+            red[wave][3 + k] = hi[k];
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int k = threadIdx.x;
+        uint32_t v = red[0][k];
+        for (int w = 1; w < BBOX_BLOCK / WAVE; w++) v = k < 3 ? min(v, red[w][k]) : max(v, red[w][k]);
+        if (k < 3) atomicMin(&mm[k], v);
+        else atomicMax(&mm[k], v);
     }
 }
 
 int launch_bbox(const float4* pts, int64_t n, uint32_t* mm, hipStream_t stream) {
-    int blocks = grid_for(n, 256);
-    if (blocks > 2048) blocks = 2048;
+    int64_t blocks = (n + 4 * (int64_t)BBOX_BLOCK - 1) / (4 * (int64_t)BBOX_BLOCK);
+    if (blocks > 256) blocks = 256;
     if (blocks < 1) blocks = 1;
-    hipLaunchKernelGGL(k_bbox, dim3(blocks), dim3(256), 0, stream, pts, n, mm);
+    hipLaunchKernelGGL(k_bbox, dim3((unsigned)blocks), dim3(BBOX_BLOCK), 0, stream, pts, n, mm);
     return (int)hipGetLastError();
 }
 
@@ -2409,7 +2437,11 @@ __global__ __launch_bounds__(256) void k_deriv_compat(CompatGridDev g, const flo
     /* which angular second-derivative vector sits in block (i, j), i, j >= 3
      * (point_hessian_ layout, kernel.cpp:528-558): a b c / b d e / c e f */
     const int hsel[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    for (int64_t idx = blockIdx.x * 256ll + threadIdx.x; idx < n; idx += (int64_t)gridDim.x * 256) {
+    /* sixteen threads per point, one per (z, y) probe row of its search (at most
+     * four steps per axis), so that a small scan still spreads over the GPU */
+    for (int64_t t = blockIdx.x * 256ll + threadIdx.x; t < 16 * n; t += (int64_t)gridDim.x * 256) {
+        const int64_t idx = t >> 4;
+        const int kzs = (int)(t >> 2) & 3, kys = (int)t & 3;
         const float4 p4 = scan[idx];
         const float in[3] = {p4.x, p4.y, p4.z};
         float pt[3];
@@ -2440,11 +2472,23 @@ __global__ __launch_bounds__(256) void k_deriv_compat(CompatGridDev g, const flo
         const float radiusFinal = (float)(radius + 0.001f);
         /* the probe loops step in float like the reference; the step counters only
          * bound them for coordinates so large that adding the resolution no longer
-         * changes the float (where the reference would not terminate) */
-        int kz = 0;
-        for (float z = (float)(pt[2] - radius); z <= pt[2] + radiusFinal && kz < 4; z += g.resolution, kz++) {
-            int ky = 0;
-            for (float y = (float)(pt[1] - radius); y <= pt[1] + radiusFinal && ky < 4; y += g.resolution, ky++) {
+         * changes the float (where the reference would not terminate).  This thread
+         * takes step kzs of the z loop and step kys of the y loop, reached by the
+         * same sequence of float additions, if the loops get that far. */
+        float z = (float)(pt[2] - radius);
+        bool zin = z <= pt[2] + radiusFinal;
+        for (int k = 0; k < kzs && zin; k++) {
+            z += g.resolution;
+            zin = z <= pt[2] + radiusFinal;
+        }
+        float y = (float)(pt[1] - radius);
+        bool yin = y <= pt[1] + radiusFinal;
+        for (int k = 0; k < kys && yin; k++) {
+            y += g.resolution;
+            yin = y <= pt[1] + radiusFinal;
+        }
+        if (zin && yin) {
+            {
                 int kx = 0;
                 for (float xx = (float)(pt[0] - radius); xx <= pt[0] + radiusFinal && kx < 4; xx += g.resolution, kx++) {
                     if ((xx < g.minv[0]) || (xx > g.maxv[0]) || (y < g.minv[1]) || (y > g.maxv[1]) || (z < g.minv[2]) ||
