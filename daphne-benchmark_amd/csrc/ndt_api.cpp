@@ -233,11 +233,46 @@ static int count_below(ndt_ctx* c, const uint64_t* keys, int64_t n, uint64_t lim
 
 /* ------------------------------------------------------------------ grid build */
 
+/* Host-to-device copy through the context's pinned staging buffer (grown in
+ * powers of two, at least 4 MiB): a pageable hipMemcpyAsync is staged by the
+ * runtime through buffers shared by every stream, which serialises contexts that
+ * upload concurrently.  The staging buffer is refilled only after the previous
+ * copy from it has completed. */
+static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (c->pin_pending) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->pin_pending = false;
+    }
+    if (!c->pin || c->pin_bytes < bytes) {
+        if (c->pin) {
+            (void)hipHostFree(c->pin);
+            c->pin = nullptr;
+            c->pin_bytes = 0;
+        }
+        size_t want = (size_t)4 << 20;
+        while (want < bytes) want <<= 1;
+        hipError_t e = hipHostMalloc(&c->pin, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->pin = nullptr;
+            return fail(NDT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+        }
+        c->pin_bytes = want;
+    }
+    std::memcpy(c->pin, src, bytes);
+    HIP_TRY(hipMemcpyAsync(dst, c->pin, bytes, hipMemcpyHostToDevice, c->stream));
+    c->pin_pending = true;
+    return 0;
+}
+
+
 static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
     const double res = (double)g->params.resolution;
     BUF_TRY(c->pts, sizeof(float4) * m);
-    HIP_TRY(hipMemcpyAsync(c->pts.p, map, sizeof(float4) * m, hipMemcpyHostToDevice, c->stream));
-    DevBuf mm;
+    {
+        const int eu = upload_staged(c, c->pts.p, map, sizeof(float4) * m);
+        if (eu) return eu;
+    }
+    DevBuf& mm = c->bk_mm;
     BUF_TRY(mm, 64);
     uint32_t init[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
     HIP_TRY(hipMemcpyAsync(mm.p, init, sizeof(init), hipMemcpyHostToDevice, c->stream));
@@ -256,7 +291,9 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     }
     const int B = g->bits[0] + g->bits[1] + g->bits[2];
     if (B + 5 > 62) return fail(NDT_ERR_UNSUPPORTED, "map extent too large for the cell key");
-    DevBuf keys, vals, keys_t, vals_t, scratch, flags, ranks, starts, sc2;
+    DevBuf &keys = c->bk_keys, &vals = c->bk_vals, &keys_t = c->bk_keys_t, &vals_t = c->bk_vals_t;
+    DevBuf &scratch = c->bk_scratch, &flags = c->bk_flags, &ranks = c->bk_ranks, &starts = c->bk_starts;
+    DevBuf& sc2 = c->bk_sc2;
     BUF_TRY(keys, sizeof(uint64_t) * m);
     BUF_TRY(vals, sizeof(uint32_t) * m);
     BUF_TRY(keys_t, sizeof(uint64_t) * m);
@@ -387,37 +424,6 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     int ex = 0;
     d.res_pow2 = std::frexp(res, &ex) == 0.5 ? 1 : 0;
     d.inv_resolution_d = 1.0 / res;
-    return 0;
-}
-
-/* Host-to-device copy through the context's pinned staging buffer (grown in
- * powers of two, at least 4 MiB): a pageable hipMemcpyAsync is staged by the
- * runtime through buffers shared by every stream, which serialises contexts that
- * upload concurrently.  The staging buffer is refilled only after the previous
- * copy from it has completed. */
-static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
-    if (c->pin_pending) {
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        c->pin_pending = false;
-    }
-    if (!c->pin || c->pin_bytes < bytes) {
-        if (c->pin) {
-            (void)hipHostFree(c->pin);
-            c->pin = nullptr;
-            c->pin_bytes = 0;
-        }
-        size_t want = (size_t)4 << 20;
-        while (want < bytes) want <<= 1;
-        hipError_t e = hipHostMalloc(&c->pin, want, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            c->pin = nullptr;
-            return fail(NDT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-        }
-        c->pin_bytes = want;
-    }
-    std::memcpy(c->pin, src, bytes);
-    HIP_TRY(hipMemcpyAsync(dst, c->pin, bytes, hipMemcpyHostToDevice, c->stream));
-    c->pin_pending = true;
     return 0;
 }
 
