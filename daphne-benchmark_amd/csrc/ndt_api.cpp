@@ -662,22 +662,28 @@ static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int abo
 
 /* Chunk descriptors of a batch: chunk size, count and first partial slot per scan. */
 static void make_descs(const int64_t* offsets, const int64_t* counts, int32_t S, const ndt_params* p,
-                       std::vector<ScanDesc>& desc, int64_t* slots_out, int64_t* max_chunks_out) {
+                       std::vector<ScanDesc>& desc, int64_t* slots_out, int64_t* max_chunks_out,
+                       int64_t* first_pass_out) {
     desc.resize(S);
-    int64_t slots = 0, max_chunks = 0;
+    int64_t slots = 0, max_chunks = 0, first = 0;
     for (int32_t s = 0; s < S; s++) {
         const int64_t chunk = p->points_per_wg > 0 ? p->points_per_wg : align_chunk_points(counts[s]);
         desc[s].offset = offsets[s];
         desc[s].n = (int32_t)counts[s];
         desc[s].chunk = (int32_t)chunk;
-        desc[s].pad = 0;
         desc[s].nchunks = (int32_t)((counts[s] + chunk - 1) / chunk);
+        const int64_t fine = fine_chunk((int32_t)chunk);
+        desc[s].fnchunks = (int32_t)((counts[s] + fine - 1) / fine);
         desc[s].chunk_base = slots;
-        slots += desc[s].nchunks;
-        max_chunks = std::max<int64_t>(max_chunks, desc[s].nchunks);
+        desc[s].qbase = first;
+        /* slots for the finer of the two chunkings */
+        slots += desc[s].fnchunks;
+        first += desc[s].nchunks;
+        max_chunks = std::max<int64_t>(max_chunks, desc[s].fnchunks);
     }
     *slots_out = slots;
     *max_chunks_out = max_chunks;
+    *first_pass_out = first;
 }
 
 /* Work-queue items needed by a launch: every scan's chunks for each of its
@@ -704,7 +710,8 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     AlignParams prm = make_align_params(p, S);
     std::vector<ScanDesc> desc;
     int64_t slots = 0, max_chunks = 0;
-    make_descs(offsets, counts, S, p, desc, &slots, &max_chunks);
+    int64_t first_pass = 0;
+    make_descs(offsets, counts, S, p, desc, &slots, &max_chunks, &first_pass);
     const uint64_t cap = queue_capacity(slots, p->max_iterations);
     const bool queue = p->scheduler != NDT_SCHED_ROUNDS && cap <= MAX_QUEUE_ITEMS && S < (1 << ITEM_SCAN_BITS) &&
                        max_chunks < (1 << ITEM_CHUNK_BITS);
@@ -736,7 +743,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         HIP_TRY(hipMemsetAsync(c->items.p, 0, sizeof(uint64_t) * cap, c->stream));
         LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), S, prm, c->states.as<ScanState>(), c->sync.as<RoundSync>(),
                                       c->stream));
-        LAUNCH_TRY(launch_init_queue(c->desc.as<ScanDesc>(), S, (uint64_t)slots, c->queue.as<WorkQueue>(),
+        LAUNCH_TRY(launch_init_queue(c->desc.as<ScanDesc>(), S, (uint64_t)first_pass, c->queue.as<WorkQueue>(),
                                      c->items.as<uint64_t>(), c->done.as<uint32_t>(), c->stream));
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
         LAUNCH_TRY(launch_align_wq(g->dev, pts, c->desc.as<ScanDesc>(), nwg, block, prm,
@@ -915,14 +922,14 @@ static bool same_align_params(const ndt_params& a, const ndt_params& b) {
 
 /* the captured launch sequence; the caller ends the capture whatever this returns */
 static int capture_body(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, GraphPlan& gp, const AlignParams& prm,
-                        int64_t slots, uint64_t cap, int32_t nwg, int block) {
+                        int64_t first_pass, uint64_t cap, int32_t nwg, int block) {
     const int32_t S = sc->S;
     HIP_TRY(hipMemcpyAsync(gp.guesses.p, gp.h_guess, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
     LAUNCH_TRY(launch_zero_u32(gp.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(gp.items.p, (int64_t)(2 * cap), c->stream));
     LAUNCH_TRY(launch_init_states(gp.guesses.as<double>(), S, prm, gp.states.as<ScanState>(), gp.sync.as<RoundSync>(),
                                   c->stream));
-    LAUNCH_TRY(launch_init_queue(gp.desc.as<ScanDesc>(), S, (uint64_t)slots, gp.queue.as<WorkQueue>(),
+    LAUNCH_TRY(launch_init_queue(gp.desc.as<ScanDesc>(), S, (uint64_t)first_pass, gp.queue.as<WorkQueue>(),
                                  gp.items.as<uint64_t>(), gp.done.as<uint32_t>(), c->stream));
     LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), gp.desc.as<ScanDesc>(), nwg, block, prm,
                                gp.states.as<ScanState>(), gp.queue.as<WorkQueue>(), gp.items.as<uint64_t>(), cap,
@@ -945,7 +952,8 @@ static int graph_prepare(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, const ndt
     const int32_t S = sc->S;
     std::vector<ScanDesc> desc;
     int64_t slots = 0, max_chunks = 0;
-    make_descs(sc->offsets.data(), sc->counts.data(), S, p, desc, &slots, &max_chunks);
+    int64_t first_pass = 0;
+    make_descs(sc->offsets.data(), sc->counts.data(), S, p, desc, &slots, &max_chunks, &first_pass);
     const uint64_t cap = queue_capacity(slots, p->max_iterations);
     if (p->scheduler == NDT_SCHED_ROUNDS || cap > MAX_QUEUE_ITEMS || S >= (1 << ITEM_SCAN_BITS) ||
         max_chunks >= (1 << ITEM_CHUNK_BITS))
@@ -972,7 +980,7 @@ static int graph_prepare(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, const ndt
     HIP_TRY(hipMemcpy(gp.desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice));
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    int e = capture_body(c, g, sc, gp, prm, slots, cap, nwg, block);
+    int e = capture_body(c, g, sc, gp, prm, first_pass, cap, nwg, block);
     hipGraph_t graph = nullptr;
     hipError_t he = hipStreamEndCapture(c->stream, &graph);
     if (e) {

@@ -305,8 +305,30 @@ struct ScanDesc {
     int32_t n;          /* points */
     int32_t nchunks;    /* chunks of `chunk` points */
     int32_t chunk;      /* points per partial sum */
-    int32_t pad;
+    int32_t fnchunks;   /* chunks of fine_chunk(chunk) points (late passes) */
+    int64_t qbase;      /* queue position of the scan's first-pass chunks */
 };
+
+/* Passes from this one on cut a scan into chunks a quarter the size (at least 256
+ * points): the scans that need many passes set a batch's finish time, and finer
+ * chunks spread each of their late passes over more workgroups (on the 512-scan,
+ * 1 M-point-map batch: 6.45 ms against 7.0 ms with one chunk size; passes from 20,
+ * 24, 32 or 36 on, or chunks an eighth or half the size, gave 6.5-7.0 ms).  The
+ * chunking is a function of the pass number only, so results stay independent of
+ * scheduling. */
+#ifndef NDT_FINE_PASS
+#define NDT_FINE_PASS 28
+#endif
+#ifndef NDT_FINE_DIV
+#define NDT_FINE_DIV 4
+#endif
+constexpr uint32_t FINE_PASS = NDT_FINE_PASS;
+NDT_HD int32_t fine_chunk(int32_t chunk) {
+    const int32_t q = chunk / NDT_FINE_DIV;
+    return q >= 256 ? q : (chunk < 256 ? chunk : 256);
+}
+NDT_HD int32_t pass_chunk(const ScanDesc& d, uint32_t pass) { return pass >= FINE_PASS ? fine_chunk(d.chunk) : d.chunk; }
+NDT_HD int32_t pass_nchunks(const ScanDesc& d, uint32_t pass) { return pass >= FINE_PASS ? d.fnchunks : d.nchunks; }
 
 struct AlignParams {
     double step_size, trans_eps, d1, d2;

@@ -1894,7 +1894,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         }
         __syncthreads();
         if (nactive == 0) break;
-        for (int a = tid; a < (int)nactive; a += BLOCK) s_prefix[a] = (uint32_t)scans[s_active[a]].nchunks;
+        for (int a = tid; a < (int)nactive; a += BLOCK) {
+            const int s = (int)s_active[a];
+            s_prefix[a] = (uint32_t)pass_nchunks(scans[s], (uint32_t)ld_agent_i32(&states[s].passes));
+        }
         __syncthreads();
         const uint32_t nchunks = lds_exclusive_scan<BLOCK>(s_prefix, (int)nactive, lw);
         if (tid == 0) s_prefix[nactive] = nchunks;
@@ -1927,8 +1930,9 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             int pairs = 0;
             {
                 const PassParams<T> u = load_uniform(pp);
-                const int64_t pb = sd.offset + k * sd.chunk;
-                const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * (int64_t)sd.chunk);
+                const int64_t cs = pass_chunk(sd, (uint32_t)ld_agent_i32(&states[s].passes));
+                const int64_t pb = sd.offset + k * cs;
+                const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * cs);
                 pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
             }
             double v[NRED];
@@ -1948,7 +1952,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const int s = (int)s_active[ai];
             const ScanDesc sd = scans[s];
             for (int q = tid; q < STATE_WORDS; q += BLOCK) sstate[q] = ld_agent_u64((const uint64_t*)(states + s) + q);
-            reduce_slots(partials, sd.chunk_base, sd.nchunks, gred, red);
+            __syncthreads();
+            reduce_slots(partials, sd.chunk_base, pass_nchunks(sd, (uint32_t)((ScanState*)sstate)->passes), gred, red);
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
             __syncthreads();
             for (int q = tid; q < STATE_WORDS; q += BLOCK) st_agent_u64((uint64_t*)(states + s) + q, sstate[q]);
@@ -2051,7 +2056,7 @@ __global__ void k_init_queue(const ScanDesc* __restrict__ scans, int32_t S, uint
     const int s = blockIdx.x;
     if (s >= S) return;
     const ScanDesc sd = scans[s];
-    for (int k = threadIdx.x; k < sd.nchunks; k += blockDim.x) items[sd.chunk_base + k] = make_item(0, s, k);
+    for (int k = threadIdx.x; k < sd.nchunks; k += blockDim.x) items[sd.qbase + k] = make_item(0, s, k);
     if (threadIdx.x == 0) done_cnt[s] = 0u;
     if (s == 0 && threadIdx.x == 0) {
         q->head = 0u;
@@ -2125,10 +2130,12 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         Acc<T, SUM> acc;
         acc.zero();
         int pairs = 0;
+        const int32_t nck = pass_nchunks(sd, pass);
         {
             const PassParams<T> u = load_uniform(pp);
-            const int64_t pb = sd.offset + (int64_t)k * sd.chunk;
-            const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * sd.chunk);
+            const int64_t cs = pass_chunk(sd, pass);
+            const int64_t pb = sd.offset + (int64_t)k * cs;
+            const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * cs);
             pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
         }
         double v[NRED];
@@ -2141,7 +2148,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         __syncthreads();
         if (tid == 0) {
             const uint32_t old = __hip_atomic_fetch_add(&done_cnt[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            s_last = (old + 1u == (uint32_t)sd.nchunks) ? 1 : 0;
+            s_last = (old + 1u == (uint32_t)nck) ? 1 : 0;
         }
         __syncthreads();
         const uint64_t t2 = __builtin_amdgcn_s_memrealtime();
@@ -2149,7 +2156,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             /* every chunk of this pass is in: reduce in chunk order, G interleaved groups */
             if (tid == 0) __hip_atomic_store(&done_cnt[s], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             for (int w = tid; w < STATE_WORDS; w += BLOCK) sstate[w] = ld_agent_u64((const uint64_t*)(states + s) + w);
-            reduce_slots(partials, sd.chunk_base, sd.nchunks, gred, red);
+            reduce_slots(partials, sd.chunk_base, nck, gred, red);
             if (tid == 0) {
                 ScanState& st = *(ScanState*)sstate;
                 ctl_step(st, red, prm, out + s);
@@ -2164,15 +2171,16 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             stores_complete();
             __syncthreads();
             if (!s_done) {
+                const int32_t nnext = pass_nchunks(sd, pass + 1);
                 if (tid == 0)
-                    s_base = __hip_atomic_fetch_add(&q->tail, (uint32_t)sd.nchunks, __ATOMIC_RELAXED,
+                    s_base = __hip_atomic_fetch_add(&q->tail, (uint32_t)nnext, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);
                 __syncthreads();
                 const uint64_t base = s_base;
-                if (base + (uint64_t)sd.nchunks > cap) {
+                if (base + (uint64_t)nnext > cap) {
                     if (tid == 0) __hip_atomic_store(&q->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 } else {
-                    for (int c = tid; c < sd.nchunks; c += BLOCK) st_agent_u64(items + base + c, make_item(pass + 1, s, c));
+                    for (int c = tid; c < nnext; c += BLOCK) st_agent_u64(items + base + c, make_item(pass + 1, s, c));
                 }
             } else if (tid == 0) {
                 __hip_atomic_fetch_sub(&q->remaining, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
