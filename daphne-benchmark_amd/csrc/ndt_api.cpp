@@ -675,8 +675,8 @@ static void make_descs(const int64_t* offsets, const int64_t* counts, int32_t S,
         const int64_t fine = fine_chunk((int32_t)chunk);
         desc[s].fnchunks = (int32_t)((counts[s] + fine - 1) / fine);
         desc[s].chunk_base = slots;
+        /* first-pass items are packed in scan order; slots cover the finer chunking */
         desc[s].qbase = first;
-        /* slots for the finer of the two chunkings */
         slots += desc[s].fnchunks;
         first += desc[s].nchunks;
         max_chunks = std::max<int64_t>(max_chunks, desc[s].fnchunks);
