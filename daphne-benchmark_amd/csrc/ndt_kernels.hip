@@ -2405,16 +2405,20 @@ int launch_deriv_ndt(NdtGridDev grid, const float4* pts, int64_t n, const double
     return (int)hipGetLastError();
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partials, int nblocks, int width, double* __restrict__ out) {
-    int k = threadIdx.x;
-    if (k >= width) return;
+/* out[k] = sum over b of partials[b][k], one wave per k in a fixed order: lane l
+ * adds b = l, l + 64, ... in turn, then the lanes are added by wave_sum. */
+__global__ __launch_bounds__(WAVE) void k_reduce_partials(const double* __restrict__ partials, int nblocks, int width,
+                                                          double* __restrict__ out) {
+    const int k = blockIdx.x;
     double s = 0.0;
-    for (int b = 0; b < nblocks; b++) s += partials[(int64_t)b * width + k];
-    out[k] = s;
+    for (int b = threadIdx.x; b < nblocks; b += WAVE) s += partials[(int64_t)b * width + k];
+    s = wave_sum(s);
+    if (threadIdx.x == 0) out[k] = s;
 }
 
 int launch_reduce_partials(const double* partials, int nblocks, int width, double* out, hipStream_t stream) {
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64), 0, stream, partials, nblocks, width, out);
+    if (width <= 0) return 0;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(width), dim3(WAVE), 0, stream, partials, nblocks, width, out);
     return (int)hipGetLastError();
 }
 
