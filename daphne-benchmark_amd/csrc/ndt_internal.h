@@ -313,7 +313,8 @@ struct ScanDesc {
  * points): the scans that need many passes set a batch's finish time, and finer
  * chunks spread each of their late passes over more workgroups (on the 512-scan,
  * 1 M-point-map batch: 6.45 ms against 7.0 ms with one chunk size; passes from 20,
- * 24, 32 or 36 on, or chunks an eighth or half the size, gave 6.5-7.0 ms).  The
+ * 24, 32 or 36 on, chunks an eighth or half the size, or a further level of
+ * sixteenth-size chunks from pass 36, 40 or 44 on, gave 6.45-7.0 ms).  The
  * chunking is a function of the pass number only, so results stay independent of
  * scheduling. */
 #ifndef NDT_FINE_PASS
