@@ -233,37 +233,55 @@ static int count_below(ndt_ctx* c, const uint64_t* keys, int64_t n, uint64_t lim
 
 /* ------------------------------------------------------------------ grid build */
 
-/* Host-to-device copy through the context's pinned staging buffer (grown in
- * powers of two, at least 4 MiB): a pageable hipMemcpyAsync is staged by the
- * runtime through buffers shared by every stream, which serialises contexts that
- * upload concurrently.  The staging buffer is refilled only after the previous
- * copy from it has completed. */
-static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
+/* The context's pinned staging buffer, grown (in powers of two, at least 4 MiB)
+ * to hold `bytes`.  Before it is replaced, the copies in flight from it complete. */
+static int pinned_reserve(ndt_ctx* c, size_t bytes) {
+    if (c->pin && c->pin_bytes >= bytes) return 0;
     if (c->pin_pending) {
         HIP_TRY(hipStreamSynchronize(c->stream));
         c->pin_pending = false;
     }
-    if (!c->pin || c->pin_bytes < bytes) {
-        if (c->pin) {
-            (void)hipHostFree(c->pin);
-            c->pin = nullptr;
-            c->pin_bytes = 0;
-        }
-        size_t want = (size_t)4 << 20;
-        while (want < bytes) want <<= 1;
-        hipError_t e = hipHostMalloc(&c->pin, want, hipHostMallocDefault);
-        if (e != hipSuccess) {
-            c->pin = nullptr;
-            return fail(NDT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
-        }
-        c->pin_bytes = want;
+    if (c->pin) {
+        (void)hipHostFree(c->pin);
+        c->pin = nullptr;
+        c->pin_bytes = 0;
     }
-    std::memcpy(c->pin, src, bytes);
-    HIP_TRY(hipMemcpyAsync(dst, c->pin, bytes, hipMemcpyHostToDevice, c->stream));
+    size_t want = (size_t)4 << 20;
+    while (want < bytes) want <<= 1;
+    hipError_t e = hipHostMalloc(&c->pin, want, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        c->pin = nullptr;
+        return fail(NDT_ERR_NOMEM, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    }
+    c->pin_bytes = want;
+    return 0;
+}
+
+/* Host-to-device copies (one or two) through the pinned staging buffer: a pageable
+ * hipMemcpyAsync is staged by the runtime through buffers shared by every stream,
+ * which serialises contexts that upload concurrently.  The staging buffer is
+ * refilled only after the previous copies from it have completed. */
+static int upload_staged2(ndt_ctx* c, void* d1, const void* s1, size_t b1, void* d2, const void* s2, size_t b2) {
+    if (c->pin_pending) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->pin_pending = false;
+    }
+    const size_t off2 = (b1 + 255) & ~(size_t)255;
+    int e = pinned_reserve(c, off2 + b2);
+    if (e) return e;
+    std::memcpy(c->pin, s1, b1);
+    HIP_TRY(hipMemcpyAsync(d1, c->pin, b1, hipMemcpyHostToDevice, c->stream));
+    if (b2) {
+        std::memcpy((char*)c->pin + off2, s2, b2);
+        HIP_TRY(hipMemcpyAsync(d2, (char*)c->pin + off2, b2, hipMemcpyHostToDevice, c->stream));
+    }
     c->pin_pending = true;
     return 0;
 }
 
+static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
+    return upload_staged2(c, dst, src, bytes, nullptr, nullptr, 0);
+}
 
 static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
     const double res = (double)g->params.resolution;
@@ -654,10 +672,15 @@ static int results_from_out(const ScanOut* out, const int64_t* counts, int32_t S
 
 /* Fills results from the device outputs of S scans in the context's buffers. */
 static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results) {
-    std::vector<ScanOut> out(S);
-    HIP_TRY(hipMemcpyAsync(out.data(), c->out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    /* read back through the pinned staging buffer (stream order puts this copy
+     * after the uploads from it) */
+    const size_t bytes = sizeof(ScanOut) * (size_t)S;
+    int e = pinned_reserve(c, bytes);
+    if (e) return e;
+    HIP_TRY(hipMemcpyAsync(c->pin, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return results_from_out(out.data(), counts, S, aborted, c, results);
+    c->pin_pending = false;
+    return results_from_out((const ScanOut*)c->pin, counts, S, aborted, c, results);
 }
 
 /* Chunk descriptors of a batch: chunk size, count and first partial slot per scan. */
@@ -730,8 +753,11 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     BUF_TRY(c->partials, sizeof(double) * 32 * slots);
     BUF_TRY(c->small, 64 * sizeof(double));
     HIP_TRY(hipMemsetAsync(c->out.p, 0, sizeof(ScanOut) * S, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->guesses.p, guesses, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
-    HIP_TRY(hipMemcpyAsync(c->desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice, c->stream));
+    {
+        const int eu = upload_staged2(c, c->guesses.p, guesses, sizeof(double) * 16 * S, c->desc.p, desc.data(),
+                                      sizeof(ScanDesc) * S);
+        if (eu) return eu;
+    }
     int32_t launches = 0;
     double prof_sum[4] = {0, 0, 0, 0};
     int aborted = 0;
