@@ -345,9 +345,35 @@ size_t scan_scratch_bytes(int64_t n) {
     return (size_t)(nb + 1) * sizeof(uint32_t);
 }
 
+/* Exclusive scan of a short array in one workgroup (in may equal out): thread t
+ * scans its contiguous run of ceil(n / 1024) values after a block scan of the
+ * run sums.  One launch instead of three for the radix sort's histograms. */
+constexpr int64_t SCAN_SMALL_MAX = 16384;
+__global__ __launch_bounds__(1024) void k_scan_small(const uint32_t* in, uint32_t* out, int64_t n,
+                                                     uint32_t* __restrict__ total) {
+    __shared__ uint32_t lw[1024 / WAVE];
+    const int64_t per = (n + blockDim.x - 1) / blockDim.x;
+    const int64_t b = threadIdx.x * per;
+    uint32_t s = 0;
+    for (int64_t i = b; i < b + per && i < n; i++) s += in[i];
+    uint32_t ex;
+    const uint32_t tot = block_exclusive_scan(s, &ex, lw);
+    uint32_t run = ex;
+    for (int64_t i = b; i < b + per && i < n; i++) {
+        const uint32_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
 int exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* total_dev, void* scratch,
                        hipStream_t stream) {
     if (n <= 0) return 0;
+    if (n <= SCAN_SMALL_MAX) {
+        hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, stream, in, out, n, total_dev);
+        return (int)hipGetLastError();
+    }
     int64_t nb = (n + SC_TILE - 1) / SC_TILE;
     uint32_t* bsum = (uint32_t*)scratch;
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SC_BLOCK), 0, stream, in, n, bsum);
