@@ -15,6 +15,7 @@ import ctypes as C
 import gzip
 import os
 import struct
+from collections.abc import Sequence
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -514,16 +515,47 @@ _RESULT_DTYPE = np.dtype({"names": ["final_transformation", "final_transformatio
                            "itemsize": C.sizeof(NdtResult)})
 
 
-def _results(out, n: int) -> list:
-    """Result dicts of the first n entries of a ctypes NdtResult array, converted
-    with one structured view instead of field by field."""
-    a = np.frombuffer(out, dtype=_RESULT_DTYPE, count=n)
-    f32 = a["final_transformation"].reshape(n, 4, 4).copy()
-    f64 = a["final_transformation_f64"].reshape(n, 4, 4).copy()
-    score, tp = a["score"].tolist(), a["trans_probability"].tolist()
-    it, cv, ps, st = a["iterations"].tolist(), a["converged"].tolist(), a["passes"].tolist(), a["status"].tolist()
-    return [dict(final_f32=f32[i], final_f64=f64[i], score=score[i], trans_probability=tp[i], iterations=it[i],
-                 converged=bool(cv[i]), passes=ps[i], status=st[i]) for i in range(n)]
+class Results(Sequence):
+    """Alignment results of a batch, one dict per scan (final_f32, final_f64,
+    score, trans_probability, iterations, converged, passes, status), made from
+    the returned records (in one vectorised pass) when first read: a caller that
+    reads none of them does not pay for the conversion."""
+
+    __slots__ = ("_a", "_d")
+
+    def __init__(self, out, n: int):
+        self._a = np.frombuffer(out, dtype=_RESULT_DTYPE, count=n).copy()
+        self._d = None
+
+    def _dicts(self) -> list:
+        if self._d is None:
+            a, n = self._a, len(self._a)
+            f32 = a["final_transformation"].reshape(n, 4, 4).copy()
+            f64 = a["final_transformation_f64"].reshape(n, 4, 4).copy()
+            score, tp = a["score"].tolist(), a["trans_probability"].tolist()
+            it, cv, ps, st = (a["iterations"].tolist(), a["converged"].tolist(), a["passes"].tolist(),
+                              a["status"].tolist())
+            self._d = [dict(final_f32=f32[i], final_f64=f64[i], score=score[i], trans_probability=tp[i],
+                            iterations=it[i], converged=bool(cv[i]), passes=ps[i], status=st[i]) for i in range(n)]
+        return self._d
+
+    def __len__(self) -> int:
+        return len(self._a)
+
+    def __getitem__(self, i):
+        return self._dicts()[i]
+
+    def __iter__(self):
+        return iter(self._dicts())
+
+    def field(self, name: str) -> np.ndarray:
+        """One record field for every scan as an array (for example "passes")."""
+        return self._a[name].copy()
+
+
+def _results(out, n: int) -> Results:
+    """Results of the first n entries of a ctypes NdtResult array."""
+    return Results(out, n)
 
 
 def _result(r: NdtResult) -> dict:
