@@ -911,6 +911,7 @@ extern "C" int ndt_scans_upload(ndt_ctx* c, const float* const* scans, const int
     if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
     HIP_TRY(hipSetDevice(c->device));
     ndt_scans* sc = new (std::nothrow) ndt_scans();
+    if (sc) sc->serial = next_grid_serial();
     if (!sc) return fail(NDT_ERR_NOMEM, "host allocation failed");
     sc->ctx = c;
     sc->S = S;

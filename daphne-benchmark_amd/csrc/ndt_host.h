@@ -217,6 +217,7 @@ struct GraphPlan {
 
 struct ndt_scans {
     ndt::CtxRef ctx;
+    uint64_t serial = 0; /* unique per uploaded batch (derived data cached elsewhere refers to it) */
     int32_t S = 0;
     std::vector<int64_t> counts, offsets;
     int64_t total = 0;

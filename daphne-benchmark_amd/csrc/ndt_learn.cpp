@@ -97,6 +97,7 @@ int pick_splits(int M, int N, int K, int cus) {
 struct ndt_bevmap {
     ndt::CtxRef ctx;
     std::atomic<int> refs{1}; /* the creator's, plus one per dataset made on it */
+    uint64_t serial = 0;      /* unique per BEV map */
     DevBuf pts, start, lut;
     TileGridDev dev{};
     double origin[3] = {0, 0, 0};
@@ -112,8 +113,11 @@ struct ndt_model {
     /* batch workspace */
     int wsB = 0, splits1 = 1, splits2 = 1;
     DevBuf X, XT, P1, H1, H1T, P2, H2, Y, dY, lossb, dZ2, dZ2T, dH1, dZ1T, target, weights, lossacc, bpart;
-    /* prediction inputs */
+    /* prediction inputs; sbev holds the scan grids of the batch with serial
+     * sbev_scans on the BEV map with serial sbev_bm (they do not depend on the
+     * guesses, so repeated predictions for one batch reuse them) */
     DevBuf sbev, idx, gl, off, cnt;
+    uint64_t sbev_scans = 0, sbev_bm = 0;
 };
 
 /* Drops one reference to a BEV map; the last frees it. */
@@ -197,6 +201,10 @@ extern "C" int ndt_bevmap_build(ndt_ctx* c, const float* map, int64_t m, const d
     std::vector<uint16_t> lut(65536);
     for (int k = 0; k < 65536; k++) lut[(size_t)k] = f2bf_host((float)std::log1p((double)k));
     ndt_bevmap* bm = new ndt_bevmap();
+    {
+        static std::atomic<uint64_t> next_bm{1};
+        bm->serial = next_bm.fetch_add(1);
+    }
     bm->ctx = c;
     bm->m = finite;
     for (int k = 0; k < 3; k++) bm->origin[k] = origin ? origin[k] : 0.0;
@@ -805,7 +813,12 @@ static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const nd
     int e = ensure_workspace(md, Bp);
     if (e) return e;
     hipStream_t st = c->stream;
-    if ((e = scan_grids(md, bm, sc, md->sbev))) return e;
+    if (md->sbev_scans != sc->serial || md->sbev_bm != bm->serial) {
+        md->sbev_scans = md->sbev_bm = 0;
+        if ((e = scan_grids(md, bm, sc, md->sbev))) return e;
+        md->sbev_scans = sc->serial;
+        md->sbev_bm = bm->serial;
+    }
     std::vector<int32_t> idx((size_t)Bp);
     std::vector<double> gl;
     guesses_local(bm, guesses, S, gl);
