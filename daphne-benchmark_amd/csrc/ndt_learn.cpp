@@ -240,16 +240,18 @@ extern "C" int ndt_bevmap_destroy(ndt_bevmap* bm) {
 /* ------------------------------------------------------------------ model */
 
 static int ensure_workspace(ndt_model* md, int B) {
-    if (md->wsB >= B) return 0;
     const int64_t f0 = md->f0, f1 = md->f1, f2 = md->f2;
+    /* the split-K counts depend on this call's rows only, so that a result does not
+     * depend on the sizes of earlier calls */
     md->splits1 = pick_splits(B, md->f1, md->f0, md->ctx->cus);
     md->splits2 = pick_splits(B, md->f2, md->f1, md->ctx->cus);
+    BUF_TRY(md->P1, 4 * (size_t)md->splits1 * B * f1);
+    BUF_TRY(md->P2, 4 * (size_t)md->splits2 * B * f2);
+    if (md->wsB >= B) return 0;
     BUF_TRY(md->X, 2 * (size_t)B * f0);
     BUF_TRY(md->XT, 2 * (size_t)B * f0);
-    BUF_TRY(md->P1, 4 * (size_t)md->splits1 * B * f1);
     BUF_TRY(md->H1, 2 * (size_t)B * f1);
     BUF_TRY(md->H1T, 2 * (size_t)B * f1);
-    BUF_TRY(md->P2, 4 * (size_t)md->splits2 * B * f2);
     BUF_TRY(md->H2, 2 * (size_t)B * f2);
     BUF_TRY(md->Y, 4 * (size_t)B * NOUT);
     BUF_TRY(md->dY, 4 * (size_t)B * NOUT);

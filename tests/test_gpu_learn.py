@@ -141,7 +141,26 @@ def trained(A, ctx):
     ev, ev_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 256, 77), 900000 + np.arange(256))
     evS = A.Scans(ctx, ev)
     eg = A.perturb_poses(ev_gt, np.random.default_rng(9))
-    return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg)
+    return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg, ev=ev)
+
+
+def test_prediction_reuses_scan_grids_per_batch(A, ctx, trained):
+    """A model keeps the scan grids of the last batch it featurised: predictions for
+    the same batch with other guesses, after a prediction for another batch, equal
+    those of a fresh model loaded with the same weights."""
+    t = trained
+    other = A.Scans(ctx, t["ev"][:40])
+    g2 = A.perturb_poses(t["ev_gt"], np.random.default_rng(21))
+    m = t["model"]
+    p1 = m.predict(t["bm"], t["evS"], t["eg"])
+    po = m.predict(t["bm"], other, g2[:40])
+    p2 = m.predict(t["bm"], t["evS"], g2)
+    fresh = A.Model(ctx, seed=3)
+    for k in range(6):
+        fresh.set(k, m.get(k))
+    assert np.array_equal(p1, fresh.predict(t["bm"], t["evS"], t["eg"]))
+    assert np.array_equal(p2, fresh.predict(t["bm"], t["evS"], g2))
+    assert np.array_equal(po, fresh.predict(t["bm"], other, g2[:40]))
 
 
 def _success(results, gt):
