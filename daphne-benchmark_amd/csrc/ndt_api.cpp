@@ -132,6 +132,9 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
         delete c;
         return fail(NDT_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
     }
+    /* once per device, with the device current: the address of the workspace array
+     * of ndt_align_scans_noalloc (the other entry points do not need it) */
+    c->scratch_err = noalloc_scratch(device, &c->scratch);
     *out = c;
     return 0;
 }
@@ -934,6 +937,114 @@ extern "C" int ndt_scans_destroy(ndt_scans* sc) {
     (void)hipSetDevice(sc->ctx->device);
     delete sc;
     (void)hipGetLastError();
+    return 0;
+}
+
+/* ------------------------------------------------------------------ allocation-free alignment */
+
+/* ndt_align_scans without any allocation: the launch sequence of the captured
+ * submission, with every buffer carved from the device's workspace array (its
+ * address looked up when the context was created), the chunk descriptors made on
+ * the device from the batch's point counts, and the results converted on the device
+ * and copied straight into the caller's array.  Calls on one device share the
+ * array, so calls that overlap on a device race; every region a launch reads
+ * before writing is cleared by fill kernels first. */
+extern "C" int ndt_align_scans_noalloc(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const double* guesses,
+                                       const ndt_params* p, ndt_result* results) {
+    if (!c || !g || !sc || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
+    int e = check_params(p);
+    if (e) return e;
+    if (p->mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "batched alignment runs in ndt mode");
+    if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
+    e = check_poses(guesses, sc->S, "guess");
+    if (e) return e;
+    if (!c->scratch)
+        return fail(NDT_ERR_HIP, std::string("workspace address of the device unknown: ") +
+                                     hipGetErrorString((hipError_t)c->scratch_err));
+    if (p->scheduler == NDT_SCHED_ROUNDS)
+        return fail(NDT_ERR_UNSUPPORTED, "the allocation-free entry point runs the work-queue scheduler only");
+    HIP_TRY(hipSetDevice(c->device));
+    const int32_t S = sc->S;
+    const int32_t chunk = p->points_per_wg > 0 ? p->points_per_wg : ALIGN_CHUNK_POINTS;
+    const int32_t fine = fine_chunk(chunk);
+    int64_t slots = 0, first_pass = 0, max_chunks = 0;
+    for (int32_t s = 0; s < S; s++) {
+        const int64_t n = sc->counts[(size_t)s];
+        const int64_t fn = (n + fine - 1) / fine;
+        slots += fn;
+        first_pass += (n + chunk - 1) / chunk;
+        max_chunks = std::max(max_chunks, fn);
+    }
+    const uint64_t cap = queue_capacity(slots, p->max_iterations);
+    if (cap > MAX_QUEUE_ITEMS || S >= (1 << ITEM_SCAN_BITS) || max_chunks >= (1 << ITEM_CHUNK_BITS))
+        return fail(NDT_ERR_UNSUPPORTED, "batch too large for the work-queue scheduler");
+    /* workspace layout, 256-byte aligned regions */
+    uint64_t used = 0;
+    auto take = [&used](uint64_t bytes) {
+        const uint64_t at = used;
+        used = (used + bytes + 255) & ~(uint64_t)255;
+        return at;
+    };
+    const uint64_t o_counts = take(sizeof(int64_t) * (uint64_t)S);
+    const uint64_t o_desc = take(sizeof(ScanDesc) * (uint64_t)S);
+    const uint64_t o_guess = take(sizeof(double) * 16 * (uint64_t)S);
+    const uint64_t o_states = take(sizeof(ScanState) * (uint64_t)S);
+    const uint64_t o_sync = take(sizeof(RoundSync));
+    const uint64_t o_out = take(sizeof(ScanOut) * (uint64_t)S);
+    const uint64_t o_part = take(sizeof(double) * 32 * (uint64_t)slots);
+    const uint64_t o_queue = take(sizeof(WorkQueue));
+    const uint64_t o_items = take(sizeof(uint64_t) * cap);
+    const uint64_t o_done = take(sizeof(uint32_t) * (uint64_t)S);
+    const uint64_t o_prof = take(sizeof(double) * 8);
+    const uint64_t o_res = take(sizeof(ndt_result) * (uint64_t)S);
+    if (used > NOALLOC_SCRATCH_BYTES)
+        return fail(NDT_ERR_UNSUPPORTED, "batch needs " + std::to_string(used) +
+                                             " bytes of workspace; the allocation-free entry point has " +
+                                             std::to_string(NOALLOC_SCRATCH_BYTES) + " per device");
+    const int block = p->block_size ? p->block_size : 256;
+    AlignParams prm = make_align_params(p, S);
+    int bpc = 0;
+    LAUNCH_TRY(align_wq_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
+    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
+    const int32_t nwg = bpc * c->cus;
+    char* ws = (char*)c->scratch;
+    int64_t* d_counts = (int64_t*)(ws + o_counts);
+    ScanDesc* d_desc = (ScanDesc*)(ws + o_desc);
+    double* d_guess = (double*)(ws + o_guess);
+    ScanState* d_states = (ScanState*)(ws + o_states);
+    RoundSync* d_sync = (RoundSync*)(ws + o_sync);
+    ScanOut* d_out = (ScanOut*)(ws + o_out);
+    double* d_part = (double*)(ws + o_part);
+    WorkQueue* d_queue = (WorkQueue*)(ws + o_queue);
+    uint64_t* d_items = (uint64_t*)(ws + o_items);
+    uint32_t* d_done = (uint32_t*)(ws + o_done);
+    double* d_prof = (double*)(ws + o_prof);
+    ndt_result* d_res = (ndt_result*)(ws + o_res);
+    HIP_TRY(hipMemcpyAsync(d_counts, sc->counts.data(), sizeof(int64_t) * (size_t)S, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(d_guess, guesses, sizeof(double) * 16 * (size_t)S, hipMemcpyHostToDevice, c->stream));
+    LAUNCH_TRY(launch_make_descs(d_counts, S, chunk, d_desc, c->stream));
+    LAUNCH_TRY(launch_zero_u32(d_out, (int64_t)(sizeof(ScanOut) * (size_t)S / 4), c->stream));
+    LAUNCH_TRY(launch_zero_u32(d_items, (int64_t)(2 * cap), c->stream));
+    LAUNCH_TRY(launch_zero_u32(d_prof, 16, c->stream));
+    LAUNCH_TRY(launch_init_states(d_guess, S, prm, d_states, d_sync, c->stream));
+    LAUNCH_TRY(launch_init_queue(d_desc, S, (uint64_t)first_pass, d_queue, d_items, d_done, c->stream));
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), d_desc, nwg, block, prm, d_states, d_queue, d_items, cap,
+                               d_done, d_part, d_out, d_prof, 0, c->stream));
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    LAUNCH_TRY(launch_noalloc_results(d_out, d_desc, S, d_prof, d_res, c->stream));
+    double prof[8];
+    HIP_TRY(hipMemcpyAsync(results, d_res, sizeof(ndt_result) * (size_t)S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(prof, d_prof, sizeof(prof), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->last_ms = ms;
+    c->last_launches = 1;
+    c->last_pairs = (int64_t)prof[5];
+    for (int k = 0; k < 4; k++) c->last_profile[k] = prof[k];
+    if (prof[4] != 0.0) return fail(NDT_ERR_TIMEOUT, "a device-side wait exceeded its bound");
     return 0;
 }
 

@@ -140,6 +140,10 @@ struct ndt_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int cus = 0;
+    /* device address of the allocation-free entry point's workspace on this
+     * context's device (looked up at creation), or null with the lookup's error */
+    void* scratch = nullptr;
+    int scratch_err = 0;
     char name[256] = {0};
     /* workspaces */
     DevBuf pts, desc, states, sync, partials, out, guesses, small, partials2, queue, items, done;

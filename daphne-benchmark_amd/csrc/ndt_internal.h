@@ -465,6 +465,20 @@ int launch_deriv_compat(CompatGridDev grid, const float4* scan, int64_t n, const
                         double d1, double d2, double* partials, int nblocks, hipStream_t stream);
 int launch_reduce_partials(const double* partials, int nblocks, int width, double* out, hipStream_t stream);
 
+/* allocation-free alignment (ndt_align_scans_noalloc): a file-scope device array of
+ * NOALLOC_SCRATCH_BYTES per device holds the whole workspace of a call */
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+/* device address of this device's workspace array (call with `device` current) */
+int noalloc_scratch(int device, void** ptr);
+/* chunk descriptors of S scans from their point counts (device memory) */
+int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, ScanDesc* desc, hipStream_t stream);
+/* ndt_result records from the outputs of a launch; prof[5] receives the pair total */
+int launch_noalloc_results(const ScanOut* out, const ScanDesc* desc, int32_t S, double* prof, ndt_result* res,
+                           hipStream_t stream);
+static_assert(sizeof(ndt_result) == 232, "ndt_result layout shared by host and device");
+static_assert(sizeof(ScanOut) % 4 == 0, "ScanOut is cleared as 32-bit words");
+
 }  // namespace ndt
 
 #endif

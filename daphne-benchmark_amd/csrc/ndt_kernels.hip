@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_cooperative_groups.h>
 
+#include <mutex>
+
 #include "ndt_internal.h"
 
 namespace ndt {
@@ -2608,6 +2610,147 @@ int launch_deriv_compat(CompatGridDev grid, const float4* scan, int64_t n, const
     a.d1 = d1;
     a.d2 = d2;
     hipLaunchKernelGGL(k_deriv_compat, dim3(nblocks), dim3(256), 0, stream, grid, scan, n, a, partials);
+    return (int)hipGetLastError();
+}
+
+/* ------------------------------------------------------------------ allocation-free alignment
+ * Workspace of ndt_align_scans_noalloc, the entry point for callers that cannot
+ * allocate device memory.  It is a file-scope device array: every device the
+ * library runs on holds its own copy from the loading of the code object until the
+ * process ends.  On the host the array's name stands for a host-side shadow, not
+ * for device memory, so host code never passes it to a kernel or a copy:
+ * noalloc_scratch() looks up the device address once per device (with that device
+ * current) and every launch and copy of the entry point uses that address. */
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+int noalloc_scratch(int device, void** ptr) {
+    *ptr = nullptr;
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *ptr = g_scratch_ptr[device];
+    return 0;
+}
+
+/* Inclusive prefix sum over the 64 lanes of a wave. */
+__device__ inline int64_t wave_inclusive_scan(int64_t v) {
+    const int lane = threadIdx.x & (WAVE - 1);
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const int64_t u = __shfl_up(v, d, WAVE);
+        if (lane >= d) v += u;
+    }
+    return v;
+}
+
+constexpr int DESC_BLOCK = 1024;
+
+/* Chunk descriptors of a device-resident batch from its point counts, as the host
+ * makes them for the other entry points: every scan cut into `chunk`-point chunks
+ * (and fine_chunk(chunk)-point chunks for late passes), points, partial-sum slots
+ * and first-pass queue items laid out in scan order.  One workgroup; exclusive
+ * prefix sums tile by tile with a running carry. */
+__global__ __launch_bounds__(DESC_BLOCK) void k_make_descs(const int64_t* __restrict__ counts, int32_t S,
+                                                           int32_t chunk, ScanDesc* __restrict__ desc) {
+    __shared__ int64_t wsum[3][DESC_BLOCK / WAVE];
+    __shared__ int64_t carry[3];
+    const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
+    const int32_t fine = fine_chunk(chunk);
+    if (tid < 3) carry[tid] = 0;
+    __syncthreads();
+    for (int32_t base = 0; base < S; base += DESC_BLOCK) {
+        const int32_t s = base + tid;
+        const int64_t n = s < S ? counts[s] : 0;
+        int64_t v[3] = {n, (n + fine - 1) / fine, (n + chunk - 1) / chunk};
+        int64_t inc[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            inc[k] = wave_inclusive_scan(v[k]);
+            if (lane == WAVE - 1) wsum[k][wave] = inc[k];
+        }
+        __syncthreads();
+        int64_t ex[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            int64_t before = carry[k];
+            for (int w = 0; w < wave; w++) before += wsum[k][w];
+            ex[k] = before + inc[k] - v[k];
+        }
+        if (s < S) {
+            ScanDesc d;
+            d.offset = ex[0];
+            d.n = (int32_t)n;
+            d.chunk = chunk;
+            d.nchunks = (int32_t)v[2];
+            d.fnchunks = (int32_t)v[1];
+            d.chunk_base = ex[1];
+            d.qbase = ex[2];
+            desc[s] = d;
+        }
+        __syncthreads();
+        if (tid < 3) {
+            int64_t t = 0;
+            for (int w = 0; w < DESC_BLOCK / WAVE; w++) t += wsum[tid][w];
+            carry[tid] += t;
+        }
+        __syncthreads();
+    }
+}
+
+int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, ScanDesc* desc, hipStream_t stream) {
+    hipLaunchKernelGGL(k_make_descs, dim3(1), dim3(DESC_BLOCK), 0, stream, counts, S, chunk, desc);
+    return (int)hipGetLastError();
+}
+
+/* Results of a finished launch in the C ABI's layout (the host conversion of the
+ * other entry points, on the device): a scan that never finished in an aborted
+ * launch is marked as timed out; prof[5] receives the pairs of all scans. */
+__global__ __launch_bounds__(DESC_BLOCK) void k_noalloc_results(const ScanOut* __restrict__ out,
+                                                                const ScanDesc* __restrict__ desc, int32_t S,
+                                                                double* prof, ndt_result* __restrict__ res) {
+    __shared__ double wp[DESC_BLOCK / WAVE];
+    const int tid = threadIdx.x;
+    const bool aborted = prof[4] != 0.0;
+    double pairs = 0.0;
+    for (int32_t s = tid; s < S; s += DESC_BLOCK) {
+        const ScanOut o = out[s];
+        ndt_result r;
+        for (int k = 0; k < 16; k++) {
+            r.final_transformation_f64[k] = o.final_m[k];
+            r.final_transformation[k] = (float)o.final_m[k];
+        }
+        r.fitness_score = 0.0;
+        r.score = o.score;
+        r.trans_probability = desc[s].n > 0 ? o.score / (double)desc[s].n : 0.0;
+        r.iterations = o.iterations;
+        r.converged = o.converged;
+        r.passes = o.passes;
+        r.status = (aborted && o.final_m[15] == 0.0) ? NDT_ERR_TIMEOUT : o.status;
+        res[s] = r;
+        pairs += o.pairs;
+    }
+    pairs = wave_sum(pairs);
+    if ((tid & (WAVE - 1)) == 0) wp[tid / WAVE] = pairs;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < DESC_BLOCK / WAVE; w++) t += wp[w];
+        prof[5] = t;
+    }
+}
+
+int launch_noalloc_results(const ScanOut* out, const ScanDesc* desc, int32_t S, double* prof, ndt_result* res,
+                           hipStream_t stream) {
+    hipLaunchKernelGGL(k_noalloc_results, dim3(1), dim3(DESC_BLOCK), 0, stream, out, desc, S, prof, res);
     return (int)hipGetLastError();
 }
 

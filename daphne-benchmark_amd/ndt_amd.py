@@ -441,6 +441,7 @@ def hip_lib():
         L.ndt_scans_upload.argtypes = [P, P, P, C.c_int32, C.POINTER(P)]
         L.ndt_scans_destroy.argtypes = [P]
         L.ndt_align_scans.argtypes = [P, P, P, P, C.POINTER(NdtParams), P]
+        L.ndt_align_scans_noalloc.argtypes = [P, P, P, P, C.POINTER(NdtParams), P]
         L.ndt_compat_callback.argtypes = [P, P, C.c_int64, P, P, C.c_int64, C.POINTER(NdtResult)]
         L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
@@ -479,7 +480,7 @@ EXPORTED_SYMBOLS = ("ndt_params_default", "ndt_last_error", "ndt_version", "ndt_
                     "ndt_ctx_device_info", "ndt_grid_build", "ndt_grid_info_get", "ndt_grid_destroy",
                     "ndt_grid_voxels", "ndt_align", "ndt_align_batch", "ndt_scans_upload", "ndt_scans_destroy",
                     "ndt_align_scans", "ndt_compat_callback", "ndt_derivatives", "ndt_ctx_last_timing",
-                    "ndt_ctx_last_profile", "ndt_fitness_score", "ndt_autotune")
+                    "ndt_ctx_last_profile", "ndt_fitness_score", "ndt_autotune", "ndt_align_scans_noalloc")
 LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_model_create", "ndt_model_destroy",
                  "ndt_model_dims", "ndt_model_save", "ndt_model_load", "ndt_model_get", "ndt_model_set",
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
@@ -712,6 +713,15 @@ class Scans:
         guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(self.S, 16)
         out = (NdtResult * self.S)()
         _check(hip_lib().ndt_align_scans(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm), out))
+        return _results(out, self.S)
+
+    def align_noalloc(self, grid: Grid, guesses, prm: NdtParams | None = None) -> list:
+        """align() through the allocation-free entry point (fixed device workspace)."""
+        prm = prm or grid.params
+        guesses = np.ascontiguousarray(guesses, dtype=np.float64).reshape(self.S, 16)
+        out = (NdtResult * self.S)()
+        _check(hip_lib().ndt_align_scans_noalloc(self.ctx.h, grid.h, self.h, guesses.ctypes.data, C.byref(prm),
+                                                 out))
         return _results(out, self.S)
 
     def autotune(self, grid: Grid, guesses, prm: NdtParams | None = None, cache_path: str | None = None):

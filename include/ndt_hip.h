@@ -205,6 +205,20 @@ int ndt_fitness_score(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans
 int ndt_autotune(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans, const double* guesses,
                  ndt_params* params, const char* cache_path, double* best_ms);
 
+/* ndt_align_scans for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference: it exists for those callers only.  Same arguments,
+ * results and kernels as ndt_align_scans (with the work-queue scheduler, results
+ * equal bit for bit), but every workspace buffer lives in a fixed device array of
+ * 64 MiB per device, held from the library's first use of the device until the
+ * process ends; the call allocates nothing.  A batch whose workspace (about
+ * 2.9 KB per partial-sum slot at 30 iterations, plus 1.5 KB per scan) exceeds the
+ * array returns NDT_ERR_UNSUPPORTED, as does scheduler NDT_SCHED_ROUNDS.  All
+ * calls on one device share the array: calls that overlap on a device (from two
+ * threads or contexts) race and must be serialised by the caller. */
+int ndt_align_scans_noalloc(ndt_ctx* ctx, const ndt_grid* grid, const ndt_scans* scans,
+                            const double* guesses, const ndt_params* params, ndt_result* results);
+
 /* Timing of the last ndt_align / ndt_align_batch / ndt_align_scans call on this
  * context: device milliseconds of the alignment kernel(s), measured with HIP
  * events on the context's stream, and the number of kernel launches. */
