@@ -715,6 +715,8 @@ void orc_ndt_params_default(orc_ndt_params* p) {
     p->max_iterations = 30;
     p->min_points = 6;
     p->min_eig_ratio = 0.01;
+    p->hessian_at_accept = 0;
+    p->pad = 0;
 }
 
 void orc_sym3_eig(const double Ain[9], double evals[3], double evecs[9]) {
@@ -1028,20 +1030,29 @@ double orc_ndt_score(const orc_ndt_grid* g, const orc_ndt_params* prm, const flo
     return ndt_pass(g, prm, d1, d2, scan4, n, p, NULL, NULL, 0, 0, NULL);
 }
 
-static double ndt_step_length(ndt_ctx* c, const double x[6], double step_dir[6], double step_init,
-                              double step_max, double step_min, double* score, double grad[6],
-                              double H[36], double x_t[6]) {
-    /* computeStepLengthMT (kernel.cpp:1015-1144) with the More-Thuente loop live:
-     * interval_converged starts false.  Every pass computes the Hessian as well, so
-     * the Hessian at the accepted step is at hand without a separate pass. */
-    double phi_0 = -*score;
-    double d_phi_0 = -(dot6(grad, step_dir));
-    for (int i = 0; i < 6; i++) x_t[i] = x[i];
+int orc_mt_search(orc_line_fn phi, void* ctx, double phi_0, double d_phi_0, double step_init,
+                  double step_max, double step_min, int32_t flags, orc_mt_out* out) {
+    /* computeStepLengthMT (kernel.cpp:1015-1144) over a scalar function: phi(t) and
+     * phi'(t) along the caller's direction.  An ascent direction is reversed
+     * (kernel.cpp:1024-1037), so that the search runs on phi_s(a) = phi(sign a),
+     * phi_s'(a) = sign phi'(sign a).  The More-Thuente loop is live: interval_converged
+     * starts false (the reference starts it at (step_max - step_min) > 0, :1053,
+     * which never iterates; ORC_MT_DEAD_LOOP restores that).  With
+     * ORC_MT_HESSIAN_AT_ACCEPT the evaluations inside the loop ask for no Hessian
+     * (:1123) and one more evaluation with the Hessian follows at the returned step
+     * when the loop ran (:1141-1142). */
+    memset(out, 0, sizeof(*out));
+    out->sign = 1;
     if (d_phi_0 >= 0) {
-        if (d_phi_0 == 0) return 0;
+        if (d_phi_0 == 0) {
+            out->exit_reason = ORC_MT_NO_DESCENT;
+            return 0;
+        }
         d_phi_0 *= -1;
-        for (int i = 0; i < 6; i++) step_dir[i] *= -1;
+        out->sign = -1;
     }
+    const double sg = (double)out->sign;
+    const int hess_in_loop = !(flags & ORC_MT_HESSIAN_AT_ACCEPT);
     int max_step_iterations = 10, step_iterations = 0;
     double mu = 1.e-4, nu = 0.9;
     double a_l = 0, a_u = 0;
@@ -1049,15 +1060,15 @@ static double ndt_step_length(ndt_ctx* c, const double x[6], double step_dir[6],
     double g_l = dpsi_mt(d_phi_0, d_phi_0, mu);
     double f_u = psi_mt(a_u, phi_0, phi_0, d_phi_0, mu);
     double g_u = dpsi_mt(d_phi_0, d_phi_0, mu);
-    int interval_converged = 0, open_interval = 1;
+    int interval_converged = (flags & ORC_MT_DEAD_LOOP) ? ((step_max - step_min) > 0) : 0;
+    int open_interval = 1;
     double a_t = step_init;
     a_t = std_min(a_t, step_max);
     a_t = std_max(a_t, step_min);
-    for (int i = 0; i < 6; i++) x_t[i] = x[i] + step_dir[i] * a_t;
-    *score = ndt_pass(c->g, c->prm, c->d1, c->d2, c->scan4, c->n, x_t, grad, H, 1, 1, NULL);
-    c->passes++;
-    double phi_t = -*score;
-    double d_phi_t = -(dot6(grad, step_dir));
+    double dphi;
+    double phi_t = phi(ctx, sg * a_t, 1, &dphi);
+    out->evals++;
+    double d_phi_t = sg * dphi;
     double psi_t = psi_mt(a_t, phi_t, phi_0, d_phi_0, mu);
     double d_psi_t = dpsi_mt(d_phi_t, d_phi_0, mu);
     while (!interval_converged && step_iterations < max_step_iterations &&
@@ -1068,11 +1079,9 @@ static double ndt_step_length(ndt_ctx* c, const double x[6], double step_dir[6],
             a_t = orc_mt_trial_value(a_l, f_l, g_l, a_u, f_u, g_u, a_t, phi_t, d_phi_t);
         a_t = std_min(a_t, step_max);
         a_t = std_max(a_t, step_min);
-        for (int r = 0; r < 6; r++) x_t[r] = x[r] + step_dir[r] * a_t;
-        *score = ndt_pass(c->g, c->prm, c->d1, c->d2, c->scan4, c->n, x_t, grad, H, 1, 1, NULL);
-        c->passes++;
-        phi_t = -*score;
-        d_phi_t = -(dot6(grad, step_dir));
+        phi_t = phi(ctx, sg * a_t, hess_in_loop, &dphi);
+        out->evals++;
+        d_phi_t = sg * dphi;
         psi_t = psi_mt(a_t, phi_t, phi_0, d_phi_0, mu);
         d_psi_t = dpsi_mt(d_phi_t, d_phi_0, mu);
         if (open_interval && (psi_t <= 0 && d_psi_t >= 0)) {
@@ -1088,7 +1097,42 @@ static double ndt_step_length(ndt_ctx* c, const double x[6], double step_dir[6],
             interval_converged = orc_mt_update_interval(&a_l, &f_l, &g_l, &a_u, &f_u, &g_u, a_t, phi_t, d_phi_t);
         step_iterations++;
     }
-    return a_t;
+    if (psi_t <= 0 && d_phi_t <= -nu * d_phi_0)
+        out->exit_reason = ORC_MT_ACCEPTED;
+    else if (interval_converged)
+        out->exit_reason = ORC_MT_INTERVAL;
+    else
+        out->exit_reason = ORC_MT_MAX_ITER;
+    if (step_iterations && !hess_in_loop) {
+        (void)phi(ctx, sg * a_t, 1, &dphi);
+        out->evals++;
+    }
+    out->a = a_t;
+    out->loop_iterations = step_iterations;
+    return 0;
+}
+
+typedef struct ndt_line {
+    ndt_ctx* c;
+    const double* x;
+    const double* dir;
+    double* x_t;
+    double* score;
+    double* grad;
+    double* H;
+} ndt_line;
+
+/* phi(t) = -score(x + t dir), phi'(t) = -grad . dir: one derivative pass at the trial
+ * pose (kernel.cpp:1058-1066, :1114-1128), leaving score, gradient and (when asked
+ * for) Hessian of the last trial in the driver's state. */
+static double ndt_line_eval(void* vctx, double t, int with_hessian, double* dphi) {
+    ndt_line* L = (ndt_line*)vctx;
+    for (int i = 0; i < 6; i++) L->x_t[i] = L->x[i] + L->dir[i] * t;
+    *L->score = ndt_pass(L->c->g, L->c->prm, L->c->d1, L->c->d2, L->c->scan4, L->c->n, L->x_t, L->grad,
+                         with_hessian ? L->H : NULL, 1, with_hessian, NULL);
+    L->c->passes++;
+    *dphi = -(dot6(L->grad, L->dir));
+    return -*L->score;
 }
 
 int orc_ndt_align(const orc_ndt_grid* g, const orc_ndt_params* prm, const float* scan4, int64_t n,
@@ -1126,7 +1170,14 @@ int orc_ndt_align(const orc_ndt_grid* g, const orc_ndt_params* prm, const float*
             break;
         }
         for (int i = 0; i < 6; i++) delta_p[i] /= norm;
-        double a = ndt_step_length(&c, p, delta_p, norm, prm->step_size, prm->trans_eps / 2, &score, grad, H, x_t);
+        /* computeStepLengthMT (kernel.cpp:1015-1144), phi(0) = -score, phi'(0) = -grad . dir */
+        ndt_line L = {&c, p, delta_p, x_t, &score, grad, H};
+        orc_mt_out mt;
+        orc_mt_search(ndt_line_eval, &L, -score, -(dot6(grad, delta_p)), norm, prm->step_size, prm->trans_eps / 2,
+                      prm->hessian_at_accept ? ORC_MT_HESSIAN_AT_ACCEPT : 0, &mt);
+        if (mt.sign < 0)
+            for (int i = 0; i < 6; i++) delta_p[i] *= -1;
+        const double a = mt.a;
         for (int i = 0; i < 6; i++) p[i] = p[i] + delta_p[i] * a;
         iter++;
         if (iter > 1 && fabs(a) < prm->trans_eps) {

@@ -79,6 +79,34 @@ int orc_mt_update_interval(double* a_l, double* f_l, double* g_l, double* a_u, d
 double orc_mt_trial_value(double a_l, double f_l, double g_l, double a_u, double f_u, double g_u,
                           double a_t, double f_t, double g_t);
 
+/* More-Thuente line search (computeStepLengthMT, kernel.cpp:1015-1144) over a scalar
+ * function.  phi(ctx, t, with_hessian, &dphi) returns phi(t) and writes phi'(t) for
+ * a signed step t along the caller's direction; with_hessian tells an NDT caller
+ * whether the evaluation is one whose Hessian is kept.  phi_0 = phi(0), d_phi_0 =
+ * phi'(0).  The result: step length a >= 0 along sign * direction. */
+typedef double (*orc_line_fn)(void* ctx, double t, int with_hessian, double* dphi);
+enum {
+    ORC_MT_ACCEPTED = 0,   /* sufficient decrease and curvature condition hold at a */
+    ORC_MT_INTERVAL = 1,   /* the interval of uncertainty collapsed (updateIntervalMT) */
+    ORC_MT_MAX_ITER = 2,   /* 10 loop iterations without an accepted step */
+    ORC_MT_NO_DESCENT = 3  /* phi'(0) == 0: no step, no evaluation */
+};
+enum {
+    ORC_MT_DEAD_LOOP = 1,         /* planted mistake for tests: the reference's start
+                                     interval_converged = (step_max - step_min) > 0 (:1053) */
+    ORC_MT_HESSIAN_AT_ACCEPT = 2  /* loop evaluations without the Hessian, one more with it at
+                                     the returned step when the loop ran (:1123, :1141-1142) */
+};
+typedef struct orc_mt_out {
+    double a;
+    int32_t sign;            /* -1 when phi'(0) > 0 reversed the direction */
+    int32_t evals;           /* evaluations of phi */
+    int32_t loop_iterations; /* iterations of the More-Thuente loop */
+    int32_t exit_reason;     /* ORC_MT_* */
+} orc_mt_out;
+int orc_mt_search(orc_line_fn phi, void* ctx, double phi_0, double d_phi_0, double step_init,
+                  double step_max, double step_min, int32_t flags, orc_mt_out* out);
+
 /* ---------------- compat (reference semantics) ---------------- */
 int orc_compat_grid_build(const float* map4, int64_t m, float resolution, orc_compat_grid* g);
 void orc_compat_grid_free(orc_compat_grid* g);
@@ -111,6 +139,11 @@ typedef struct orc_ndt_params {
     int32_t max_iterations;
     int32_t min_points;
     double min_eig_ratio;
+    int32_t hessian_at_accept; /* 1: the line search's trial passes skip the Hessian, which
+                                  one more pass computes at the accepted step (the
+                                  reference's order, kernel.cpp:1123,1141-1142); 0: every
+                                  pass computes it (fewer passes, same poses) */
+    int32_t pad;
 } orc_ndt_params;
 
 void orc_ndt_params_default(orc_ndt_params* p);

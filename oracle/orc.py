@@ -37,7 +37,18 @@ class OrcCompatGrid(C.Structure):
 
 class OrcNdtParams(C.Structure):
     _fields_ = [("resolution", D), ("step_size", D), ("trans_eps", D), ("outlier_ratio", D),
-                ("max_iterations", C.c_int32), ("min_points", C.c_int32), ("min_eig_ratio", D)]
+                ("max_iterations", C.c_int32), ("min_points", C.c_int32), ("min_eig_ratio", D),
+                ("hessian_at_accept", C.c_int32), ("pad", C.c_int32)]
+
+
+class OrcMtOut(C.Structure):
+    _fields_ = [("a", D), ("sign", C.c_int32), ("evals", C.c_int32), ("loop_iterations", C.c_int32),
+                ("exit_reason", C.c_int32)]
+
+
+LINE_FN = C.CFUNCTYPE(D, C.c_void_p, D, C.c_int, C.POINTER(D))
+MT_ACCEPTED, MT_INTERVAL, MT_MAX_ITER, MT_NO_DESCENT = range(4)
+MT_DEAD_LOOP, MT_HESSIAN_AT_ACCEPT = 1, 2
 
 
 class OrcNdtVoxel(C.Structure):
@@ -72,6 +83,8 @@ def lib():
         L.orc_mt_update_interval.restype = C.c_int
         L.orc_mt_trial_value.argtypes = [D] * 9
         L.orc_mt_trial_value.restype = D
+        L.orc_mt_search.argtypes = [LINE_FN, C.c_void_p, D, D, D, D, D, C.c_int32, C.POINTER(OrcMtOut)]
+        L.orc_mt_search.restype = C.c_int
         L.orc_compat_grid_build.argtypes = [P, C.c_int64, C.c_float, C.POINTER(OrcCompatGrid)]
         L.orc_compat_grid_free.argtypes = [C.POINTER(OrcCompatGrid)]
         L.orc_compat_radius_search.argtypes = [C.POINTER(OrcCompatGrid), C.c_float, P, D, P, P]
@@ -214,6 +227,24 @@ def solve6(A, b):
 
 def mt_trial(*args):
     return lib().orc_mt_trial_value(*[float(a) for a in args])
+
+
+def mt_search(phi, phi_0, d_phi_0, step_init, step_max, step_min, flags=0):
+    """The oracle's More-Thuente search on a Python scalar function phi(t) -> (value,
+    derivative), t the signed step along the direction.  Returns (a, sign, evals,
+    loop iterations, exit reason, [(t, with_hessian), ...] of every evaluation)."""
+    calls = []
+
+    def cb(_ctx, t, with_hessian, dphi):
+        v, d = phi(t)
+        calls.append((t, with_hessian))
+        dphi[0] = d
+        return v
+
+    fn = LINE_FN(cb)
+    out = OrcMtOut()
+    lib().orc_mt_search(fn, None, phi_0, d_phi_0, step_init, step_max, step_min, flags, C.byref(out))
+    return out.a, out.sign, out.evals, out.loop_iterations, out.exit_reason, calls
 
 
 def mt_update(a_l, f_l, g_l, a_u, f_u, g_u, a_t, f_t, g_t):
