@@ -128,14 +128,15 @@ def _harness_time(exe, cwd, args=(), env=None):
     return per, (p.returncode == 0 and "result ok" in p.stdout)
 
 
-def daphne_harness_leg(cases_n: int) -> dict:
+def daphne_harness_leg(cases_n: int, extent: float = 100.0, map_points: int = 100_000,
+                       scan_points: int = 2000, seed: int = 3) -> dict:
     """The reference's own benchmark harness and the drop-in on one DAPHNE testcase
-    file (cfg0-sized testcases: 100 k-point map, 2 k-point scan, guess within
-    0.5 m / 2 deg).  The compiled reference (oracle/_ref: its single-threaded CPU
-    build and its OpenMP build) and daphne-benchmark_amd/ndt_mapping_hip read the
-    same ndt_input.dat and check the same ndt_output.dat, written by the reference's
-    datagen build; each prints its kernel time per testcase (grid build and
-    alignment, compat semantics)."""
+    file (by default cfg0-sized testcases: 100 k-point map, 2 k-point scan, guess
+    within 0.5 m / 2 deg).  The compiled reference (oracle/_ref: its
+    single-threaded CPU build and its OpenMP build) and
+    daphne-benchmark_amd/ndt_mapping_hip read the same ndt_input.dat and check the
+    same ndt_output.dat, written by the reference's datagen build; each prints its
+    kernel time per testcase (grid build and alignment, compat semantics)."""
     import shutil
     import subprocess
     import tempfile
@@ -146,11 +147,11 @@ def daphne_harness_leg(cases_n: int) -> dict:
     missing = [v for v in list(exe.values()) + [hip] if not os.path.exists(v)]
     if missing:
         return {"skipped": "not built: " + ", ".join(os.path.relpath(m, REPO) for m in missing)}
-    scene = A.scene_cfg(3, extent=100.0)
-    mp = A.make_map(scene, 100_000)
+    scene = A.scene_cfg(seed, extent=extent)
+    mp = A.make_map(scene, map_points)
     cases = []
     for k in range(cases_n):
-        scan, _gt, guess = A.make_scan(scene, mp, A.scan_cfg(2000, 0.5, 2.0), 500 + k)
+        scan, _gt, guess = A.make_scan(scene, mp, A.scan_cfg(scan_points, 0.5, 2.0), 500 + k)
         cases.append(A.Testcase(guess.astype(np.float32), scan, mp))
     d = tempfile.mkdtemp()
     try:
@@ -167,7 +168,7 @@ def daphne_harness_leg(cases_n: int) -> dict:
             r.fitness_score = 0.0  # never set by the reference
         A.write_ndt_output(os.path.join(data, "ndt_output.dat"), res)
         threads = os.cpu_count() or 1
-        out = {"testcases": cases_n, "map_points": int(mp.shape[0]), "scan_points": 2000,
+        out = {"testcases": cases_n, "map_points": int(mp.shape[0]), "scan_points": scan_points,
                "unit": "seconds per testcase (the harness's own timer)"}
         out["reference_cpu"], ok_cpu = _harness_time(exe["cpu"], run)
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
@@ -182,6 +183,124 @@ def daphne_harness_leg(cases_n: int) -> dict:
         return out
     finally:
         shutil.rmtree(d, ignore_errors=True)
+
+
+def cfg1_leg(ctx, n_scans: int, cpu_sample: int, harness_cases: int) -> dict:
+    """Plain NDT at real scale: a 1 M-point map of a 300 m scene, its voxel grid
+    built on the GPU, and n_scans sequential 20 k-point scans (guesses +-0.5 m /
+    +-2 deg), one ndt_align call per scan from host memory (upload, Morton order and
+    one launch each, as a caller aligning scans as they arrive would).  Beside it:
+    the float64 oracle on a sample of the same scans on the host cores, and the
+    reference's own harness (compiled reference, CPU and OpenMP builds, grid rebuilt
+    per testcase) on testcases of the same sizes."""
+    t0 = time.time()
+    w = A.make_workload("cfg1", seed=1, scans=n_scans)
+    gen_s = time.time() - t0
+    prm = A.params(A.NDT_MODE_NDT)
+    grid = A.Grid(ctx, w.map_xyzw, prm)
+    for i in range(min(3, n_scans)):
+        grid.align(w.scans[i], w.guesses[i], prm)
+    t = time.perf_counter()
+    res = [grid.align(sc, g, prm) for sc, g in zip(w.scans, w.guesses)]
+    wall = time.perf_counter() - t
+    out = {"workload": f"cfg1: {w.map_xyzw.shape[0]}-point map of a 300 m scene, {n_scans} sequential "
+                       f"{w.scans[0].shape[0]}-point scans, guesses +-0.5 m / +-2 deg",
+           "value": n_scans / wall, "unit": "alignments/s", "ms_per_scan": wall / n_scans * 1e3,
+           "grid_build_ms": grid.info()["build_ms"],
+           "mean_iterations": float(np.mean([r["iterations"] for r in res])),
+           "mean_passes": float(np.mean([r["passes"] for r in res])),
+           "converged": float(np.mean([r["converged"] for r in res])),
+           "success_vs_truth": float(np.mean([np.linalg.norm(r["final_f64"][:3, 3] - g[:3, 3]) < 0.05
+                                              for r, g in zip(res, w.gt)])),
+           "generation_s": gen_s}
+    if cpu_sample > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import orc
+        og = orc.NdtGrid(w.map_xyzw)
+        k = min(cpu_sample, n_scans)
+        threads = os.cpu_count() or 1
+        tc = time.perf_counter()
+        ref = og.align_many(w.scans[:k], w.guesses[:k], threads=threads)
+        cpu_s = time.perf_counter() - tc
+        err = np.array([orc.pose_error(r["final_f64"], o["final_f64"]) for r, o in zip(res[:k], ref)])
+        out["cpu_oracle"] = {"value": k / cpu_s, "unit": "alignments/s", "cores": threads, "scans": k,
+                             "kind": "port (float64 oracle, grid built once, one scan per thread)"}
+        out["accuracy_vs_fp64_oracle"] = {
+            "scans": k, "within_1cm_0.05deg": int(np.sum((err[:, 0] <= 0.01) & (err[:, 1] <= np.deg2rad(0.05)))),
+            "within_1mm_1e-4rad": int(np.sum((err[:, 0] <= 1e-3) & (err[:, 1] <= 1e-4))),
+            "same_iterations": int(sum(r["iterations"] == o["iterations"] for r, o in zip(res[:k], ref)))}
+    if harness_cases > 0:
+        try:
+            out["daphne_harness"] = daphne_harness_leg(harness_cases, extent=300.0, map_points=1_000_000,
+                                                       scan_points=20000, seed=1)
+        except Exception as e:  # reported, never fatal to the bench line
+            out["daphne_harness"] = {"error": f"{type(e).__name__}: {e}"}
+    return out
+
+
+def precision_study(args) -> dict:
+    """The cfg3 batch (the cfg2 batch with the scene at UTM-like coordinates) in the
+    four arithmetic modes, each against the all-float64 CPU oracle started from the
+    same guesses on a sample of scans: alignments/s, convergence and per-scan
+    translation / rotation error.  A sampled scan counts as reproducible when the
+    oracle's result moves by less than 1e-4 m when its guess moves by 1e-12 m."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import orc
+    w = A.make_workload("cfg3", seed=1, scans=args.scans)
+    S = len(w.scans)
+    ctx = A.Context(0)
+    grid = A.Grid(ctx, w.map_xyzw, A.params(A.NDT_MODE_NDT, origin=w.origin))
+    scans = A.Scans(ctx, w.scans)
+    idx = np.sort(np.random.default_rng(0).choice(S, size=min(args.cpu_sample, S), replace=False))
+    og = orc.NdtGrid(w.map_xyzw)
+    local = w.guesses[idx].copy()
+    local[:, :3, 3] -= np.array(w.origin)
+    ref = og.align_many([w.scans[i] for i in idx], local, threads=0)
+    local[:, 0, 3] += 1e-12
+    ref2 = og.align_many([w.scans[i] for i in idx], local, threads=0)
+    stable = np.array([orc.pose_error(a["final_f64"], b["final_f64"])[0] < 1e-4 for a, b in zip(ref, ref2)])
+    modes = {"fp64": (A.NDT_POINT_FP64, A.NDT_SUM_FP64), "fp32/fp32": (A.NDT_POINT_FP32, A.NDT_SUM_FP32),
+             "fp32/comp32": (A.NDT_POINT_FP32, A.NDT_SUM_COMP32), "fp32/fp64": (A.NDT_POINT_FP32, A.NDT_SUM_FP64)}
+    out = {"workload": f"cfg3: {S} scans, origin {list(w.origin)}", "oracle_sample": len(idx),
+           "oracle_reproducible": int(stable.sum()), "modes": {}}
+    for name, (pp, sp) in modes.items():
+        prm = A.params(A.NDT_MODE_NDT, point_precision=pp, sum_precision=sp, origin=w.origin, block_size=256,
+                       points_per_wg=4096)
+        scans.align(grid, w.guesses, prm)
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            res = scans.align(grid, w.guesses, prm)
+        rate = S * args.steps / (time.perf_counter() - t)
+        err = []
+        for k, i in enumerate(idx):
+            o = ref[k]["final_f64"].copy()
+            o[:3, 3] += np.array(w.origin)
+            err.append(orc.pose_error(res[i]["final_f64"], o))
+        err = np.array(err)
+        ok = (err[:, 0] <= 0.01) & (err[:, 1] <= np.deg2rad(0.05))
+        out["modes"][name] = {
+            "alignments_per_s": rate, "kernel_ms": ctx.last_timing()[0],
+            "converged": float(np.mean([r["converged"] for r in res])),
+            "within_1cm_0.05deg": int(ok.sum()), "within_1cm_0.05deg_reproducible": int(ok[stable].sum()),
+            "within_1mm_1e-4rad": int(np.sum((err[:, 0] <= 1e-3) & (err[:, 1] <= 1e-4))),
+            "translation_m": {"median": float(np.median(err[:, 0])), "max": float(err[:, 0].max())},
+            "rotation_rad": {"median": float(np.median(err[:, 1])), "max": float(err[:, 1].max())}}
+        log(f"{name}: {rate:.0f}/s, {ok[stable].sum()} of {stable.sum()} reproducible within 1 cm / 0.05 deg")
+    return out
+
+
+def hbm_copy_calibration(mib: int) -> dict:
+    """One device-to-device copy of a known size through torch (mib MiB read, mib
+    MiB written): profiling it with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE gives the
+    factors between those counters and bytes moved."""
+    import torch
+    n = mib * 1024 * 1024 // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copied_mib": mib}
 
 
 def main():
@@ -207,10 +326,23 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="default block size and chunk instead of the tuned ones")
     ap.add_argument("--block-size", type=int, default=0, help="fixed launch shape (with --points-per-wg): no tuning")
     ap.add_argument("--points-per-wg", type=int, default=0, help="fixed chunk size (with --block-size): no tuning")
+    ap.add_argument("--cfg1-scans", type=int, default=1000, help="sequential scans of the cfg1 leg (0: off)")
+    ap.add_argument("--cfg1-harness-cases", type=int, default=6,
+                    help="cfg1-sized DAPHNE testcases for the reference harness comparison (0: off)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="only the cfg3 precision study (four arithmetic modes against the float64 oracle)")
+    ap.add_argument("--hbm-copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only one device copy of MIB MiB (calibration of rocprofv3 traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); every box here has one GPU", file=sys.stderr)
         sys.exit(2)
+    if args.hbm_copy_calibration:
+        print(json.dumps(hbm_copy_calibration(args.hbm_copy_calibration)))
+        return
+    if args.precision_study:
+        print(json.dumps(precision_study(args)))
+        return
 
     t0 = time.time()
     w = A.make_workload(args.workload, seed=1, scans=args.scans)
@@ -411,6 +543,13 @@ def main():
                 out["daphne_harness"] = daphne_harness_leg(args.harness_cases)
             except Exception as e:  # reported, never fatal to the bench line
                 out["daphne_harness"] = {"error": f"{type(e).__name__}: {e}"}
+    if args.cfg1_scans > 0:
+        out["cfg1_sequential"] = cfg1_leg(ctx, args.cfg1_scans, 0 if args.no_cpu else 16,
+                                          0 if args.no_cpu else args.cfg1_harness_cases)
+        log(f"cfg1 sequential {out['cfg1_sequential']['value']:.0f}/s")
+    # every device handle closed, newest first, before the interpreter and the HIP
+    # runtime shut down
+    A.close_all()
     print(json.dumps(out))
 
 

@@ -99,6 +99,8 @@ static int check_params(const ndt_params* p) {
     if (p->points_per_wg < 0 || p->points_per_wg > (1 << 24))
         return fail(NDT_ERR_INVALID, "points_per_wg must be in 0 .. 2^24");
     if (p->scheduler < NDT_SCHED_DEFAULT || p->scheduler > NDT_SCHED_QUEUE) return fail(NDT_ERR_INVALID, "bad scheduler");
+    if (p->hessian_at_accept != 0 && p->hessian_at_accept != 1)
+        return fail(NDT_ERR_INVALID, "hessian_at_accept must be 0 or 1");
     return 0;
 }
 
@@ -404,9 +406,12 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
     BUF_TRY(g->vs64, 5 * sizeof(double2) * (nv + 1));
     LAUNCH_TRY(launch_voxel_s64(g->mean_c.as<double>(), g->icov_c.as<double>(), nv, g->vs64.as<double2>(), c->stream));
-    /* dense query-cell index when the map's cell box is at most 2^28 cells (2 GiB) */
+    /* dense query-cell index when the map's cell box is at most 2^28 cells (2 GiB);
+     * the environment variable NDT_FORCE_HASHED_INDEX=1 makes any map use the hashed
+     * index, so that tests run that path on maps of every size */
     const double box = (double)g->dims[0] * (double)g->dims[1] * (double)g->dims[2];
-    const bool dense = box <= (double)(1 << 28);
+    const char* force_hash = std::getenv("NDT_FORCE_HASHED_INDEX");
+    const bool dense = box <= (double)(1 << 28) && !(force_hash && std::strcmp(force_hash, "1") == 0);
     if (dense) {
         const size_t cells = (size_t)g->dims[0] * (size_t)g->dims[1] * (size_t)g->dims[2];
         BUF_TRY(g->dense, sizeof(uint2) * cells);
@@ -426,6 +431,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     g->info.query_cells = nq;
     g->info.candidates = ne;
     g->info.hash_capacity = cap;
+    g->info.dense_cells = dense ? (int64_t)box : 0;
     NdtGridDev& d = g->dev;
     d.dense = dense ? g->dense.as<uint2>() : nullptr;
     for (int k = 0; k < 3; k++) d.dims[k] = g->dims[k];
@@ -642,6 +648,7 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
     a.point_fp64 = p->point_precision == NDT_POINT_FP64;
     a.sum_mode = a.point_fp64 ? NDT_SUM_FP64 : p->sum_precision;
     a.nscans = S;
+    a.hess_at_accept = p->hessian_at_accept;
     a.wait_ticks = wait_limit_ticks();
     return a;
 }
@@ -1055,7 +1062,8 @@ static bool same_align_params(const ndt_params& a, const ndt_params& b) {
            a.trans_eps == b.trans_eps && a.max_iterations == b.max_iterations && a.outlier_ratio == b.outlier_ratio &&
            a.point_precision == b.point_precision && a.sum_precision == b.sum_precision &&
            a.origin[0] == b.origin[0] && a.origin[1] == b.origin[1] && a.origin[2] == b.origin[2] &&
-           a.block_size == b.block_size && a.points_per_wg == b.points_per_wg && a.scheduler == b.scheduler;
+           a.block_size == b.block_size && a.points_per_wg == b.points_per_wg && a.scheduler == b.scheduler &&
+           a.hessian_at_accept == b.hessian_at_accept;
 }
 
 /* the captured launch sequence; the caller ends the capture whatever this returns */
@@ -1197,8 +1205,9 @@ static std::string tune_key(ndt_ctx* c, const ndt_scans* sc, const ndt_params* p
     double total = (double)sc->total, mag = 1.0;
     while (total >= 1000.0 * mag) mag *= 10.0;
     const double rounded = std::round(total / mag) * mag;
-    std::snprintf(buf, sizeof(buf), "dev=%.120s;cus=%d;pp=%d;sp=%d;S=%d;N=%.0f", c->name, c->cus, p->point_precision,
-                  p->point_precision == NDT_POINT_FP64 ? NDT_SUM_FP64 : p->sum_precision, sc->S, rounded);
+    std::snprintf(buf, sizeof(buf), "dev=%.120s;cus=%d;pp=%d;sp=%d;ha=%d;S=%d;N=%.0f", c->name, c->cus,
+                  p->point_precision, p->point_precision == NDT_POINT_FP64 ? NDT_SUM_FP64 : p->sum_precision,
+                  p->hessian_at_accept, sc->S, rounded);
     std::string k(buf);
     for (auto& ch : k)
         if (ch == ' ' || ch == '\n' || ch == '\t') ch = '_';

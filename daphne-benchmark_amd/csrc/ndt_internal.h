@@ -275,7 +275,11 @@ struct CompatGridDev {
 
 /* ---------------------------------------------------------------- align */
 
-enum Phase : int32_t { PH_INIT = 0, PH_MT_FIRST = 1, PH_MT_LOOP = 2, PH_DONE = 3 };
+/* PH_MT_HESS: one more pass at an accepted step whose line-search loop ran with
+ * gradient-only passes (hess_at_accept) */
+enum Phase : int32_t { PH_INIT = 0, PH_MT_FIRST = 1, PH_MT_LOOP = 2, PH_DONE = 3, PH_MT_HESS = 4 };
+/* the pass a scan in this phase runs needs no Hessian (with hess_at_accept) */
+NDT_HD bool grad_only_pass(int32_t phase) { return phase == PH_MT_LOOP; }
 
 /* Per-scan Newton / line-search state, resident in device memory for the whole
  * alignment.  Exchanged between workgroups as 64-bit words. */
@@ -339,6 +343,8 @@ struct AlignParams {
     int32_t point_fp64;
     int32_t sum_mode;
     int32_t nscans;
+    int32_t hess_at_accept; /* line-search loop passes without the Hessian (ndt_params) */
+    int32_t pad;
     uint64_t wait_ticks; /* bound of one device-side wait, s_memrealtime ticks (100 MHz) */
 };
 

@@ -1111,7 +1111,7 @@ __device__ inline float prefilter_limit(const PassParams<T>& u) {
  * still runs every instruction, on zeroed inputs and with zero weights, and adds
  * +-0 to every sum (the sums are unchanged), so the accumulators need no copies at
  * control-flow joins.  Returns 1 when the pair contributes. */
-template <typename T, int SUM>
+template <typename T, int SUM, bool HESS = true>
 __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const double xd0, const double xd1,
                                const double xd2, const float r0, const float r1, const float r2, const float4 p,
                                uint32_t idx, const PassParams<T>& u, Acc<T, SUM>& acc) {
@@ -1206,6 +1206,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
         acc.fma(4, f, u3);
         acc.fma(5, f, u4);
         acc.fma(6, f, u5);
+        if constexpr (!HESS) return ok ? 1 : 0;
         const T fm = f * m;
         const T w0 = fm * q0, w1 = fm * q1, w2 = fm * q2, w3 = fm * u3, w4 = fm * u4, w5 = fm * u5;
         acc.fma(7, w0, q0);   acc.fma(7, f, c00);
@@ -1236,6 +1237,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
         acc.add(4, f * u3);
         acc.add(5, f * u4);
         acc.add(6, f * u5);
+        if constexpr (!HESS) return ok ? 1 : 0;
         const T mu0 = m * q0, mu1 = m * q1, mu2 = m * q2, mu3 = m * u3, mu4 = m * u4;
         acc.add(7, f * (mu0 * q0 + c00));
         acc.add(8, f * (mu0 * q1 + c01));
@@ -1264,7 +1266,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
 
 /* The candidates of one looked-up point selected by mask.  Returns the number of
  * pairs. */
-template <typename T, int SUM>
+template <typename T, int SUM, bool HESS = true>
 __device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, uint32_t mask,
                                   const PassParams<T>& u, Acc<T, SUM>& acc) {
     int pairs = 0;
@@ -1272,7 +1274,7 @@ __device__ inline int point_pairs(const NdtGridDev& g, const float4 p, const Poi
         const uint32_t k = (uint32_t)__builtin_ctz(mask);
         mask &= mask - 1;
         const uint32_t idx = (sizeof(T) == 4) ? L.begin + k : __float_as_uint(g.head[L.begin + k].w);
-        pairs += pair_add<T, SUM>(g, true, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, idx, u, acc);
+        pairs += pair_add<T, SUM, HESS>(g, true, L.x0, L.x1, L.x2, L.r0, L.r1, L.r2, p, idx, u, acc);
     }
     return pairs;
 }
@@ -1374,7 +1376,7 @@ __device__ inline void pair_source(const WavePairs& wp, const uint32_t n, const 
  * Each lane adds the terms of the pairs it evaluates to its own accumulators; the
  * assignment depends only on the wave's points, so results are reproducible.
  * Every lane of the wave must call this. */
-template <typename T, int SUM>
+template <typename T, int SUM, bool HESS = true>
 __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const PointLook& L, const uint32_t cnt,
                                  const PassParams<T>& u, Acc<T, SUM>& acc, WavePairs& wp, uint32_t& carry,
                                  const bool last) {
@@ -1402,7 +1404,7 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
         float4 ps;
         uint32_t idx;
         pair_source<T>(wp, n, carry, all, base < carry, p, L, valid, x0, x1, x2, r0, r1, r2, ps, idx);
-        pairs += pair_add<T, SUM>(g, valid, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
+        pairs += pair_add<T, SUM, HESS>(g, valid, x0, x1, x2, r0, r1, r2, ps, idx, u, acc);
     }
     if (full < all) {
         /* the remainder moves to carry slots 0 ..: pair full + j to slot j.  When no
@@ -1442,7 +1444,7 @@ __device__ inline int wave_pairs(const NdtGridDev& g, const float4 p, const Poin
 
 /* One scan point: lookup, prefilter over the (at most 27) candidate heads with
  * their loads in flight together, then the pairs.  Returns the number of pairs. */
-template <typename T, int SUM>
+template <typename T, int SUM, bool HESS = true>
 __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, const PassParams<T>& u, Acc<T, SUM>& acc) {
     PointLook L;
     if (!point_lookup(g, p, u.R, u.t, L)) return 0;
@@ -1450,7 +1452,7 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
     uint32_t mask = 0;
 #pragma unroll 4
     for (uint32_t k = 0; k < L.count; k++) mask |= prefilter_bit(g, L, k, lim);
-    return point_pairs<T, SUM>(g, p, L, mask, u, acc);
+    return point_pairs<T, SUM, HESS>(g, p, L, mask, u, acc);
 }
 
 /* Points [pb, pe) of one chunk at one trial pose, added to this thread's
@@ -1461,7 +1463,7 @@ __device__ inline int point_accumulate(const NdtGridDev& g, const float4 p, cons
  * leading the next round (wave_pairs).  With the hashed index each
  * thread handles its own points.  Both schedulers run their chunks through this,
  * so their partial sums are bit-identical. */
-template <typename T, int SUM, int BLOCK>
+template <typename T, int SUM, int BLOCK, bool HESS = true>
 __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __restrict__ pts, const int64_t pb,
                                        const int64_t pe, const PassParams<T>& u, Acc<T, SUM>& acc,
                                        WavePairs& wp) {
@@ -1508,13 +1510,14 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
             PointLook ln;
             uint2 en;
             lookup_issue(g, pn, vn, u.R, u.t, ln, en);
-            pairs += wave_pairs<T, SUM>(g, pc, lc, cnt, u, acc, wp, carry, ib + BLOCK >= pe);
+            pairs += wave_pairs<T, SUM, HESS>(g, pc, lc, cnt, u, acc, wp, carry, ib + BLOCK >= pe);
             pc = pn;
             lc = ln;
             ec = en;
         }
     } else {
-        for (int64_t i = pb + threadIdx.x; i < pe; i += BLOCK) pairs += point_accumulate<T, SUM>(g, pts[i], u, acc);
+        for (int64_t i = pb + threadIdx.x; i < pe; i += BLOCK)
+            pairs += point_accumulate<T, SUM, HESS>(g, pts[i], u, acc);
     }
     return pairs;
 }
@@ -1682,10 +1685,30 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
     }
 }
 
+/* accepts the line search's step x_t = p + a_t dir, whose score, gradient and
+ * Hessian are in the state, and starts the next Newton iteration */
+__device__ inline void ctl_accept(ScanState& st, const AlignParams& prm, ScanOut* out) {
+    for (int i = 0; i < 6; i++) st.p[i] = st.p[i] + st.dir[i] * st.a_t;
+    st.iter++;
+    if (st.iter > 1 && fabs(st.a_t) < prm.trans_eps) {
+        ctl_finish(st, prm, 1, out);
+        return;
+    }
+    ctl_newton_start(st, prm, out);
+}
+
 __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* red, const AlignParams& prm, ScanOut* out) {
     const double mu = 1.e-4, nu = 0.9;
     st.passes++;
     st.pairs += red[NACC];
+    if (st.phase == PH_MT_HESS) {
+        /* the Hessian at the accepted step of a line search whose loop ran
+         * (computeHessian, kernel.cpp:1141-1142); score and gradient are the last
+         * trial's */
+        for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];
+        ctl_accept(st, prm, out);
+        return;
+    }
     if (st.phase == PH_INIT) {
         st.score = red[0];
         for (int i = 0; i < 6; i++) st.grad[i] = red[1 + i];
@@ -1729,17 +1752,16 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
         st.phase = PH_MT_LOOP;
         return;
     }
-    /* accept x_t: its pass supplies score, gradient and Hessian */
-    for (int i = 0; i < 6; i++) st.p[i] = st.p[i] + st.dir[i] * st.a_t;
+    /* accept x_t: its pass supplies score and gradient, and the Hessian unless the
+     * loop's passes skipped it (then one more pass at x_t computes it) */
     st.score = red[0];
     for (int i = 0; i < 6; i++) st.grad[i] = g_t[i];
-    for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];
-    st.iter++;
-    if (st.iter > 1 && fabs(st.a_t) < prm.trans_eps) {
-        ctl_finish(st, prm, 1, out);
+    if (prm.hess_at_accept && st.phase == PH_MT_LOOP) {
+        st.phase = PH_MT_HESS;
         return;
     }
-    ctl_newton_start(st, prm, out);
+    for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];
+    ctl_accept(st, prm, out);
 }
 
 __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, AlignParams prm, ScanState* states,
@@ -1888,7 +1910,7 @@ __device__ inline void reduce_slots(const double* partials, int64_t base, int nc
     __syncthreads();
 }
 
-template <typename T, int SUM, int BLOCK>
+template <typename T, int SUM, int BLOCK, bool HA>
 __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align(
     NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, int32_t S,
     AlignParams prm, ScanState* states, RoundSync* rs, double* partials, ScanOut* out, double* prof_out) {
@@ -1901,7 +1923,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ uint32_t s_active[MAX_BATCH];
     __shared__ uint32_t s_prefix[MAX_BATCH + 1];
     __shared__ uint32_t lw[BLOCK / WAVE];
-    __shared__ int s_nactive, s_pp_scan, s_flag;
+    __shared__ int s_nactive, s_pp_scan, s_flag, s_grad_only;
     const int tid = threadIdx.x;
     const int wg = blockIdx.x;
     const int nwg = gridDim.x;
@@ -1950,7 +1972,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             if (s != s_pp_scan) {
                 __syncthreads();
                 if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
-                if (tid == 0) s_pp_scan = s;
+                if (tid == 0) {
+                    s_pp_scan = s;
+                    s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
+                }
                 __syncthreads();
             }
             Acc<T, SUM> acc;
@@ -1961,7 +1986,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
                 const int64_t cs = pass_chunk(sd, (uint32_t)ld_agent_i32(&states[s].passes));
                 const int64_t pb = sd.offset + k * cs;
                 const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * cs);
-                pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
+                if (HA && s_grad_only)
+                    pairs = chunk_accumulate<T, SUM, BLOCK, false>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
+                else
+                    pairs = chunk_accumulate<T, SUM, BLOCK, true>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
             }
             double v[NRED];
 #pragma unroll
@@ -2012,13 +2040,17 @@ static int launch_align_t(NdtGridDev g, const float4* pts, const ScanDesc* scans
                           int32_t nwg, AlignParams prm, ScanState* states, RoundSync* rs, double* partials,
                           ScanOut* out, double* prof, hipStream_t stream) {
     void* args[] = {&g, &pts, &scans, &S, &prm, &states, &rs, &partials, &out, &prof};
-    return (int)hipLaunchCooperativeKernel((const void*)k_align<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
-                                           stream);
+    const void* kern = prm.hess_at_accept ? (const void*)k_align<T, SUM, BLOCK, true>
+                                          : (const void*)k_align<T, SUM, BLOCK, false>;
+    return (int)hipLaunchCooperativeKernel(kern, dim3(nwg), dim3(BLOCK), args, 0, stream);
 }
 
+/* occupancy of the kernel without the gradient-only passes (the other one holds the
+ * same registers and LDS, checked by the launches that use it) */
 template <typename T, int SUM, int BLOCK>
 static int max_blocks_t(int* bpc) {
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align<T, SUM, BLOCK>, BLOCK, 0);
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align<T, SUM, BLOCK, false>, BLOCK,
+                                                             0);
 }
 
 #define NDT_DISPATCH(BLOCKSZ, FP64, SUMMODE, CALL)                                         \
@@ -2100,7 +2132,7 @@ int launch_init_queue(const ScanDesc* scans, int32_t S, uint64_t total_chunks, W
     return (int)hipGetLastError();
 }
 
-template <typename T, int SUM, int BLOCK>
+template <typename T, int SUM, int BLOCK, bool HA>
 __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? ALIGN_MIN_WAVES_F64 : ALIGN_MIN_WAVES_F32) void k_align_wq(
     NdtGridDev g, const float4* __restrict__ pts, const ScanDesc* __restrict__ scans, AlignParams prm,
     ScanState* states, WorkQueue* q, uint64_t* items, uint64_t cap, uint32_t* done_cnt, double* partials,
@@ -2113,7 +2145,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ WavePairs wslots[BLOCK / WAVE];
     __shared__ uint64_t s_item, s_ppkey;
     __shared__ uint32_t s_base;
-    __shared__ int s_last, s_done;
+    __shared__ int s_last, s_done, s_grad_only;
     const int tid = threadIdx.x;
     double prof[3] = {0, 0, 0};
     int ok = 1;
@@ -2151,6 +2183,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         const ScanDesc sd = scans[s];
         if (ppkey != s_ppkey) {
             if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
+            if (tid == 0) s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
             __syncthreads();
             if (tid == 0) s_ppkey = ppkey;
             __syncthreads();
@@ -2164,7 +2197,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const int64_t cs = pass_chunk(sd, pass);
             const int64_t pb = sd.offset + (int64_t)k * cs;
             const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * cs);
-            pairs = chunk_accumulate<T, SUM, BLOCK>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
+            if (HA && s_grad_only)
+                pairs = chunk_accumulate<T, SUM, BLOCK, false>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
+            else
+                pairs = chunk_accumulate<T, SUM, BLOCK, true>(g, pts, pb, pe, u, acc, wslots[tid / WAVE]);
         }
         double v[NRED];
 #pragma unroll
@@ -2242,13 +2278,18 @@ static int launch_align_wq_t(NdtGridDev g, const float4* pts, const ScanDesc* sc
                              uint32_t* done_cnt, double* partials, ScanOut* out, double* prof, int cooperative,
                              hipStream_t stream) {
     if (!cooperative) {
-        hipLaunchKernelGGL((k_align_wq<T, SUM, BLOCK>), dim3(nwg), dim3(BLOCK), 0, stream, g, pts, scans, prm, states, q,
-                           items, cap, done_cnt, partials, out, prof);
+        if (prm.hess_at_accept)
+            hipLaunchKernelGGL((k_align_wq<T, SUM, BLOCK, true>), dim3(nwg), dim3(BLOCK), 0, stream, g, pts, scans, prm,
+                               states, q, items, cap, done_cnt, partials, out, prof);
+        else
+            hipLaunchKernelGGL((k_align_wq<T, SUM, BLOCK, false>), dim3(nwg), dim3(BLOCK), 0, stream, g, pts, scans, prm,
+                               states, q, items, cap, done_cnt, partials, out, prof);
         return (int)hipGetLastError();
     }
     void* args[] = {&g, &pts, &scans, &prm, &states, &q, &items, &cap, &done_cnt, &partials, &out, &prof};
-    return (int)hipLaunchCooperativeKernel((const void*)k_align_wq<T, SUM, BLOCK>, dim3(nwg), dim3(BLOCK), args, 0,
-                                           stream);
+    const void* kern = prm.hess_at_accept ? (const void*)k_align_wq<T, SUM, BLOCK, true>
+                                          : (const void*)k_align_wq<T, SUM, BLOCK, false>;
+    return (int)hipLaunchCooperativeKernel(kern, dim3(nwg), dim3(BLOCK), args, 0, stream);
 }
 
 __global__ void k_zero_u32(uint32_t* __restrict__ dst, int64_t n) {
@@ -2266,7 +2307,8 @@ int launch_zero_u32(void* dst, int64_t n, hipStream_t stream) {
 
 template <typename T, int SUM, int BLOCK>
 static int max_blocks_wq_t(int* bpc) {
-    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align_wq<T, SUM, BLOCK>, BLOCK, 0);
+    return (int)hipOccupancyMaxActiveBlocksPerMultiprocessor(bpc, (const void*)k_align_wq<T, SUM, BLOCK, false>,
+                                                             BLOCK, 0);
 }
 
 int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu) {

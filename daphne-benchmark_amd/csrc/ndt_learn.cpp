@@ -113,11 +113,8 @@ struct ndt_model {
     /* batch workspace */
     int wsB = 0, splits1 = 1, splits2 = 1;
     DevBuf X, XT, P1, H1, H1T, P2, H2, Y, dY, lossb, dZ2, dZ2T, dH1, dZ1T, target, weights, lossacc, bpart;
-    /* prediction inputs; sbev holds the scan grids of the batch with serial
-     * sbev_scans on the BEV map with serial sbev_bm (they do not depend on the
-     * guesses, so repeated predictions for one batch reuse them) */
+    /* prediction inputs: the scan grids of the batch being predicted for */
     DevBuf sbev, idx, gl, off, cnt;
-    uint64_t sbev_scans = 0, sbev_bm = 0;
 };
 
 /* Drops one reference to a BEV map; the last frees it. */
@@ -815,12 +812,9 @@ static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const nd
     int e = ensure_workspace(md, Bp);
     if (e) return e;
     hipStream_t st = c->stream;
-    if (md->sbev_scans != sc->serial || md->sbev_bm != bm->serial) {
-        md->sbev_scans = md->sbev_bm = 0;
-        if ((e = scan_grids(md, bm, sc, md->sbev))) return e;
-        md->sbev_scans = sc->serial;
-        md->sbev_bm = bm->serial;
-    }
+    /* the scan grids are made at every prediction (a batch's scans may be
+     * predicted for once only, and their grids are 64 KiB each) */
+    if ((e = scan_grids(md, bm, sc, md->sbev))) return e;
     std::vector<int32_t> idx((size_t)Bp);
     std::vector<double> gl;
     guesses_local(bm, guesses, S, gl);

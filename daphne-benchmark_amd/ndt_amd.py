@@ -11,10 +11,12 @@ Synthetic inputs come from ``libndt_scene.so`` (include/ndt_scene.h).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import gzip
 import os
 import struct
+import weakref
 from collections.abc import Sequence
 from dataclasses import dataclass, field
 
@@ -386,7 +388,7 @@ class NdtParams(C.Structure):
                 ("max_iterations", C.c_int32), ("outlier_ratio", C.c_double), ("min_points", C.c_int32),
                 ("min_eig_ratio", C.c_double), ("point_precision", C.c_int32), ("sum_precision", C.c_int32),
                 ("origin", C.c_double * 3), ("block_size", C.c_int32), ("points_per_wg", C.c_int32),
-                ("scheduler", C.c_int32)]
+                ("scheduler", C.c_int32), ("hessian_at_accept", C.c_int32)]
 
 
 class NdtResult(C.Structure):
@@ -566,12 +568,39 @@ def _result(r: NdtResult) -> dict:
                 converged=bool(r.converged), passes=r.passes, status=r.status)
 
 
+# Handles still open, in creation order (weak references).  close_all() closes them
+# newest first; it runs at interpreter exit, so that no device memory, stream or
+# captured graph is left for the HIP runtime's own teardown (or a profiler attached
+# to it) to release after the library's users are gone.
+_live: list = []
+
+
+def _track(obj) -> None:
+    if len(_live) >= 4096:
+        _live[:] = [r for r in _live if r() is not None]
+    _live.append(weakref.ref(obj))
+
+
+@atexit.register
+def close_all() -> None:
+    """Closes every open handle of this module, newest first (handles first, then
+    the contexts they were made on)."""
+    while _live:
+        obj = _live.pop()()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
 class Context:
     """One HIP device context (one stream)."""
 
     def __init__(self, device: int = 0):
         self.h = C.c_void_p()
         _check(hip_lib().ndt_ctx_create(device, C.byref(self.h)))
+        _track(self)
 
     def close(self):
         if self.h:
@@ -619,6 +648,7 @@ class Grid:
         m = np.ascontiguousarray(map_xyzw, dtype=np.float32)
         self.h = C.c_void_p()
         _check(hip_lib().ndt_grid_build(ctx.h, m.ctypes.data, m.shape[0], C.byref(prm), C.byref(self.h)))
+        _track(self)
 
     def close(self):
         if self.h:
@@ -696,6 +726,7 @@ class Scans:
         self.h = C.c_void_p()
         _check(hip_lib().ndt_scans_upload(ctx.h, ptrs, self.counts.ctypes.data, S, C.byref(self.h)))
         self.S = S
+        _track(self)
 
     def close(self):
         if self.h:
@@ -796,6 +827,7 @@ class BevMap(_Handle):
         o = np.ascontiguousarray(origin, dtype=np.float64)
         self.h = C.c_void_p()
         _check(hip_lib().ndt_bevmap_build(ctx.h, m.ctypes.data, m.shape[0], o.ctypes.data, C.byref(self.h)))
+        _track(self)
 
     def features(self, scans: "Scans", scan_index, guesses) -> np.ndarray:
         """(B, 65536) uint16 bf16 bit patterns: [scan grid | map grid at the guess]."""
@@ -819,6 +851,7 @@ class Model(_Handle):
             _check(hip_lib().ndt_model_load(ctx.h, os.fsencode(path), C.byref(self.h)))
         else:
             _check(hip_lib().ndt_model_create(ctx.h, f0, f1, f2, seed, C.byref(self.h)))
+        _track(self)
         d = np.zeros(3, np.int32)
         _check(hip_lib().ndt_model_dims(self.h, d.ctypes.data))
         self.dims = tuple(int(v) for v in d)
@@ -900,6 +933,7 @@ class Dataset(_Handle):
         self.h = C.c_void_p()
         _check(hip_lib().ndt_dataset_create(ctx.h, bevmap.h, scans.h, idx.ctypes.data, g.ctypes.data, t.ctypes.data,
                                             n, C.byref(self.h)))
+        _track(self)
 
     def add(self, scans: "Scans", scan_index, guesses, targets):
         idx = np.ascontiguousarray(scan_index, dtype=np.int32)

@@ -72,6 +72,13 @@ typedef struct ndt_params {
     int32_t points_per_wg;    /* scan points per partial sum (ndt align); 0 = 2048 (batched
                                  throughput), 256 suits single-scan latency */
     int32_t scheduler;        /* NDT_SCHED_* (ndt align) */
+    int32_t hessian_at_accept; /* ndt align: 0 (default) = every derivative pass also sums the
+                                  Hessian, so the accepted step's Hessian is at hand; 1 = the
+                                  line-search loop's trial passes sum score and gradient only and
+                                  one more pass computes the Hessian at the accepted step, the
+                                  reference's order (computeDerivatives(..., false) at
+                                  kernel.cpp:1123, computeHessian at :1141-1142).  Same poses,
+                                  more passes. */
 } ndt_params;
 
 typedef struct ndt_result {
@@ -93,7 +100,8 @@ typedef struct ndt_grid_info {
     int64_t query_cells;     /* hashed lookup cells (ndt: occupied cells dilated by one) */
     int64_t candidates;      /* candidate entries of the lookup lists */
     int64_t hash_capacity;
-    int64_t dense_cells;     /* compat: cells of the reference's dense grid */
+    int64_t dense_cells;     /* compat: cells of the reference's dense grid; ndt: cells of the
+                                dense query-cell index, 0 when the index is hashed */
     int32_t dims[3];         /* compat: dense grid dimensions */
     float min_corner[3];     /* compat: minVoxel */
     float max_corner[3];     /* compat: maxVoxel */

@@ -144,10 +144,9 @@ def trained(A, ctx):
     return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg, ev=ev)
 
 
-def test_prediction_reuses_scan_grids_per_batch(A, ctx, trained):
-    """A model keeps the scan grids of the last batch it featurised: predictions for
-    the same batch with other guesses, after a prediction for another batch, equal
-    those of a fresh model loaded with the same weights."""
+def test_predictions_depend_only_on_weights_and_inputs(A, ctx, trained):
+    """Predictions for a batch, with a prediction for another batch in between,
+    equal those of a fresh model loaded with the same weights."""
     t = trained
     other = A.Scans(ctx, t["ev"][:40])
     g2 = A.perturb_poses(t["ev_gt"], np.random.default_rng(21))

@@ -53,12 +53,16 @@ def test_derivative_pass(A, orc, gpu_grid, cfg0, point, summ, tol):
 
 
 def _compare_runs(res, ref, dt_tol, dr_tol, orc):
+    """Scans whose GPU run differs from the oracle's: pose beyond the tolerances, or
+    a different iteration count, derivative-pass count (a line-search divergence
+    that lands on the same pose) or converged flag."""
     bad = []
     for i, (r, o) in enumerate(zip(res, ref)):
         dt, dr = orc.pose_error(r["final_f64"], o["final_f64"])
-        if not (dt <= dt_tol and dr <= dr_tol and r["iterations"] == o["iterations"]
+        if not (dt <= dt_tol and dr <= dr_tol and r["iterations"] == o["iterations"] and r["passes"] == o["passes"]
                 and r["converged"] == o["converged"] and r["status"] == 0):
-            bad.append((i, dt, dr, r["iterations"], o["iterations"], r["converged"], o["converged"]))
+            bad.append((i, dt, dr, r["iterations"], o["iterations"], r["passes"], o["passes"], r["converged"],
+                        o["converged"]))
     return bad
 
 
@@ -71,6 +75,29 @@ def test_align_fp64_points_matches_oracle(A, orc, gpu_grid, cfg0):
     res = [gpu_grid.align(s[0], s[2], prm=prm) for s in scans]
     ref = og.align_many([s[0] for s in scans], np.stack([s[2] for s in scans]), threads=8)
     assert _compare_runs(res, ref, 1e-3, 1e-4, orc) == []
+
+
+def test_hessian_at_accept_matches_oracle(A, orc, gpu_grid, cfg0):
+    """Line-search loop passes without the Hessian and one more pass at the accepted
+    step (the reference's order): the same iterations, passes and flags as the
+    oracle run the same way, and the same poses as the default mode (the Hessian
+    at the accepted step is the same sum either way), with exactly the extra
+    passes the oracle makes."""
+    scene, mp, og = cfg0
+    scans = [A.make_scan(scene, mp, A.scan_cfg(2000, 1.0, 3.0), 700 + s) for s in range(10)]
+    pts, gs = [s[0] for s in scans], np.stack([s[2] for s in scans])
+    res = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, hessian_at_accept=1))
+    full = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT))
+    oa = orc.NdtGrid(mp, orc.ndt_params(hessian_at_accept=1))
+    ref = oa.align_many(pts, gs, threads=8)
+    ref_full = og.align_many(pts, gs, threads=8)
+    assert _compare_runs(res, ref, 1e-3, 1e-4, orc) == []
+    extra = 0
+    for r, f, o, of in zip(res, full, ref, ref_full):
+        assert np.array_equal(r["final_f64"], f["final_f64"])
+        assert r["passes"] - f["passes"] == o["passes"] - of["passes"]
+        extra += r["passes"] - f["passes"]
+    assert extra > 0
 
 
 def test_align_fp32_precision_study(A, orc, gpu_grid, cfg0):
