@@ -656,7 +656,7 @@ static AlignParams make_align_params(const ndt_params* p, int32_t S) {
 /* Fills results from the outputs of S scans (host copy); aborted marks scans that
  * never finished as timed out. */
 static int results_from_out(const ScanOut* out, const int64_t* counts, int32_t S, int aborted, ndt_ctx* c,
-                            ndt_result* results) {
+                            ndt_result* results, int32_t* used_alt = nullptr) {
     int64_t pairs = 0;
     for (int32_t s = 0; s < S; s++) {
         ndt_result& r = results[s];
@@ -673,6 +673,7 @@ static int results_from_out(const ScanOut* out, const int64_t* counts, int32_t S
         r.status = out[s].status;
         /* a scan that never finished has no final pose */
         if (aborted && out[s].final_m[15] == 0.0) r.status = NDT_ERR_TIMEOUT;
+        if (used_alt) used_alt[s] = out[s].used_alt;
         pairs += (int64_t)out[s].pairs;
     }
     c->last_pairs = pairs;
@@ -681,7 +682,8 @@ static int results_from_out(const ScanOut* out, const int64_t* counts, int32_t S
 }
 
 /* Fills results from the device outputs of S scans in the context's buffers. */
-static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results) {
+static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int aborted, ndt_result* results,
+                           int32_t* used_alt = nullptr) {
     /* read back through the pinned staging buffer (stream order puts this copy
      * after the uploads from it) */
     const size_t bytes = sizeof(ScanOut) * (size_t)S;
@@ -690,7 +692,7 @@ static int collect_results(ndt_ctx* c, const int64_t* counts, int32_t S, int abo
     HIP_TRY(hipMemcpyAsync(c->pin, c->out.p, bytes, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->pin_pending = false;
-    return results_from_out((const ScanOut*)c->pin, counts, S, aborted, c, results);
+    return results_from_out((const ScanOut*)c->pin, counts, S, aborted, c, results, used_alt);
 }
 
 /* Chunk descriptors of a batch: chunk size, count and first partial slot per scan. */
@@ -721,8 +723,9 @@ static void make_descs(const int64_t* offsets, const int64_t* counts, int32_t S,
 
 /* Work-queue items needed by a launch: every scan's chunks for each of its
  * possible passes. */
-static uint64_t queue_capacity(int64_t total_chunks, int32_t max_iterations) {
-    return (uint64_t)total_chunks * (uint64_t)max_passes(max_iterations);
+static uint64_t queue_capacity(int64_t total_chunks, int32_t max_iterations, bool fallback = false) {
+    return (uint64_t)total_chunks *
+           (uint64_t)(fallback ? max_passes_fallback(max_iterations) : max_passes(max_iterations));
 }
 constexpr uint64_t MAX_QUEUE_ITEMS = 1ull << 28;
 
@@ -736,7 +739,7 @@ constexpr uint64_t MAX_QUEUE_ITEMS = 1ull << 28;
  * scans and is also used when the queue would exceed MAX_QUEUE_ITEMS. */
 static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const int64_t* offsets,
                         const int64_t* counts, int32_t S, const double* guesses, const ndt_params* p,
-                        ndt_result* results) {
+                        ndt_result* results, const double* alt = nullptr, int32_t* used_alt = nullptr) {
     if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
     if (S <= 0) return fail(NDT_ERR_INVALID, "no scans");
     const int block = p->block_size ? p->block_size : 256;
@@ -745,7 +748,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     int64_t slots = 0, max_chunks = 0;
     int64_t first_pass = 0;
     make_descs(offsets, counts, S, p, desc, &slots, &max_chunks, &first_pass);
-    const uint64_t cap = queue_capacity(slots, p->max_iterations);
+    const uint64_t cap = queue_capacity(slots, p->max_iterations, alt != nullptr);
     const bool queue = p->scheduler != NDT_SCHED_ROUNDS && cap <= MAX_QUEUE_ITEMS && S < (1 << ITEM_SCAN_BITS) &&
                        max_chunks < (1 << ITEM_CHUNK_BITS);
     if (p->scheduler == NDT_SCHED_QUEUE && !queue)
@@ -755,7 +758,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     else LAUNCH_TRY(align_kernel_max_blocks(block, prm.point_fp64, prm.sum_mode, &bpc));
     if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
     const int32_t nwg = bpc * c->cus;
-    BUF_TRY(c->guesses, sizeof(double) * 16 * S);
+    BUF_TRY(c->guesses, sizeof(double) * 16 * S * (alt ? 2 : 1));
     BUF_TRY(c->states, sizeof(ScanState) * S);
     BUF_TRY(c->sync, sizeof(RoundSync) * ((S + MAX_BATCH_SCANS - 1) / MAX_BATCH_SCANS));
     BUF_TRY(c->out, sizeof(ScanOut) * S);
@@ -767,7 +770,12 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         const int eu = upload_staged2(c, c->guesses.p, guesses, sizeof(double) * 16 * S, c->desc.p, desc.data(),
                                       sizeof(ScanDesc) * S);
         if (eu) return eu;
+        if (alt) {
+            const int ea = upload_staged(c, c->guesses.as<double>() + 16 * (size_t)S, alt, sizeof(double) * 16 * S);
+            if (ea) return ea;
+        }
     }
+    const double* d_alt = alt ? c->guesses.as<double>() + 16 * (size_t)S : nullptr;
     int32_t launches = 0;
     double prof_sum[4] = {0, 0, 0, 0};
     int aborted = 0;
@@ -777,8 +785,8 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         BUF_TRY(c->items, sizeof(uint64_t) * cap);
         BUF_TRY(c->done, sizeof(uint32_t) * S);
         HIP_TRY(hipMemsetAsync(c->items.p, 0, sizeof(uint64_t) * cap, c->stream));
-        LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), S, prm, c->states.as<ScanState>(), c->sync.as<RoundSync>(),
-                                      c->stream));
+        LAUNCH_TRY(launch_init_states(c->guesses.as<double>(), d_alt, S, prm, c->states.as<ScanState>(),
+                                      c->sync.as<RoundSync>(), c->stream));
         LAUNCH_TRY(launch_init_queue(c->desc.as<ScanDesc>(), S, (uint64_t)first_pass, c->queue.as<WorkQueue>(),
                                      c->items.as<uint64_t>(), c->done.as<uint32_t>(), c->stream));
         HIP_TRY(hipEventRecord(c->ev0, c->stream));
@@ -798,7 +806,8 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         for (int32_t s0 = 0; s0 < S; s0 += MAX_BATCH_SCANS) {
             const int32_t n = std::min<int32_t>(MAX_BATCH_SCANS, S - s0);
             RoundSync* rs = c->sync.as<RoundSync>() + s0 / MAX_BATCH_SCANS;
-            LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0, n, prm,
+            LAUNCH_TRY(launch_init_states(c->guesses.as<double>() + 16 * (size_t)s0,
+                                          d_alt ? d_alt + 16 * (size_t)s0 : nullptr, n, prm,
                                           c->states.as<ScanState>() + s0, rs, c->stream));
             HIP_TRY(hipEventRecord(c->ev0, c->stream));
             LAUNCH_TRY(launch_align(g->dev, pts, c->desc.as<ScanDesc>() + s0, n, nwg, block, prm,
@@ -819,7 +828,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
     c->last_ms = ms_total;
     c->last_launches = launches;
     for (int k = 0; k < 4; k++) c->last_profile[k] = prof_sum[k];
-    return collect_results(c, counts, S, aborted, results);
+    return collect_results(c, counts, S, aborted, results, used_alt);
 }
 
 /* Spreads the bits of a 10-bit value to every third bit (Morton interleave). */
@@ -1034,7 +1043,7 @@ extern "C" int ndt_align_scans_noalloc(ndt_ctx* c, const ndt_grid* g, const ndt_
     LAUNCH_TRY(launch_zero_u32(d_out, (int64_t)(sizeof(ScanOut) * (size_t)S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(d_items, (int64_t)(2 * cap), c->stream));
     LAUNCH_TRY(launch_zero_u32(d_prof, 16, c->stream));
-    LAUNCH_TRY(launch_init_states(d_guess, S, prm, d_states, d_sync, c->stream));
+    LAUNCH_TRY(launch_init_states(d_guess, nullptr, S, prm, d_states, d_sync, c->stream));
     LAUNCH_TRY(launch_init_queue(d_desc, S, (uint64_t)first_pass, d_queue, d_items, d_done, c->stream));
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), d_desc, nwg, block, prm, d_states, d_queue, d_items, cap,
@@ -1073,8 +1082,8 @@ static int capture_body(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, GraphPlan&
     HIP_TRY(hipMemcpyAsync(gp.guesses.p, gp.h_guess, sizeof(double) * 16 * S, hipMemcpyHostToDevice, c->stream));
     LAUNCH_TRY(launch_zero_u32(gp.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(gp.items.p, (int64_t)(2 * cap), c->stream));
-    LAUNCH_TRY(launch_init_states(gp.guesses.as<double>(), S, prm, gp.states.as<ScanState>(), gp.sync.as<RoundSync>(),
-                                  c->stream));
+    LAUNCH_TRY(launch_init_states(gp.guesses.as<double>(), nullptr, S, prm, gp.states.as<ScanState>(),
+                                  gp.sync.as<RoundSync>(), c->stream));
     LAUNCH_TRY(launch_init_queue(gp.desc.as<ScanDesc>(), S, (uint64_t)first_pass, gp.queue.as<WorkQueue>(),
                                  gp.items.as<uint64_t>(), gp.done.as<uint32_t>(), c->stream));
     LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), gp.desc.as<ScanDesc>(), nwg, block, prm,
@@ -1181,7 +1190,8 @@ extern "C" int ndt_align_scans(ndt_ctx* c, const ndt_grid* g, const ndt_scans* s
 
 /* Aligns the scans idx[0..n) of a device-resident batch (ndt mode). */
 int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const int32_t* idx, int32_t n,
-                           const double* guesses, const ndt_params* p, ndt_result* results) {
+                           const double* guesses, const ndt_params* p, ndt_result* results, const double* alt,
+                           int32_t* used_alt) {
     if (!c || !g || !sc || !idx || !guesses || !results) return fail(NDT_ERR_INVALID, "null argument");
     int e = check_params(p);
     if (e) return e;
@@ -1194,8 +1204,9 @@ int ndt::align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
     }
     e = check_poses(guesses, n, "guess");
     if (e) return e;
+    if (alt && (e = check_poses(alt, n, "refined guess"))) return e;
     HIP_TRY(hipSetDevice(c->device));
-    return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results);
+    return align_packed(c, g, sc->pts.as<float4>(), off.data(), cnt.data(), n, guesses, p, results, alt, used_alt);
 }
 
 /* ------------------------------------------------------------------ autotuner */

@@ -277,7 +277,13 @@ struct CompatGridDev {
 
 /* PH_MT_HESS: one more pass at an accepted step whose line-search loop ran with
  * gradient-only passes (hess_at_accept) */
-enum Phase : int32_t { PH_INIT = 0, PH_MT_FIRST = 1, PH_MT_LOOP = 2, PH_DONE = 3, PH_MT_HESS = 4 };
+/* PH_PROBE: the pass at the given guess that opens a launch with refined starts
+ * (its score decides the fallback) */
+enum Phase : int32_t { PH_INIT = 0, PH_MT_FIRST = 1, PH_MT_LOOP = 2, PH_DONE = 3, PH_MT_HESS = 4, PH_PROBE = 5 };
+/* runs of a scan in a launch with refined starts: the probe at the given guess,
+ * NDT from the refined guess, then (when that run did not converge or ended with a
+ * lower score than the given guess has) plain NDT from the given guess */
+enum FbStage : int32_t { FB_NONE = 0, FB_PROBE = 1, FB_REFINED = 2, FB_FALLBACK = 3 };
 /* the pass a scan in this phase runs needs no Hessian (with hess_at_accept) */
 NDT_HD bool grad_only_pass(int32_t phase) { return phase == PH_MT_LOOP; }
 
@@ -294,12 +300,21 @@ struct ScanState {
     double final_m[16];
     double pairs;      /* (point, voxel) pairs summed over all passes */
     double pp[32];     /* pass parameters of x_t (rotation, translation, axes, constants) */
+    double alt[6];     /* launches with refined starts: the start not being run (the refined
+                          guess during the probe, then the given guess) */
+    double probe_score; /* score at the given guess (the probe pass) */
+    int32_t fb_stage, used_alt; /* FbStage; 1 when the result is the refined run's */
     int32_t iter, converged;
     int32_t phase, done;
     int32_t step_iterations, open_interval;
     int32_t interval_converged, passes;
-    int32_t status, pad;
+    int32_t status;
+    int32_t run_base;  /* pass count at the start of the current run (chunking follows the
+                          pass number within the run, so a run in a launch with refined
+                          starts splits and sums its passes as a launch of its own does) */
 };
+/* pass number within the scan's current run */
+NDT_HD uint32_t run_pass(const ScanState& st) { return (uint32_t)(st.passes - st.run_base); }
 constexpr int STATE_WORDS = sizeof(ScanState) / 8;
 static_assert(sizeof(ScanState) % 8 == 0, "state must be whole 64-bit words");
 
@@ -375,8 +390,11 @@ NDT_HD uint64_t make_item(uint32_t pass, uint32_t scan, uint32_t chunk) {
            (uint64_t)chunk;
 }
 /* upper bound of the derivative passes of one scan: the first pass, then per
- * Newton iteration at most 1 + 10 line-search evaluations */
-NDT_HD int64_t max_passes(int32_t max_iterations) { return 1 + 11 * (int64_t)max_iterations; }
+ * Newton iteration at most 1 + 10 line-search evaluations and, with
+ * hess_at_accept, one Hessian pass; a launch with refined starts adds the probe
+ * and a second run */
+NDT_HD int64_t max_passes(int32_t max_iterations) { return 1 + 12 * (int64_t)max_iterations; }
+NDT_HD int64_t max_passes_fallback(int32_t max_iterations) { return 1 + 2 * max_passes(max_iterations); }
 
 /* per-scan output written by the device */
 struct ScanOut {
@@ -385,6 +403,7 @@ struct ScanOut {
     double pairs;
     double t_prologue, t_compute, t_barrier, t_control; /* microseconds, workgroup 0 of the scan */
     int32_t iterations, converged, passes, status;
+    int32_t used_alt, pad; /* launches with refined starts: 1 when the refined run was kept */
 };
 
 /* ---------------------------------------------------------------- launchers
@@ -447,8 +466,10 @@ int launch_align_wq(NdtGridDev g, const float4* pts, const ScanDesc* scans, int3
 /* writes n zero words (32-bit) from dst: a kernel, so that graphs can hold it */
 int launch_zero_u32(void* dst, int64_t n, hipStream_t stream);
 int align_wq_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
-int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
-                       hipStream_t stream);
+/* alt (optional, S x 16): refined starts; then every scan opens with a probe pass at
+ * its given guess, runs from the refined guess and falls back to the given guess */
+int launch_init_states(const double* guesses, const double* alt, int32_t S, AlignParams prm, ScanState* states,
+                       RoundSync* rs, hipStream_t stream);
 int align_kernel_max_blocks(int block_size, int point_fp64, int sum_mode, int* blocks_per_cu);
 constexpr int MAX_BATCH_SCANS = 2048;
 /* Scan points per partial sum of the align kernels when ndt_params.points_per_wg

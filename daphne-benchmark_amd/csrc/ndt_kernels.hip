@@ -1628,13 +1628,36 @@ __device__ __attribute__((noinline)) void ctl_finish(ScanState& st, const AlignP
     out->passes = st.passes;
     out->status = st.status;
     out->pairs = st.pairs;
+    out->used_alt = st.used_alt;
+}
+
+/* The end of a run.  In a launch with refined starts, a run from the refined guess
+ * that did not converge, or ended with a lower score than the given guess has,
+ * gives way to plain NDT from the given guess (the scan goes on with its next pass);
+ * otherwise the scan finishes. */
+__device__ inline void ctl_end(ScanState& st, const AlignParams& prm, int converged, ScanOut* out) {
+    if (st.fb_stage == FB_REFINED && !(converged && st.score >= st.probe_score)) {
+        st.fb_stage = FB_FALLBACK;
+        st.iter = 0;
+        st.converged = 0;
+        for (int k = 0; k < 6; k++) {
+            st.x_t[k] = st.alt[k];
+            st.p[k] = st.alt[k];
+        }
+        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
+        st.phase = PH_INIT;
+        st.run_base = st.passes;
+        return;
+    }
+    st.used_alt = st.fb_stage == FB_REFINED;
+    ctl_finish(st, prm, converged, out);
 }
 
 __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const AlignParams& prm, ScanOut* out) {
     const double mu = 1.e-4;
     for (;;) {
         if (st.iter >= prm.max_iterations) {
-            ctl_finish(st, prm, 0, out);
+            ctl_end(st, prm, 0, out);
             return;
         }
         double H[36], ng[6], delta[6];
@@ -1646,7 +1669,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
         double norm = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2] + delta[3] * delta[3] +
                            delta[4] * delta[4] + delta[5] * delta[5]);
         if (norm == 0 || norm != norm) {
-            ctl_finish(st, prm, norm == norm, out);
+            ctl_end(st, prm, norm == norm, out);
             return;
         }
         for (int i = 0; i < 6; i++) st.dir[i] = delta[i] / norm;
@@ -1657,7 +1680,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
                 /* no descent direction: zero step */
                 st.iter++;
                 if (st.iter > 1 && 0.0 < prm.trans_eps) {
-                    ctl_finish(st, prm, 1, out);
+                    ctl_end(st, prm, 1, out);
                     return;
                 }
                 continue;
@@ -1691,7 +1714,7 @@ __device__ inline void ctl_accept(ScanState& st, const AlignParams& prm, ScanOut
     for (int i = 0; i < 6; i++) st.p[i] = st.p[i] + st.dir[i] * st.a_t;
     st.iter++;
     if (st.iter > 1 && fabs(st.a_t) < prm.trans_eps) {
-        ctl_finish(st, prm, 1, out);
+        ctl_end(st, prm, 1, out);
         return;
     }
     ctl_newton_start(st, prm, out);
@@ -1701,6 +1724,21 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
     const double mu = 1.e-4, nu = 0.9;
     st.passes++;
     st.pairs += red[NACC];
+    if (st.phase == PH_PROBE) {
+        /* the given guess's score; the run starts at the refined guess */
+        st.probe_score = red[0];
+        for (int k = 0; k < 6; k++) {
+            const double given = st.x_t[k];
+            st.x_t[k] = st.alt[k];
+            st.p[k] = st.alt[k];
+            st.alt[k] = given;
+        }
+        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
+        st.fb_stage = FB_REFINED;
+        st.phase = PH_INIT;
+        st.run_base = st.passes;
+        return;
+    }
     if (st.phase == PH_MT_HESS) {
         /* the Hessian at the accepted step of a line search whose loop ran
          * (computeHessian, kernel.cpp:1141-1142); score and gradient are the last
@@ -1764,14 +1802,9 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
     ctl_accept(st, prm, out);
 }
 
-__global__ void k_init_states(const double* __restrict__ guesses, int32_t S, AlignParams prm, ScanState* states,
-                              RoundSync* rs) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    ScanState st;
-    uint64_t* w = (uint64_t*)&st;
-    for (int k = 0; k < STATE_WORDS; k++) w[k] = 0;
-    const double* G = guesses + 16 * s;
+/* pose parameters (x, y, z, roll, pitch, yaw) of a row-major world pose in the
+ * map-local frame */
+__device__ inline void pose_params(const double* G, const AlignParams& prm, double* x) {
     double M[16];
     for (int k = 0; k < 16; k++) M[k] = G[k];
     M[3] -= prm.origin[0];
@@ -1779,15 +1812,30 @@ __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, Ali
     M[11] -= prm.origin[2];
     double ea[3];
     pose_euler(M, ea);
-    st.x_t[0] = M[3];
-    st.x_t[1] = M[7];
-    st.x_t[2] = M[11];
-    st.x_t[3] = ea[0];
-    st.x_t[4] = ea[1];
-    st.x_t[5] = ea[2];
+    x[0] = M[3];
+    x[1] = M[7];
+    x[2] = M[11];
+    x[3] = ea[0];
+    x[4] = ea[1];
+    x[5] = ea[2];
+}
+
+__global__ void k_init_states(const double* __restrict__ guesses, const double* __restrict__ alt, int32_t S,
+                              AlignParams prm, ScanState* states, RoundSync* rs) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    ScanState st;
+    uint64_t* w = (uint64_t*)&st;
+    for (int k = 0; k < STATE_WORDS; k++) w[k] = 0;
+    pose_params(guesses + 16 * s, prm, st.x_t);
     for (int k = 0; k < 6; k++) st.p[k] = st.x_t[k];
     pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
     st.phase = PH_INIT;
+    if (alt) {
+        pose_params(alt + 16 * s, prm, st.alt);
+        st.fb_stage = FB_PROBE;
+        st.phase = PH_PROBE;
+    }
     states[s] = st;
     if (s == 0) {
         rs->count = 0;
@@ -1795,9 +1843,9 @@ __global__ void k_init_states(const double* __restrict__ guesses, int32_t S, Ali
     }
 }
 
-int launch_init_states(const double* guesses, int32_t S, AlignParams prm, ScanState* states, RoundSync* rs,
-                       hipStream_t stream) {
-    hipLaunchKernelGGL(k_init_states, dim3(grid_for(S, 64)), dim3(64), 0, stream, guesses, S, prm, states, rs);
+int launch_init_states(const double* guesses, const double* alt, int32_t S, AlignParams prm, ScanState* states,
+                       RoundSync* rs, hipStream_t stream) {
+    hipLaunchKernelGGL(k_init_states, dim3(grid_for(S, 64)), dim3(64), 0, stream, guesses, alt, S, prm, states, rs);
     return (int)hipGetLastError();
 }
 
@@ -1946,7 +1994,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         if (nactive == 0) break;
         for (int a = tid; a < (int)nactive; a += BLOCK) {
             const int s = (int)s_active[a];
-            s_prefix[a] = (uint32_t)pass_nchunks(scans[s], (uint32_t)ld_agent_i32(&states[s].passes));
+            s_prefix[a] = (uint32_t)pass_nchunks(scans[s], (uint32_t)(ld_agent_i32(&states[s].passes) -
+                                                                       ld_agent_i32(&states[s].run_base)));
         }
         __syncthreads();
         const uint32_t nchunks = lds_exclusive_scan<BLOCK>(s_prefix, (int)nactive, lw);
@@ -1983,7 +2032,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             int pairs = 0;
             {
                 const PassParams<T> u = load_uniform(pp);
-                const int64_t cs = pass_chunk(sd, (uint32_t)ld_agent_i32(&states[s].passes));
+                const int64_t cs = pass_chunk(sd, (uint32_t)(ld_agent_i32(&states[s].passes) -
+                                                            ld_agent_i32(&states[s].run_base)));
                 const int64_t pb = sd.offset + k * cs;
                 const int64_t pe = sd.offset + min((int64_t)sd.n, (k + 1) * cs);
                 if (HA && s_grad_only)
@@ -2009,7 +2059,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             const ScanDesc sd = scans[s];
             for (int q = tid; q < STATE_WORDS; q += BLOCK) sstate[q] = ld_agent_u64((const uint64_t*)(states + s) + q);
             __syncthreads();
-            reduce_slots(partials, sd.chunk_base, pass_nchunks(sd, (uint32_t)((ScanState*)sstate)->passes), gred, red);
+            reduce_slots(partials, sd.chunk_base, pass_nchunks(sd, run_pass(*(ScanState*)sstate)), gred, red);
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
             __syncthreads();
             for (int q = tid; q < STATE_WORDS; q += BLOCK) st_agent_u64((uint64_t*)(states + s) + q, sstate[q]);
@@ -2146,6 +2196,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ uint64_t s_item, s_ppkey;
     __shared__ uint32_t s_base;
     __shared__ int s_last, s_done, s_grad_only;
+    __shared__ uint32_t s_run_base, s_next_rel;
     const int tid = threadIdx.x;
     double prof[3] = {0, 0, 0};
     int ok = 1;
@@ -2183,7 +2234,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         const ScanDesc sd = scans[s];
         if (ppkey != s_ppkey) {
             if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
-            if (tid == 0) s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
+            if (tid == 0) {
+                s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
+                s_run_base = (uint32_t)ld_agent_i32(&states[s].run_base);
+            }
             __syncthreads();
             if (tid == 0) s_ppkey = ppkey;
             __syncthreads();
@@ -2191,10 +2245,11 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         Acc<T, SUM> acc;
         acc.zero();
         int pairs = 0;
-        const int32_t nck = pass_nchunks(sd, pass);
+        const uint32_t rel = pass - s_run_base; /* pass number within the scan's current run */
+        const int32_t nck = pass_nchunks(sd, rel);
         {
             const PassParams<T> u = load_uniform(pp);
-            const int64_t cs = pass_chunk(sd, pass);
+            const int64_t cs = pass_chunk(sd, rel);
             const int64_t pb = sd.offset + (int64_t)k * cs;
             const int64_t pe = sd.offset + min((int64_t)sd.n, (int64_t)(k + 1) * cs);
             if (HA && s_grad_only)
@@ -2229,13 +2284,14 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
                     ctl_finish(st, prm, 0, out + s);
                 }
                 s_done = st.done;
+                s_next_rel = run_pass(st);
             }
             __syncthreads();
             for (int w = tid; w < STATE_WORDS; w += BLOCK) st_agent_u64((uint64_t*)(states + s) + w, sstate[w]);
             stores_complete();
             __syncthreads();
             if (!s_done) {
-                const int32_t nnext = pass_nchunks(sd, pass + 1);
+                const int32_t nnext = pass_nchunks(sd, s_next_rel);
                 if (tid == 0)
                     s_base = __hip_atomic_fetch_add(&q->tail, (uint32_t)nnext, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT);

@@ -908,34 +908,16 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
             if (used_learned) used_learned[s] = learned ? 1 : 0;
         }
     } else {
-        /* NDT score at the given guesses (one pass) */
-        ndt_params probe = *p;
-        probe.max_iterations = 0;
-        std::vector<ndt_result> r0((size_t)S);
-        if ((e = ndt_align_scans(c, g, sc, guesses, &probe, r0.data()))) return e;
+        /* one launch: per scan a probe pass at the given guess, NDT from the refined
+         * guess and, where that run did not converge or ended below the given
+         * guess's score, plain NDT from the given guess */
+        std::vector<int32_t> idx((size_t)S), used((size_t)S);
+        for (int s = 0; s < S; s++) idx[(size_t)s] = s;
+        if ((e = align_scan_subset(c, g, sc, idx.data(), S, guesses, p, results, refined.data(), used.data())))
+            return e;
         account();
-        if ((e = ndt_align_scans(c, g, sc, refined.data(), p, results))) return e;
-        account();
-        std::vector<int32_t> fb;
-        for (int s = 0; s < S; s++) {
-            const bool keep = results[s].converged && results[s].score >= r0[(size_t)s].score;
-            if (used_learned) used_learned[s] = keep ? 1 : 0;
-            results[s].passes += 1; /* the probe pass */
-            if (!keep) fb.push_back(s);
-        }
-        if (!fb.empty()) {
-            std::vector<double> fg(fb.size() * 16);
-            for (size_t k = 0; k < fb.size(); k++)
-                std::memcpy(&fg[16 * k], guesses + 16 * (size_t)fb[k], 16 * sizeof(double));
-            std::vector<ndt_result> rf(fb.size());
-            if ((e = align_scan_subset(c, g, sc, fb.data(), (int32_t)fb.size(), fg.data(), p, rf.data()))) return e;
-            account();
-            for (size_t k = 0; k < fb.size(); k++) {
-                const int32_t spent = results[fb[k]].passes;
-                results[fb[k]] = rf[k];
-                results[fb[k]].passes += spent;
-            }
-        }
+        if (used_learned)
+            for (int s = 0; s < S; s++) used_learned[s] = used[(size_t)s];
     }
     c->last_ms = ms_total;
     c->last_launches = launches;

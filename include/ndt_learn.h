@@ -62,7 +62,13 @@ typedef struct ndt_train_params {
  *                        result is never worse, by the NDT score, than plain NDT;
  *   NDT_LEARNED_FALLBACK NDT runs from the refined guess; plain NDT from the given
  *                        guess replaces it where it did not converge or ended with
- *                        a lower score than the given guess itself has.
+ *                        a lower score than the given guess itself has.  All of it
+ *                        runs in one launch: each scan opens with one pass at its
+ *                        given guess (that score), runs from the refined guess and,
+ *                        when it falls back, restarts from the given guess while
+ *                        the other scans go on; each run splits and sums its passes
+ *                        as a launch of its own would, so every result equals, bit
+ *                        for bit, ndt_align_scans from one of the two guesses.
  * Either may be combined (bitwise or) with
  *   NDT_LEARNED_KEEP_ROTATION  the refined guess takes only the model's translation
  *                        and keeps the given guess's rotation. */

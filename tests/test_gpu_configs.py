@@ -95,8 +95,10 @@ def test_cfg1_sequential_scans_match_oracle(A, orc, ctx):
 
 def test_cfg2_batch_matches_oracle_sample(A, orc, cfg2):
     """The whole 512-scan batch in one launch; on the 64-scan oracle sample every
-    reproducible scan within 1 cm / 0.05 deg (and 1e-3 m / 1e-4 rad with the same
-    iterations and flag on most), and no fewer converged runs than the oracle's."""
+    reproducible scan within 1 cm / 0.05 deg and with the oracle's converged flag (and
+    1e-3 m / 1e-4 rad with the same iterations on most of them).  On the other scans
+    the oracle's own flag depends on the last bit of its guess, so their flags are
+    only reported."""
     w, prm, grid, scans, idx, ref, stable = cfg2
     res_all = scans.align(grid, w.guesses, prm)
     assert len(res_all) == 512 and all(r["status"] == 0 for r in res_all)
@@ -111,7 +113,10 @@ def test_cfg2_batch_matches_oracle_sample(A, orc, cfg2):
     assert stable.sum() >= SAMPLE // 2
     assert metric[stable].all(), np.nonzero(stable & ~metric)[0]
     assert strict[stable].mean() >= 0.9
-    assert sum(r["converged"] for r in res) >= sum(o["converged"] for o in ref)
+    assert all(res[k]["converged"] == ref[k]["converged"] for k in np.nonzero(stable)[0])
+    print(f"cfg2: converged {sum(r['converged'] for r in res)} of {SAMPLE} sampled scans (oracle "
+          f"{sum(o['converged'] for o in ref)}), {np.mean([r['converged'] for r in res_all]):.3f} of the batch",
+          flush=True)
 
 
 @pytest.mark.parametrize("mode", [("fp64", 1, 2), ("fp32/fp32", 0, 0), ("fp32/comp32", 0, 1), ("fp32/fp64", 0, 2)],
