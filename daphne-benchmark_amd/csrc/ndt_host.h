@@ -101,9 +101,10 @@ using ndt::DevBuf;
 namespace ndt {
 /* aligns the scans idx[0..n) of a device-resident batch from guesses[0..n) (ndt mode);
  * with alt (refined starts, n x 16) in one launch: a probe pass at each given guess,
- * NDT from the refined guess, and plain NDT from the given guess where the refined
- * run did not converge or ended with a lower score than the given guess has
- * (used_alt[k] = 1 where the refined run is the result; passes count all three) */
+ * NDT from the refined guess where it scores at least as high as the given guess,
+ * and plain NDT from the given guess otherwise or where the refined run ended with
+ * a lower score than the given guess has (used_alt[k] = 1 where the refined run is
+ * the result; passes count every pass spent) */
 int align_scan_subset(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const int32_t* idx, int32_t n,
                       const double* guesses, const ndt_params* p, ndt_result* results, const double* alt = nullptr,
                       int32_t* used_alt = nullptr);

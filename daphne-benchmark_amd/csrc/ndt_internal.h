@@ -281,8 +281,9 @@ struct CompatGridDev {
  * (its score decides the fallback) */
 enum Phase : int32_t { PH_INIT = 0, PH_MT_FIRST = 1, PH_MT_LOOP = 2, PH_DONE = 3, PH_MT_HESS = 4, PH_PROBE = 5 };
 /* runs of a scan in a launch with refined starts: the probe at the given guess,
- * NDT from the refined guess, then (when that run did not converge or ended with a
- * lower score than the given guess has) plain NDT from the given guess */
+ * NDT from the refined guess (when it scores at least as high there), then plain
+ * NDT from the given guess (when it does not, or when the refined run ended with
+ * a lower score than the given guess has) */
 enum FbStage : int32_t { FB_NONE = 0, FB_PROBE = 1, FB_REFINED = 2, FB_FALLBACK = 3 };
 /* the pass a scan in this phase runs needs no Hessian (with hess_at_accept) */
 NDT_HD bool grad_only_pass(int32_t phase) { return phase == PH_MT_LOOP; }
@@ -303,6 +304,8 @@ struct ScanState {
     double alt[6];     /* launches with refined starts: the start not being run (the refined
                           guess during the probe, then the given guess) */
     double probe_score; /* score at the given guess (the probe pass) */
+    double probe_d[27];  /* gradient and Hessian there: a run from the given guess starts
+                            from them (the probe is its first pass) */
     int32_t fb_stage, used_alt; /* FbStage; 1 when the result is the refined run's */
     int32_t iter, converged;
     int32_t phase, done;

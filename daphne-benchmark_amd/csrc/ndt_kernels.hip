@@ -1631,33 +1631,47 @@ __device__ __attribute__((noinline)) void ctl_finish(ScanState& st, const AlignP
     out->used_alt = st.used_alt;
 }
 
+/* Plain NDT from the given guess in a launch with refined starts.  The probe pass
+ * was at the given guess, so it is the run's first pass: the run starts from its
+ * score, gradient and Hessian, and its later passes are numbered (and chunked) as
+ * those of a launch of its own. */
+__device__ inline void ctl_fallback(ScanState& st) {
+    st.fb_stage = FB_FALLBACK;
+    st.used_alt = 0;
+    st.iter = 0;
+    st.converged = 0;
+    for (int k = 0; k < 6; k++) {
+        st.x_t[k] = st.alt[k];
+        st.p[k] = st.alt[k];
+    }
+    st.score = st.probe_score;
+    for (int i = 0; i < 6; i++) st.grad[i] = st.probe_d[i];
+    for (int k = 0; k < 21; k++) st.hess[k] = st.probe_d[6 + k];
+    st.run_base = st.passes - 1;
+}
+
 /* The end of a run.  In a launch with refined starts, a run from the refined guess
- * that did not converge, or ended with a lower score than the given guess has,
- * gives way to plain NDT from the given guess (the scan goes on with its next pass);
- * otherwise the scan finishes. */
-__device__ inline void ctl_end(ScanState& st, const AlignParams& prm, int converged, ScanOut* out) {
-    if (st.fb_stage == FB_REFINED && !(converged && st.score >= st.probe_score)) {
-        st.fb_stage = FB_FALLBACK;
-        st.iter = 0;
-        st.converged = 0;
-        for (int k = 0; k < 6; k++) {
-            st.x_t[k] = st.alt[k];
-            st.p[k] = st.alt[k];
-        }
-        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
-        st.phase = PH_INIT;
-        st.run_base = st.passes;
-        return;
+ * that ended with a lower score than the given guess has gives way to plain NDT
+ * from the given guess: returns 1, and the caller starts a Newton iteration from
+ * the state.  Otherwise the scan finishes (returns 0).  The test is the score
+ * alone: a run that used all its iterations without converging but ended above
+ * the given guess's score is kept (restarting it would put a second full run
+ * behind the batch's longest ones). */
+__device__ inline int ctl_end(ScanState& st, const AlignParams& prm, int converged, ScanOut* out) {
+    if (st.fb_stage == FB_REFINED && st.score < st.probe_score) {
+        ctl_fallback(st);
+        return 1;
     }
     st.used_alt = st.fb_stage == FB_REFINED;
     ctl_finish(st, prm, converged, out);
+    return 0;
 }
 
 __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const AlignParams& prm, ScanOut* out) {
     const double mu = 1.e-4;
     for (;;) {
         if (st.iter >= prm.max_iterations) {
-            ctl_end(st, prm, 0, out);
+            if (ctl_end(st, prm, 0, out)) continue;
             return;
         }
         double H[36], ng[6], delta[6];
@@ -1669,7 +1683,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
         double norm = sqrt(delta[0] * delta[0] + delta[1] * delta[1] + delta[2] * delta[2] + delta[3] * delta[3] +
                            delta[4] * delta[4] + delta[5] * delta[5]);
         if (norm == 0 || norm != norm) {
-            ctl_end(st, prm, norm == norm, out);
+            if (ctl_end(st, prm, norm == norm, out)) continue;
             return;
         }
         for (int i = 0; i < 6; i++) st.dir[i] = delta[i] / norm;
@@ -1680,7 +1694,7 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const 
                 /* no descent direction: zero step */
                 st.iter++;
                 if (st.iter > 1 && 0.0 < prm.trans_eps) {
-                    ctl_end(st, prm, 1, out);
+                    if (ctl_end(st, prm, 1, out)) continue;
                     return;
                 }
                 continue;
@@ -1714,8 +1728,7 @@ __device__ inline void ctl_accept(ScanState& st, const AlignParams& prm, ScanOut
     for (int i = 0; i < 6; i++) st.p[i] = st.p[i] + st.dir[i] * st.a_t;
     st.iter++;
     if (st.iter > 1 && fabs(st.a_t) < prm.trans_eps) {
-        ctl_end(st, prm, 1, out);
-        return;
+        if (!ctl_end(st, prm, 1, out)) return;
     }
     ctl_newton_start(st, prm, out);
 }
@@ -1725,8 +1738,10 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
     st.passes++;
     st.pairs += red[NACC];
     if (st.phase == PH_PROBE) {
-        /* the given guess's score; the run starts at the refined guess */
+        /* the given guess's score and derivatives; the run starts at the refined
+         * guess */
         st.probe_score = red[0];
+        for (int k = 0; k < 27; k++) st.probe_d[k] = red[1 + k];
         for (int k = 0; k < 6; k++) {
             const double given = st.x_t[k];
             st.x_t[k] = st.alt[k];
@@ -1748,6 +1763,13 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* 
         return;
     }
     if (st.phase == PH_INIT) {
+        if (st.fb_stage == FB_REFINED && red[0] < st.probe_score) {
+            /* NDT scores the refined guess below the given one: the scan runs from
+             * the given guess instead */
+            ctl_fallback(st);
+            ctl_newton_start(st, prm, out);
+            return;
+        }
         st.score = red[0];
         for (int i = 0; i < 6; i++) st.grad[i] = red[1 + i];
         for (int k = 0; k < 21; k++) st.hess[k] = red[7 + k];

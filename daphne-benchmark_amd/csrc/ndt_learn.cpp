@@ -909,8 +909,8 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
         }
     } else {
         /* one launch: per scan a probe pass at the given guess, NDT from the refined
-         * guess and, where that run did not converge or ended below the given
-         * guess's score, plain NDT from the given guess */
+         * guess where it scores at least as high, and plain NDT from the given guess
+         * otherwise or where the refined run ended below the given guess's score */
         std::vector<int32_t> idx((size_t)S), used((size_t)S);
         for (int s = 0; s < S; s++) idx[(size_t)s] = s;
         if ((e = align_scan_subset(c, g, sc, idx.data(), S, guesses, p, results, refined.data(), used.data())))

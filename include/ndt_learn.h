@@ -60,15 +60,18 @@ typedef struct ndt_train_params {
  *                        batch; the run with the better final score is kept (a
  *                        converged run beats one that did not converge), so the
  *                        result is never worse, by the NDT score, than plain NDT;
- *   NDT_LEARNED_FALLBACK NDT runs from the refined guess; plain NDT from the given
- *                        guess replaces it where it did not converge or ended with
- *                        a lower score than the given guess itself has.  All of it
- *                        runs in one launch: each scan opens with one pass at its
- *                        given guess (that score), runs from the refined guess and,
- *                        when it falls back, restarts from the given guess while
- *                        the other scans go on; each run splits and sums its passes
- *                        as a launch of its own would, so every result equals, bit
- *                        for bit, ndt_align_scans from one of the two guesses.
+ *   NDT_LEARNED_FALLBACK NDT judges the refined guess: it runs from the refined guess
+ *                        when the NDT score there is at least the given guess's, and
+ *                        plain NDT from the given guess replaces that run where it
+ *                        ended with a lower score than the given guess has;
+ *                        otherwise plain NDT runs from the given guess.
+ *                        All of it runs in one launch: each scan opens with one pass
+ *                        at its given guess (which is also the first pass of a plain
+ *                        run from there), one at the refined guess, and continues
+ *                        with the chosen run while the other scans go on; each run
+ *                        splits and sums its passes as a launch of its own would, so
+ *                        every result equals, bit for bit, ndt_align_scans from one
+ *                        of the two guesses (passes count every pass spent).
  * Either may be combined (bitwise or) with
  *   NDT_LEARNED_KEEP_ROTATION  the refined guess takes only the model's translation
  *                        and keeps the given guess's rotation. */
