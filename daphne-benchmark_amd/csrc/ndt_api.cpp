@@ -1064,6 +1064,72 @@ extern "C" int ndt_align_scans_noalloc(ndt_ctx* c, const ndt_grid* g, const ndt_
     return 0;
 }
 
+/* ------------------------------------------------------------------ submissions for callers' graphs */
+
+int ndt::align_sub_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const ndt_params* p, bool fallback,
+                           AlignSub& a) {
+    if (g->params.mode != NDT_MODE_NDT) return fail(NDT_ERR_INVALID, "grid was not built in ndt mode");
+    const int32_t S = sc->S;
+    std::vector<ScanDesc> desc;
+    int64_t slots = 0, max_chunks = 0, first_pass = 0;
+    make_descs(sc->offsets.data(), sc->counts.data(), S, p, desc, &slots, &max_chunks, &first_pass);
+    const uint64_t cap = queue_capacity(slots, p->max_iterations, fallback);
+    if (p->scheduler == NDT_SCHED_ROUNDS || cap > MAX_QUEUE_ITEMS || S >= (1 << ITEM_SCAN_BITS) ||
+        max_chunks >= (1 << ITEM_CHUNK_BITS))
+        return NDT_ERR_UNSUPPORTED;
+    a.block = p->block_size ? p->block_size : 256;
+    a.prm = make_align_params(p, S);
+    int bpc = 0;
+    LAUNCH_TRY(align_wq_max_blocks(a.block, a.prm.point_fp64, a.prm.sum_mode, &bpc));
+    if (bpc <= 0) return fail(NDT_ERR_UNSUPPORTED, "align kernel does not fit on a compute unit");
+    a.nwg = bpc * c->cus;
+    a.S = S;
+    a.cap = cap;
+    a.first_pass = first_pass;
+    a.fallback = fallback;
+    BUF_TRY(a.desc, sizeof(ScanDesc) * S);
+    BUF_TRY(a.states, sizeof(ScanState) * S);
+    BUF_TRY(a.sync, sizeof(RoundSync));
+    BUF_TRY(a.out, sizeof(ScanOut) * S);
+    BUF_TRY(a.partials, sizeof(double) * 32 * slots);
+    BUF_TRY(a.queue, sizeof(WorkQueue));
+    BUF_TRY(a.items, sizeof(uint64_t) * cap);
+    BUF_TRY(a.done, sizeof(uint32_t) * S);
+    BUF_TRY(a.prof, sizeof(double) * 8);
+    a.release_host();
+    HIP_TRY(hipHostMalloc(&a.h_out, sizeof(ScanOut) * S, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&a.h_prof, sizeof(double) * 8, hipHostMallocDefault));
+    HIP_TRY(hipMemcpyAsync(a.desc.p, desc.data(), sizeof(ScanDesc) * S, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int ndt::align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, AlignSub& a, const double* d_guesses,
+                           const double* d_alt) {
+    const int32_t S = a.S;
+    LAUNCH_TRY(launch_zero_u32(a.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
+    LAUNCH_TRY(launch_zero_u32(a.items.p, (int64_t)(2 * a.cap), c->stream));
+    LAUNCH_TRY(launch_zero_u32(a.prof.p, 16, c->stream));
+    LAUNCH_TRY(launch_init_states(d_guesses, a.fallback ? d_alt : nullptr, S, a.prm, a.states.as<ScanState>(),
+                                  a.sync.as<RoundSync>(), c->stream));
+    LAUNCH_TRY(launch_init_queue(a.desc.as<ScanDesc>(), S, (uint64_t)a.first_pass, a.queue.as<WorkQueue>(),
+                                 a.items.as<uint64_t>(), a.done.as<uint32_t>(), c->stream));
+    LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), a.desc.as<ScanDesc>(), a.nwg, a.block, a.prm,
+                               a.states.as<ScanState>(), a.queue.as<WorkQueue>(), a.items.as<uint64_t>(), a.cap,
+                               a.done.as<uint32_t>(), a.partials.as<double>(), a.out.as<ScanOut>(),
+                               a.prof.as<double>(), 0, c->stream));
+    HIP_TRY(hipMemcpyAsync(a.h_out, a.out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(a.h_prof, a.prof.p, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    return 0;
+}
+
+int ndt::align_sub_results(ndt_ctx* c, const ndt_scans* sc, const AlignSub& a, ndt_result* results,
+                           int32_t* used_alt) {
+    for (int k = 0; k < 4; k++) c->last_profile[k] = a.h_prof[k];
+    const int aborted = a.h_prof[4] != 0.0;
+    return results_from_out((const ScanOut*)a.h_out, sc->counts.data(), a.S, aborted, c, results, used_alt);
+}
+
 /* ------------------------------------------------------------------ captured submission */
 
 static bool same_align_params(const ndt_params& a, const ndt_params& b) {
@@ -1074,6 +1140,9 @@ static bool same_align_params(const ndt_params& a, const ndt_params& b) {
            a.block_size == b.block_size && a.points_per_wg == b.points_per_wg && a.scheduler == b.scheduler &&
            a.hessian_at_accept == b.hessian_at_accept;
 }
+
+bool ndt::same_params(const ndt_params& a, const ndt_params& b) { return same_align_params(a, b); }
+uint64_t ndt::wait_limit() { return wait_limit_ticks(); }
 
 /* the captured launch sequence; the caller ends the capture whatever this returns */
 static int capture_body(ndt_ctx* c, const ndt_grid* g, ndt_scans* sc, GraphPlan& gp, const AlignParams& prm,

@@ -225,6 +225,45 @@ struct GraphPlan {
     ~GraphPlan() { release(); }
 };
 
+namespace ndt {
+/* The buffers of one alignment submission of a device-resident batch in the work
+ * queue, for a caller that records the submission in its own graph: device
+ * workspace, the chunk descriptors (uploaded by align_sub_prepare) and pinned
+ * host memory for the results. */
+struct AlignSub {
+    DevBuf desc, states, sync, out, partials, queue, items, done, prof;
+    void* h_out = nullptr;
+    double* h_prof = nullptr;
+    AlignParams prm{};
+    uint64_t cap = 0;
+    int64_t first_pass = 0;
+    int32_t nwg = 0, block = 256, S = 0;
+    bool fallback = false;
+    void release_host() {
+        if (h_out) (void)hipHostFree(h_out);
+        if (h_prof) (void)hipHostFree(h_prof);
+        h_out = nullptr;
+        h_prof = nullptr;
+    }
+    ~AlignSub() { release_host(); }
+};
+/* Sizes and allocates a submission for the whole batch (fallback: with refined
+ * starts); NDT_ERR_UNSUPPORTED when the batch needs the round scheduler. */
+int align_sub_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const ndt_params* p, bool fallback,
+                      AlignSub& a);
+/* Enqueues the submission on the context's stream without synchronising (it may
+ * be recorded into a graph): set-up kernels, the work-queue launch from the
+ * device-resident guesses (and refined starts), results to the pinned buffers. */
+int align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, AlignSub& a, const double* d_guesses,
+                      const double* d_alt);
+/* Results of a completed submission (after the stream was synchronised). */
+int align_sub_results(ndt_ctx* c, const ndt_scans* sc, const AlignSub& a, ndt_result* results, int32_t* used_alt);
+/* parameters that give the same alignment launch */
+bool same_params(const ndt_params& a, const ndt_params& b);
+/* bound of a device-side wait in s_memrealtime ticks (NDT_WAIT_LIMIT_US or 2 s) */
+uint64_t wait_limit();
+} // namespace ndt
+
 struct ndt_scans {
     ndt::CtxRef ctx;
     uint64_t serial = 0; /* unique per uploaded batch (derived data cached elsewhere refers to it) */

@@ -113,8 +113,32 @@ struct ndt_model {
     /* batch workspace */
     int wsB = 0, splits1 = 1, splits2 = 1;
     DevBuf X, XT, P1, H1, H1T, P2, H2, Y, dY, lossb, dZ2, dZ2T, dH1, dZ1T, target, weights, lossacc, bpart;
-    /* prediction inputs: the scan grids of the batch being predicted for */
-    DevBuf sbev, idx, gl, off, cnt;
+    /* prediction inputs: the scan grids of the batch being predicted for, its scan
+     * offsets and counts, the feature rows' scan indices, the world and map-local
+     * guesses, and the refined guesses */
+    DevBuf sbev, idx, gl, off, cnt, gw, refined;
+    /* the captured learned step of the last (scans, grid, BEV map, parameters) */
+    struct LearnedPlan {
+        bool valid = false;
+        uint64_t scans = 0, grid = 0, bm = 0, wait = 0;
+        int device = -1, keep = 0;
+        ndt_params params{};
+        std::vector<const void*> bufs; /* workspace the graph refers to */
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        double* h_guess = nullptr;
+        ndt::AlignSub sub;
+        void release() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (h_guess) (void)hipHostFree(h_guess);
+            exec = nullptr;
+            graph = nullptr;
+            h_guess = nullptr;
+            valid = false;
+        }
+        ~LearnedPlan() { release(); }
+    } plan;
 };
 
 /* Drops one reference to a BEV map; the last frees it. */
@@ -804,45 +828,136 @@ extern "C" int ndt_model_train(ndt_ctx* c, ndt_model* md, const ndt_dataset* dsc
 
 /* ------------------------------------------------------------------ prediction */
 
-static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const ndt_scans* sc, const double* guesses,
-                       double* out_guesses) {
+static int predict_rows(int S) { return ((S + 127) / 128) * 128; }
+
+/* Sets up the prediction inputs of a batch: workspace for its feature rows, the
+ * scans' offsets and counts, the rows' scan indices (rows past S repeat scan 0). */
+static int predict_prepare(ndt_model* md, const ndt_scans* sc) {
     if (md->f0 != NDT_FEATURES) return fail(NDT_ERR_INVALID, "prediction from features needs f0 = 65536");
-    const int S = sc->S;
-    const int Bp = ((S + 127) / 128) * 128;
+    const int S = sc->S, Bp = predict_rows(S);
     int e = ensure_workspace(md, Bp);
     if (e) return e;
-    hipStream_t st = c->stream;
-    /* the scan grids are made at every prediction (a batch's scans may be
-     * predicted for once only, and their grids are 64 KiB each) */
-    if ((e = scan_grids(md, bm, sc, md->sbev))) return e;
-    std::vector<int32_t> idx((size_t)Bp);
-    std::vector<double> gl;
-    guesses_local(bm, guesses, S, gl);
-    gl.resize((size_t)(16 * Bp));
-    for (int b = 0; b < Bp; b++) {
-        idx[(size_t)b] = b < S ? b : 0;
-        if (b >= S) std::memcpy(&gl[(size_t)(16 * b)], &gl[0], 16 * sizeof(double));
-    }
+    hipStream_t st = md->ctx->stream;
+    BUF_TRY(md->sbev, 2 * (size_t)S * BEV_CELLS);
+    BUF_TRY(md->off, 8 * (size_t)S);
+    BUF_TRY(md->cnt, 4 * (size_t)S);
     BUF_TRY(md->idx, 4 * (size_t)Bp);
-    BUF_TRY(md->gl, 8 * gl.size());
-    HIP_TRY(hipMemcpyAsync(md->idx.p, idx.data(), 4 * idx.size(), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(md->gl.p, gl.data(), 8 * gl.size(), hipMemcpyHostToDevice, st));
-    if ((e = batch_features(md, bm, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(), Bp))) return e;
-    if ((e = forward_dev(md, Bp, false))) return e;
-    std::vector<float> y((size_t)(Bp * NOUT));
-    HIP_TRY(hipMemcpyAsync(y.data(), md->Y.p, 4 * y.size(), hipMemcpyDeviceToHost, st));
+    BUF_TRY(md->gl, 8 * 16 * (size_t)Bp);
+    BUF_TRY(md->gw, 8 * 16 * (size_t)S);
+    BUF_TRY(md->refined, 8 * 16 * (size_t)S);
+    std::vector<int32_t> cnt((size_t)S), idx((size_t)Bp);
+    for (int32_t k = 0; k < S; k++) cnt[(size_t)k] = (int32_t)sc->counts[(size_t)k];
+    for (int b = 0; b < Bp; b++) idx[(size_t)b] = b < S ? b : 0;
+    HIP_TRY(hipMemcpyAsync(md->off.p, sc->offsets.data(), 8 * (size_t)S, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(md->cnt.p, cnt.data(), 4 * (size_t)S, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(md->idx.p, idx.data(), 4 * (size_t)Bp, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
-    for (int s = 0; s < S; s++) {
-        double p6[6], C[16];
-        for (int k = 0; k < 6; k++) {
-            double v = (double)y[(size_t)(s * NOUT + k)];
-            if (!std::isfinite(v)) v = 0.0;
-            p6[k] = v * CORR_SCALE[k];
-        }
-        pose_matrix(p6, C);
-        mat4_mul(guesses + 16 * s, C, out_guesses + 16 * s);
-    }
     return 0;
+}
+
+/* The prediction of a prepared batch from the world guesses in md->gw: scan grids,
+ * map grids at the guesses, the forward pass, refined guesses into md->refined.
+ * Kernels only, no synchronisation: a graph may record it. */
+static int predict_enqueue(ndt_model* md, const ndt_bevmap* bm, const ndt_scans* sc, int keep_rotation) {
+    hipStream_t st = md->ctx->stream;
+    const int S = sc->S, Bp = predict_rows(S);
+    LAUNCH_TRY(launch_bev_scan(sc->pts.as<float4>(), md->off.as<int64_t>(), md->cnt.as<int32_t>(), S,
+                               bm->lut.as<uint16_t>(), md->sbev.as<uint16_t>(), st));
+    LAUNCH_TRY(launch_guesses_local(md->gw.as<double>(), S, Bp, bm->origin[0], bm->origin[1], bm->origin[2],
+                                    md->gl.as<double>(), st));
+    int e = batch_features(md, bm, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(), Bp);
+    if (e) return e;
+    if ((e = forward_dev(md, Bp, false))) return e;
+    CorrScale scale;
+    for (int k = 0; k < NOUT; k++) scale.v[k] = CORR_SCALE[k];
+    LAUNCH_TRY(launch_refine(md->Y.as<float>(), md->gw.as<double>(), S, scale, keep_rotation,
+                             md->refined.as<double>(), st));
+    return 0;
+}
+
+/* Refined guesses of a batch into host memory (the scan grids are made at every
+ * prediction: a batch's scans may be predicted for once only). */
+static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const ndt_scans* sc, const double* guesses,
+                       double* out_guesses, int keep_rotation = 0) {
+    int e = predict_prepare(md, sc);
+    if (e) return e;
+    const int S = sc->S;
+    HIP_TRY(hipMemcpyAsync(md->gw.p, guesses, 8 * 16 * (size_t)S, hipMemcpyHostToDevice, c->stream));
+    if ((e = predict_enqueue(md, bm, sc, keep_rotation))) return e;
+    HIP_TRY(hipMemcpyAsync(out_guesses, md->refined.p, 8 * 16 * (size_t)S, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+/* The workspace a captured learned step refers to. */
+static std::vector<const void*> plan_bufs(const ndt_model* md) {
+    return {md->X.p, md->P1.p, md->H1.p, md->P2.p, md->H2.p, md->Y.p, md->wb1.p, md->wb2.p, md->wb3.p,
+            md->w[NDT_B1].p, md->w[NDT_B2].p, md->w[NDT_B3].p, md->sbev.p, md->off.p, md->cnt.p, md->idx.p,
+            md->gl.p, md->gw.p, md->refined.p, (const void*)(intptr_t)md->splits1, (const void*)(intptr_t)md->splits2};
+}
+
+/* Captures the learned step with the fallback policy for a batch (once per scans,
+ * grid, BEV map, parameters and rotation choice, and again when the model's
+ * workspace moved): the guesses in from pinned memory, the prediction, the refined
+ * guesses, the set-up kernels and the work-queue launch with refined starts, the
+ * results out to pinned memory.  NDT_ERR_UNSUPPORTED when the batch needs the
+ * round scheduler. */
+static int learned_plan_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap* bm, ndt_model* md,
+                                const ndt_scans* sc, const ndt_params* p, int keep) {
+    auto& lp = md->plan;
+    if (lp.valid && lp.scans == sc->serial && lp.grid == g->serial && lp.bm == bm->serial && lp.device == c->device &&
+        lp.keep == keep && ndt::same_params(lp.params, *p) && lp.wait == ndt::wait_limit() && lp.bufs == plan_bufs(md))
+        return 0;
+    lp.release();
+    int e = predict_prepare(md, sc);
+    if (e) return e;
+    if ((e = ndt::align_sub_prepare(c, g, sc, p, true, lp.sub))) return e;
+    HIP_TRY(hipHostMalloc((void**)&lp.h_guess, 8 * 16 * (size_t)sc->S, hipHostMallocDefault));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t he = hipMemcpyAsync(md->gw.p, lp.h_guess, 8 * 16 * (size_t)sc->S, hipMemcpyHostToDevice, c->stream);
+    e = he == hipSuccess ? 0 : fail(NDT_ERR_HIP, std::string("guess upload: ") + hipGetErrorString(he));
+    if (!e) e = predict_enqueue(md, bm, sc, keep);
+    if (!e) e = ndt::align_sub_enqueue(c, g, sc, lp.sub, md->gw.as<double>(), md->refined.as<double>());
+    hipGraph_t graph = nullptr;
+    he = hipStreamEndCapture(c->stream, &graph);
+    if (e) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return e;
+    }
+    if (he != hipSuccess) return fail(NDT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+    lp.graph = graph;
+    HIP_TRY(hipGraphInstantiate(&lp.exec, lp.graph, nullptr, nullptr, 0));
+    lp.valid = true;
+    lp.scans = sc->serial;
+    lp.grid = g->serial;
+    lp.bm = bm->serial;
+    lp.device = c->device;
+    lp.keep = keep;
+    lp.params = *p;
+    lp.wait = ndt::wait_limit();
+    lp.bufs = plan_bufs(md);
+    return 0;
+}
+
+/* Replays the captured learned step with new guesses. */
+static int learned_plan_run(ndt_ctx* c, ndt_model* md, const ndt_scans* sc, const double* guesses,
+                            ndt_result* results, int32_t* used_learned) {
+    auto& lp = md->plan;
+    std::memcpy(lp.h_guess, guesses, 8 * 16 * (size_t)sc->S);
+    HIP_TRY(hipEventRecord(c->ev0, c->stream));
+    HIP_TRY(hipGraphLaunch(lp.exec, c->stream));
+    HIP_TRY(hipEventRecord(c->ev1, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    std::vector<int32_t> used((size_t)sc->S);
+    int e = ndt::align_sub_results(c, sc, lp.sub, results, used.data());
+    c->last_ms = ms;
+    c->last_launches = 1;
+    if (used_learned)
+        for (int32_t s = 0; s < sc->S; s++) used_learned[s] = used[(size_t)s];
+    return e;
 }
 
 extern "C" int ndt_model_predict(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const ndt_scans* sc,
@@ -871,15 +986,17 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
         launches += c->last_launches;
         pairs += c->last_pairs;
     };
+    if (policy == NDT_LEARNED_FALLBACK) {
+        /* the whole step as one captured graph */
+        int e = learned_plan_prepare(c, g, bm, md, sc, p, keep_rotation ? 1 : 0);
+        if (e == 0) return learned_plan_run(c, md, sc, guesses, results, used_learned);
+        if (e != NDT_ERR_UNSUPPORTED) return e;
+    }
     /* refined guesses */
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
     std::vector<double> refined((size_t)(16 * S));
-    int e = predict_dev(c, md, bm, sc, guesses, refined.data());
+    int e = predict_dev(c, md, bm, sc, guesses, refined.data(), keep_rotation ? 1 : 0);
     if (e) return e;
-    if (keep_rotation)
-        for (int s = 0; s < S; s++)
-            for (int r = 0; r < 3; r++)
-                for (int k = 0; k < 3; k++) refined[16 * (size_t)s + 4 * r + k] = guesses[16 * (size_t)s + 4 * r + k];
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipEventSynchronize(c->ev1));
     float fms = 0.f;

@@ -16,6 +16,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include "ndt_internal.h"
 #include "ndt_learn_internal.h"
 
 namespace ndt {
@@ -579,6 +580,58 @@ __global__ __launch_bounds__(64) void k_sum_ordered(const float* __restrict__ x,
 
 int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream) {
     hipLaunchKernelGGL(k_sum_ordered, dim3(1), dim3(64), 0, stream, x, n, out);
+    return (int)hipGetLastError();
+}
+
+
+/* ------------------------------------------------------------------ guesses */
+
+__global__ void k_guesses_local(const double* __restrict__ world, int32_t S, int32_t B, double ox, double oy,
+                                double oz, double* __restrict__ local) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const double* G = world + 16 * (int64_t)(b < S ? b : 0);
+    double* L = local + 16 * (int64_t)b;
+    for (int k = 0; k < 16; k++) L[k] = G[k];
+    L[3] = G[3] - ox;
+    L[7] = G[7] - oy;
+    L[11] = G[11] - oz;
+}
+
+int launch_guesses_local(const double* world, int32_t S, int32_t B, double ox, double oy, double oz, double* local,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(k_guesses_local, dim3(blocks_for(B, 64)), dim3(64), 0, stream, world, S, B, ox, oy, oz, local);
+    return (int)hipGetLastError();
+}
+
+__global__ void k_refine(const float* __restrict__ Y, const double* __restrict__ guesses, int32_t S, CorrScale scale,
+                         int32_t keep_rotation, double* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    double p6[NOUT], C[16], G[16], R[16];
+    for (int k = 0; k < NOUT; k++) {
+        double v = (double)Y[NOUT * (int64_t)s + k];
+        if (!isfinite(v)) v = 0.0;
+        p6[k] = v * scale.v[k];
+    }
+    pose_matrix(p6, C);
+    for (int k = 0; k < 16; k++) G[k] = guesses[16 * (int64_t)s + k];
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            double v = 0.0;
+            for (int k = 0; k < 4; k++) v += G[r * 4 + k] * C[k * 4 + c];
+            R[r * 4 + c] = v;
+        }
+    if (keep_rotation)
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) R[r * 4 + c] = G[r * 4 + c];
+    for (int k = 0; k < 16; k++) out[16 * (int64_t)s + k] = R[k];
+}
+
+int launch_refine(const float* Y, const double* guesses, int32_t S, CorrScale scale, int32_t keep_rotation,
+                  double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(k_refine, dim3(blocks_for(S, 64)), dim3(64), 0, stream, Y, guesses, S, scale, keep_rotation,
+                       out);
     return (int)hipGetLastError();
 }
 

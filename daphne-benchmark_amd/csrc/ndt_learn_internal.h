@@ -68,6 +68,21 @@ int launch_adam(float* w, float* m, float* v, const float* g, int64_t n, float l
 int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream);
 int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream);
 
+/* scale of each output of the model: the prediction of component k is the pose
+ * correction's component k divided by scale[k] (x, y, z in metres; roll, pitch,
+ * yaw in radians) */
+struct CorrScale {
+    double v[NOUT];
+};
+/* local[b] = world pose of guess min(b, S - 1 if b >= S: 0) minus the origin, for
+ * b < B (rows past S repeat guess 0) */
+int launch_guesses_local(const double* world, int32_t S, int32_t B, double ox, double oy, double oz, double* local,
+                         hipStream_t stream);
+/* refined guesses out[s] = guesses[s] * pose(Y[s] * scale) (R = Rx Ry Rz), a
+ * non-finite prediction counting as 0; keep_rotation keeps the guess's rotation */
+int launch_refine(const float* Y, const double* guesses, int32_t S, CorrScale scale, int32_t keep_rotation,
+                  double* out, hipStream_t stream);
+
 } // namespace ndt
 
 #endif

@@ -162,6 +162,29 @@ def test_predictions_depend_only_on_weights_and_inputs(A, ctx, trained):
     assert np.array_equal(po, fresh.predict(t["bm"], other, g2[:40]))
 
 
+def test_learned_step_replays(A, ctx, trained):
+    """The fallback policy runs as one captured graph per batch: replays with other
+    guesses in between give the same results, equal to those of a fresh model with
+    the same weights (its own capture) and to the uncaptured race's two starts."""
+    t = trained
+    ev, eg, m = t["evS"], t["eg"], t["model"]
+    pol = A.LEARNED_FALLBACK | A.LEARNED_KEEP_ROTATION
+    g2 = A.perturb_poses(t["ev_gt"], np.random.default_rng(33), xy=1.0, z=1.0, yaw_deg=3.0)
+    r1, u1 = m.align(t["grid"], t["bm"], ev, eg, t["prm"], pol)
+    r2, u2 = m.align(t["grid"], t["bm"], ev, g2, t["prm"], pol)
+    r3, u3 = m.align(t["grid"], t["bm"], ev, eg, t["prm"], pol)
+    fresh = A.Model(ctx, seed=3)
+    for k in range(6):
+        fresh.set(k, m.get(k))
+    r4, u4 = fresh.align(t["grid"], t["bm"], ev, g2, t["prm"], pol)
+    assert np.array_equal(u1, u3) and np.array_equal(u2, u4)
+    for a, b in zip(r1, r3):
+        assert np.array_equal(a["final_f64"], b["final_f64"]) and a["passes"] == b["passes"]
+    for a, b in zip(r2, r4):
+        assert np.array_equal(a["final_f64"], b["final_f64"]) and a["passes"] == b["passes"]
+    assert any(not np.array_equal(a["final_f64"], b["final_f64"]) for a, b in zip(r1, r2))
+
+
 def _success(results, gt):
     return np.array([np.linalg.norm(r["final_f64"][:3, 3] - g[:3, 3]) < 0.05 for r, g in zip(results, gt)])
 
