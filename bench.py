@@ -50,18 +50,19 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 TRAFFIC_PROFILE = "profiles/k_align_wq_cfg2_batch512_fp64.md"
 
 
-def profiled_traffic(args, prm):
-    """HBM bytes per batch launch measured with rocprofv3 (FETCH_SIZE + WRITE_SIZE,
-    calibrated) and recorded in the committed profile, when that profile covers this
-    workload, precision and launch shape; None otherwise."""
+def profiled_traffic(args, prm, kind="plain"):
+    """HBM bytes per align launch (kind "plain": the batch without the proposer,
+    "learned": the headline step's launch) measured with rocprofv3 (FETCH_SIZE +
+    WRITE_SIZE, calibrated) and recorded in the committed profile, when that profile
+    covers this workload, precision and launch shape; None otherwise."""
     import re
     try:
         with open(os.path.join(REPO, TRAFFIC_PROFILE)) as f:
             text = f.read()
     except OSError:
         return None
-    m = re.search(r"traffic_bytes_per_launch: (\d+) \(workload (\w+), (\d+) scans, (\w+), block (\d+), chunk (\d+)\)",
-                  text)
+    m = re.search(r"traffic_bytes_per_launch " + kind + r": (\d+) \(workload (\w+), (\d+) scans, (\w+), block (\d+), "
+                  r"chunk (\d+)\)", text)
     if not m:
         return None
     shape = (prm.block_size or 256, prm.points_per_wg or 2048)
@@ -69,6 +70,8 @@ def profiled_traffic(args, prm):
             args.workload, args.scans, args.precision, shape):
         return None
     return int(m.group(1))
+
+
 FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
 
@@ -393,6 +396,9 @@ def main():
                     help="cfg1-sized DAPHNE testcases for the reference harness comparison (0: off)")
     ap.add_argument("--precision-study", action="store_true",
                     help="only the cfg3 precision study (four arithmetic modes against the float64 oracle)")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="only the batched launch and the headline step (for profiling: no plain single-scan, "
+                         "upload, held-out, Hessian-at-accept, CPU, harness or cfg1 legs)")
     ap.add_argument("--hbm-copy-calibration", type=int, default=0, metavar="MIB",
                     help="only one device copy of MIB MiB (calibration of rocprofv3 traffic counters)")
     args = ap.parse_args()
@@ -405,6 +411,8 @@ def main():
     if args.precision_study:
         print(json.dumps(precision_study(args)))
         return
+    if args.only_headline:
+        args.plain_scans, args.cfg4_scans, args.cfg1_scans, args.no_cpu = 1, 0, 0, True
 
     t0 = time.time()
     w = A.make_workload(args.workload, seed=1, scans=args.scans)
@@ -453,25 +461,32 @@ def main():
         for _ in range(args.warmup):
             model.align(grid, bm, scans, w.guesses, prm, policy)
         # the timed step: scan and map grids, the MLP, the alignment launch, results
+        align_ms_l, step_ms_l = [], []
         t_l = time.perf_counter()
         for _ in range(args.steps):
             res_l, used = model.align(grid, bm, scans, w.guesses, prm, policy)
+            ms_l, _, pairs_l = ctx.last_timing()
+            align_ms_l.append(ms_l)
+            step_ms_l.append(ctx.last_step_ms())
         learned_s = (time.perf_counter() - t_l) / args.steps
-        learned_ms = ctx.last_timing()[0]
+        learned_ms = float(np.mean(step_ms_l))
         # the same step with the scans uploaded from host memory inside it (packing,
         # Morton order and the copy of every scan)
-        t_u = time.perf_counter()
-        for _ in range(args.steps):
-            fresh = A.Scans(ctx, w.scans)
-            model.align(grid, bm, fresh, w.guesses, prm, policy)
-            fresh.close()
-        upload_s = (time.perf_counter() - t_u) / args.steps
+        upload_s = float("nan")
+        if not args.only_headline:
+            t_u = time.perf_counter()
+            for _ in range(args.steps):
+                fresh = A.Scans(ctx, w.scans)
+                model.align(grid, bm, fresh, w.guesses, prm, policy)
+                fresh.close()
+            upload_s = (time.perf_counter() - t_u) / args.steps
         refined = model.predict(bm, scans, w.guesses)
         if not args.full_pose:
             refined[:, :3, :3] = w.guesses[:, :3, :3]
         policy_name = args.policy + ("" if args.full_pose else ", translation from the model")
         learned = dict(model=model, bm=bm, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
-                       refined=refined, train=tinfo, policy=policy_name, upload_s=upload_s, policy_flags=policy)
+                       align_ms=float(np.mean(align_ms_l)), pairs=pairs_l, refined=refined, train=tinfo,
+                       policy=policy_name, upload_s=upload_s, policy_flags=policy)
         log(f"batched+learned {S / learned_s:.0f}/s ({S / upload_s:.0f}/s with the scan upload in the step)")
 
     # ---- plain single-scan GPU NDT on the same scans: one submission and host sync
@@ -495,11 +510,23 @@ def main():
                                   np.arctan2(-w.gt[i][0, 1], w.gt[i][0, 0])) < np.deg2rad(0.25)
                               for i in range(len(R))]))
 
-    bytes_alg, flops_alg = algorithmic_work(res_b, counts, pairs_b)
-    avg_ms = float(np.mean(kernel_ms))
-    achieved_gbs = bytes_alg / (avg_ms * 1e-3) / 1e9
     peak_flops = FP64_PEAK_TFLOPS if args.precision == "fp64" else FP32_PEAK_TFLOPS
-    valu_frac = flops_alg / (avg_ms * 1e-3) / (peak_flops * 1e12)
+    avg_ms = float(np.mean(kernel_ms))
+
+    def roofline(results, pairs, ms, kernel, traffic_kind):
+        bytes_alg, flops_alg = algorithmic_work(results, counts, pairs)
+        achieved = bytes_alg / (ms * 1e-3) / 1e9
+        traffic = profiled_traffic(args, prm, traffic_kind)
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
+                "algorithmic_bytes": bytes_alg, "kernel_ms": ms,
+                "valu_frac": flops_alg / (ms * 1e-3) / (peak_flops * 1e12), "kernel": kernel}
+
+    roof_b = roofline(res_b, pairs_b, avg_ms, "ndt::k_align_wq (work-queue align launch of the 512-scan batch, "
+                                              "no proposer)", "plain")
+    roof_l = roofline(learned["res"], learned["pairs"], learned["align_ms"],
+                      "ndt::k_align_wq (the headline step's launch: probe pass, refined or given-guess run per "
+                      "scan)", "learned") if learned else None
     final = learned["res"] if learned else res_b
     value = S / (learned["step_s"] if learned else batched_s)
     ws = w.meta
@@ -544,47 +571,40 @@ def main():
         "launch_shape": {"block_size": prm.block_size or 256, "points_per_wg": prm.points_per_wg or 2048,
                          "autotuned": tuned_ms is not None},
         "profile_us_batched_plain": {k: float(v) for k, v in zip(("lists", "compute", "wait", "control"), profile)},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved_gbs,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved_gbs / HBM_PEAK_GBS,
-            "traffic": profiled_traffic(args, prm),
-            "traffic_source": TRAFFIC_PROFILE if profiled_traffic(args, prm) else None,
-            "valu_frac": valu_frac,
-            "kernel": "ndt::k_align_wq (work-queue align launch of the 512-scan batch, no proposer)",
-        },
+        "roofline": roof_l or roof_b,
+        "roofline_batched_no_proposer": roof_b,
         "device": f"{dev_name}, {cus} CUs",
         "generation_s": gen_s,
     }
     if learned:
         out["learned"] = dict(learned["train"], policy=learned["policy"],
                               used_refined=float(np.mean(learned["used"])),
-                              device_ms_per_step=learned["kernel_ms"],
-                              with_scan_upload_alignments_per_s=S / learned["upload_s"])
+                              device_ms_per_step=learned["kernel_ms"], align_kernel_ms=learned["align_ms"],
+                              with_scan_upload_alignments_per_s=(None if np.isnan(learned["upload_s"])
+                                                                 else S / learned["upload_s"]))
         if args.cfg4_scans > 0:
             out["cfg4_held_out"] = cfg4_leg(ctx, w, grid, learned["bm"], learned["model"], prm,
                                             learned["policy_flags"], args.cfg4_scans, max(1, min(args.steps, 5)))
             log(f"cfg4: {out['cfg4_held_out']['passes_saved_per_scan']:.2f} passes saved per scan")
     # the reference's order of passes (Hessian only at accepted steps): same poses,
     # more passes
-    prm_ha = A.params(A.NDT_MODE_NDT, point_precision=point_prec, origin=w.origin, block_size=prm.block_size,
-                      points_per_wg=prm.points_per_wg, hessian_at_accept=1)
-    scans.align(grid, w.guesses, prm_ha)
-    t_h = time.perf_counter()
-    for _ in range(max(1, min(args.steps, 5))):
-        res_h = scans.align(grid, w.guesses, prm_ha)
-    dt_h = np.array([np.abs(a["final_f64"][:3, 3] - b["final_f64"][:3, 3]).max() for a, b in zip(res_h, res_b)])
-    out["batched_hessian_at_accept"] = {
-        "value": S * max(1, min(args.steps, 5)) / (time.perf_counter() - t_h), "unit": "alignments/s",
-        "mean_passes": float(np.mean([r["passes"] for r in res_h])),
-        # the extra passes move the late-pass chunking (pass 28 on), so sums round
-        # differently there
-        "same_poses_as_default": int(sum(np.array_equal(a["final_f64"], b["final_f64"])
-                                         for a, b in zip(res_h, res_b))),
-        "within_1mm_of_default": int(np.sum(dt_h <= 1e-3))}
-
+    prm_ha = None if args.only_headline else A.params(
+        A.NDT_MODE_NDT, point_precision=point_prec, origin=w.origin, block_size=prm.block_size,
+        points_per_wg=prm.points_per_wg, hessian_at_accept=1)
+    if prm_ha is not None:
+        scans.align(grid, w.guesses, prm_ha)
+        t_h = time.perf_counter()
+        for _ in range(max(1, min(args.steps, 5))):
+            res_h = scans.align(grid, w.guesses, prm_ha)
+        dt_h = np.array([np.abs(a["final_f64"][:3, 3] - b["final_f64"][:3, 3]).max() for a, b in zip(res_h, res_b)])
+        out["batched_hessian_at_accept"] = {
+            "value": S * max(1, min(args.steps, 5)) / (time.perf_counter() - t_h), "unit": "alignments/s",
+            "mean_passes": float(np.mean([r["passes"] for r in res_h])),
+            # the extra passes move the late-pass chunking (pass 28 on), so sums round
+            # differently there
+            "same_poses_as_default": int(sum(np.array_equal(a["final_f64"], b["final_f64"])
+                                             for a, b in zip(res_h, res_b))),
+            "within_1mm_of_default": int(np.sum(dt_h <= 1e-3))}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import orc

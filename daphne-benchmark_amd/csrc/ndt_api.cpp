@@ -130,6 +130,7 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
     }
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev2)) != hipSuccess || (e = hipEventCreate(&c->ev3)) != hipSuccess ||
         (e = hipHostMalloc(&c->hsmall, HSMALL_BYTES, hipHostMallocDefault)) != hipSuccess) {
         delete c;
         return fail(NDT_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
@@ -149,6 +150,8 @@ void ndt::ctx_release(ndt_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->ev2) (void)hipEventDestroy(c->ev2);
+    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->pin) (void)hipHostFree(c->pin);
     if (c->hsmall) (void)hipHostFree(c->hsmall);
@@ -178,6 +181,12 @@ extern "C" int ndt_ctx_device_info(ndt_ctx* c, int32_t* cus, char* name, int32_t
 extern "C" int ndt_ctx_last_profile(ndt_ctx* c, double phases_us[4]) {
     if (!c || !phases_us) return fail(NDT_ERR_INVALID, "null argument");
     for (int k = 0; k < 4; k++) phases_us[k] = c->last_profile[k];
+    return 0;
+}
+
+extern "C" int ndt_ctx_last_step_time(ndt_ctx* c, double* device_ms) {
+    if (!c || !device_ms) return fail(NDT_ERR_INVALID, "null argument");
+    *device_ms = c->last_step_ms;
     return 0;
 }
 
@@ -826,6 +835,7 @@ static int align_packed(ndt_ctx* c, const ndt_grid* g, const float4* pts, const 
         }
     }
     c->last_ms = ms_total;
+    c->last_step_ms = c->last_ms;
     c->last_launches = launches;
     for (int k = 0; k < 4; k++) c->last_profile[k] = prof_sum[k];
     return collect_results(c, counts, S, aborted, results, used_alt);
@@ -1057,6 +1067,7 @@ extern "C" int ndt_align_scans_noalloc(ndt_ctx* c, const ndt_grid* g, const ndt_
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->last_step_ms = c->last_ms;
     c->last_launches = 1;
     c->last_pairs = (int64_t)prof[5];
     for (int k = 0; k < 4; k++) c->last_profile[k] = prof[k];
@@ -1105,7 +1116,7 @@ int ndt::align_sub_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
 }
 
 int ndt::align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, AlignSub& a, const double* d_guesses,
-                           const double* d_alt) {
+                           const double* d_alt, hipEvent_t before, hipEvent_t after) {
     const int32_t S = a.S;
     LAUNCH_TRY(launch_zero_u32(a.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(a.items.p, (int64_t)(2 * a.cap), c->stream));
@@ -1114,10 +1125,12 @@ int ndt::align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, A
                                   a.sync.as<RoundSync>(), c->stream));
     LAUNCH_TRY(launch_init_queue(a.desc.as<ScanDesc>(), S, (uint64_t)a.first_pass, a.queue.as<WorkQueue>(),
                                  a.items.as<uint64_t>(), a.done.as<uint32_t>(), c->stream));
+    if (before) HIP_TRY(hipEventRecord(before, c->stream));
     LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), a.desc.as<ScanDesc>(), a.nwg, a.block, a.prm,
                                a.states.as<ScanState>(), a.queue.as<WorkQueue>(), a.items.as<uint64_t>(), a.cap,
                                a.done.as<uint32_t>(), a.partials.as<double>(), a.out.as<ScanOut>(),
                                a.prof.as<double>(), 0, c->stream));
+    if (after) HIP_TRY(hipEventRecord(after, c->stream));
     HIP_TRY(hipMemcpyAsync(a.h_out, a.out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(a.h_prof, a.prof.p, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     return 0;
@@ -1234,6 +1247,7 @@ static int graph_align(ndt_ctx* c, ndt_scans* sc, const double* guesses, ndt_res
     float ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->last_step_ms = c->last_ms;
     c->last_launches = 1;
     for (int k = 0; k < 4; k++) c->last_profile[k] = gp.h_prof[k];
     const int aborted = gp.h_prof[4] != 0.0;
@@ -1647,6 +1661,7 @@ extern "C" int ndt_compat_callback(ndt_ctx* c, const float* scan, int64_t n, con
     float ms = 0;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
     c->last_ms = ms;
+    c->last_step_ms = c->last_ms;
     c->last_launches = 2 * passes;
     c->last_pairs = (int64_t)pairs;
     std::memset(result, 0, sizeof(*result));

@@ -918,7 +918,7 @@ static int learned_plan_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap*
     hipError_t he = hipMemcpyAsync(md->gw.p, lp.h_guess, 8 * 16 * (size_t)sc->S, hipMemcpyHostToDevice, c->stream);
     e = he == hipSuccess ? 0 : fail(NDT_ERR_HIP, std::string("guess upload: ") + hipGetErrorString(he));
     if (!e) e = predict_enqueue(md, bm, sc, keep);
-    if (!e) e = ndt::align_sub_enqueue(c, g, sc, lp.sub, md->gw.as<double>(), md->refined.as<double>());
+    if (!e) e = ndt::align_sub_enqueue(c, g, sc, lp.sub, md->gw.as<double>(), md->refined.as<double>(), c->ev2, c->ev3);
     hipGraph_t graph = nullptr;
     he = hipStreamEndCapture(c->stream, &graph);
     if (e) {
@@ -949,11 +949,18 @@ static int learned_plan_run(ndt_ctx* c, ndt_model* md, const ndt_scans* sc, cons
     HIP_TRY(hipGraphLaunch(lp.exec, c->stream));
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    float ms = 0.f;
+    float ms = 0.f, align_ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    /* the align kernel's own time, from the events recorded around it in the graph
+     * (the whole step's when those cannot be read) */
+    if (hipEventElapsedTime(&align_ms, c->ev2, c->ev3) != hipSuccess) {
+        (void)hipGetLastError();
+        align_ms = ms;
+    }
     std::vector<int32_t> used((size_t)sc->S);
     int e = ndt::align_sub_results(c, sc, lp.sub, results, used.data());
-    c->last_ms = ms;
+    c->last_ms = align_ms;
+    c->last_step_ms = ms;
     c->last_launches = 1;
     if (used_learned)
         for (int32_t s = 0; s < sc->S; s++) used_learned[s] = used[(size_t)s];
@@ -978,11 +985,12 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
     if (int e = check_poses(guesses, sc->S, "guess")) return e;
     HIP_TRY(hipSetDevice(c->device));
     const int S = sc->S;
-    double ms_total = 0.0;
+    double ms_total = 0.0, ms_align = 0.0;
     int32_t launches = 0;
     int64_t pairs = 0;
     auto account = [&]() {
         ms_total += c->last_ms;
+        ms_align += c->last_ms;
         launches += c->last_launches;
         pairs += c->last_pairs;
     };
@@ -1036,7 +1044,8 @@ extern "C" int ndt_align_scans_learned(ndt_ctx* c, const ndt_grid* g, const ndt_
         if (used_learned)
             for (int s = 0; s < S; s++) used_learned[s] = used[(size_t)s];
     }
-    c->last_ms = ms_total;
+    c->last_ms = ms_align;
+    c->last_step_ms = ms_total;
     c->last_launches = launches;
     c->last_pairs = pairs;
     return 0;

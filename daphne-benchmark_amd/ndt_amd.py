@@ -448,6 +448,7 @@ def hip_lib():
         L.ndt_derivatives.argtypes = [P, P, P, C.c_int64, P, P, C.POINTER(NdtParams), P, P, P, P]
         L.ndt_ctx_last_timing.argtypes = [P, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
         L.ndt_ctx_last_profile.argtypes = [P, P]
+        L.ndt_ctx_last_step_time.argtypes = [P, C.POINTER(C.c_double)]
         L.ndt_fitness_score.argtypes = [P, P, P, P, C.c_double, P, P]
         L.ndt_autotune.argtypes = [P, P, P, P, C.POINTER(NdtParams), C.c_char_p, C.POINTER(C.c_double)]
         # learned warm start (ndt_learn.h)
@@ -488,7 +489,7 @@ LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
                  "ndt_train_params_default", "ndt_dataset_create", "ndt_dataset_destroy", "ndt_dataset_add",
                  "ndt_dataset_size", "ndt_model_train",
-                 "ndt_model_predict", "ndt_align_scans_learned")
+                 "ndt_model_predict", "ndt_align_scans_learned", "ndt_ctx_last_step_time")
 
 
 def _check(rc):
@@ -623,6 +624,12 @@ class Context:
         ms, launches, pairs = C.c_double(), C.c_int32(), C.c_int64()
         _check(hip_lib().ndt_ctx_last_timing(self.h, C.byref(ms), C.byref(launches), C.byref(pairs)))
         return ms.value, launches.value, pairs.value
+
+    def last_step_ms(self) -> float:
+        """Device milliseconds of the whole last learned call (features, model, NDT)."""
+        ms = C.c_double()
+        _check(hip_lib().ndt_ctx_last_step_time(self.h, C.byref(ms)))
+        return ms.value
 
     def last_profile(self):
         """(prologue, compute, barrier, control) microseconds of the last ndt alignment."""

@@ -138,12 +138,18 @@ int ndt_model_predict(ndt_ctx* ctx, ndt_model* model, const ndt_bevmap* bm, cons
 
 /* Batched NDT with the learned warm start under `policy` (NDT_LEARNED_*).
  * used_learned[s] (optional) = 1 where the result comes from the refined guess.
- * passes in the results count every derivative pass spent on the scan.  Device
- * milliseconds of the feature, model and NDT kernels of the whole call are
- * available through ndt_ctx_last_timing. */
+ * passes in the results count every derivative pass spent on the scan.
+ * ndt_ctx_last_timing reports the NDT alignment kernel(s) of the call and
+ * ndt_ctx_last_step_time the device time of the whole call (features, model and
+ * NDT).  The fallback policy runs as one graph, captured at the first call for a
+ * (model, BEV map, scans, grid, parameters, rotation choice) and replayed after. */
 int ndt_align_scans_learned(ndt_ctx* ctx, const ndt_grid* grid, const ndt_bevmap* bm, ndt_model* model,
                             const ndt_scans* scans, const double* guesses, const ndt_params* params, int32_t policy,
                             ndt_result* results, int32_t* used_learned);
+
+/* Device milliseconds of the whole last ndt_align_scans_learned call on this
+ * context (for the other alignment calls, the alignment kernels' time). */
+int ndt_ctx_last_step_time(ndt_ctx* ctx, double* device_ms);
 
 #ifdef __cplusplus
 }
