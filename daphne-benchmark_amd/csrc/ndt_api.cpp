@@ -130,7 +130,7 @@ extern "C" int ndt_ctx_create(int32_t device, ndt_ctx** out) {
     }
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess ||
-        (e = hipEventCreate(&c->ev2)) != hipSuccess || (e = hipEventCreate(&c->ev3)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev2)) != hipSuccess ||
         (e = hipHostMalloc(&c->hsmall, HSMALL_BYTES, hipHostMallocDefault)) != hipSuccess) {
         delete c;
         return fail(NDT_ERR_HIP, std::string("stream/event creation: ") + hipGetErrorString(e));
@@ -151,7 +151,6 @@ void ndt::ctx_release(ndt_ctx* c) {
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->ev2) (void)hipEventDestroy(c->ev2);
-    if (c->ev3) (void)hipEventDestroy(c->ev3);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->pin) (void)hipHostFree(c->pin);
     if (c->hsmall) (void)hipHostFree(c->hsmall);
@@ -1116,7 +1115,7 @@ int ndt::align_sub_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, c
 }
 
 int ndt::align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, AlignSub& a, const double* d_guesses,
-                           const double* d_alt, hipEvent_t before, hipEvent_t after) {
+                           const double* d_alt) {
     const int32_t S = a.S;
     LAUNCH_TRY(launch_zero_u32(a.out.p, (int64_t)(sizeof(ScanOut) * S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(a.items.p, (int64_t)(2 * a.cap), c->stream));
@@ -1125,12 +1124,10 @@ int ndt::align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, A
                                   a.sync.as<RoundSync>(), c->stream));
     LAUNCH_TRY(launch_init_queue(a.desc.as<ScanDesc>(), S, (uint64_t)a.first_pass, a.queue.as<WorkQueue>(),
                                  a.items.as<uint64_t>(), a.done.as<uint32_t>(), c->stream));
-    if (before) HIP_TRY(hipEventRecord(before, c->stream));
     LAUNCH_TRY(launch_align_wq(g->dev, sc->pts.as<float4>(), a.desc.as<ScanDesc>(), a.nwg, a.block, a.prm,
                                a.states.as<ScanState>(), a.queue.as<WorkQueue>(), a.items.as<uint64_t>(), a.cap,
                                a.done.as<uint32_t>(), a.partials.as<double>(), a.out.as<ScanOut>(),
                                a.prof.as<double>(), 0, c->stream));
-    if (after) HIP_TRY(hipEventRecord(after, c->stream));
     HIP_TRY(hipMemcpyAsync(a.h_out, a.out.p, sizeof(ScanOut) * S, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipMemcpyAsync(a.h_prof, a.prof.p, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     return 0;

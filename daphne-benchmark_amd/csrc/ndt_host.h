@@ -145,7 +145,7 @@ struct ndt_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev2 = nullptr, ev3 = nullptr; /* around the align kernel inside captured steps */
+    hipEvent_t ev2 = nullptr; /* between the two graphs of a captured learned step */
     int cus = 0;
     /* device address of the allocation-free entry point's workspace on this
      * context's device (looked up at creation), or null with the lookup's error */
@@ -257,7 +257,7 @@ int align_sub_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, const 
  * be recorded into a graph): set-up kernels, the work-queue launch from the
  * device-resident guesses (and refined starts), results to the pinned buffers. */
 int align_sub_enqueue(ndt_ctx* c, const ndt_grid* g, const ndt_scans* sc, AlignSub& a, const double* d_guesses,
-                      const double* d_alt, hipEvent_t before = nullptr, hipEvent_t after = nullptr);
+                      const double* d_alt);
 /* Results of a completed submission (after the stream was synchronised). */
 int align_sub_results(ndt_ctx* c, const ndt_scans* sc, const AlignSub& a, ndt_result* results, int32_t* used_alt);
 /* parameters that give the same alignment launch */

@@ -123,17 +123,21 @@ struct ndt_model {
         uint64_t scans = 0, grid = 0, bm = 0, wait = 0;
         int device = -1, keep = 0;
         ndt_params params{};
-        std::vector<const void*> bufs; /* workspace the graph refers to */
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
+        std::vector<const void*> bufs; /* workspace the graphs refer to */
+        /* two graphs launched back to back: the prediction, then the alignment
+         * (events between them time the align kernel) */
+        hipGraph_t graph[2] = {nullptr, nullptr};
+        hipGraphExec_t exec[2] = {nullptr, nullptr};
         double* h_guess = nullptr;
         ndt::AlignSub sub;
         void release() {
-            if (exec) (void)hipGraphExecDestroy(exec);
-            if (graph) (void)hipGraphDestroy(graph);
+            for (int k = 0; k < 2; k++) {
+                if (exec[k]) (void)hipGraphExecDestroy(exec[k]);
+                if (graph[k]) (void)hipGraphDestroy(graph[k]);
+                exec[k] = nullptr;
+                graph[k] = nullptr;
+            }
             if (h_guess) (void)hipHostFree(h_guess);
-            exec = nullptr;
-            graph = nullptr;
             h_guess = nullptr;
             valid = false;
         }
@@ -914,20 +918,26 @@ static int learned_plan_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap*
     if ((e = ndt::align_sub_prepare(c, g, sc, p, true, lp.sub))) return e;
     HIP_TRY(hipHostMalloc((void**)&lp.h_guess, 8 * 16 * (size_t)sc->S, hipHostMallocDefault));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t he = hipMemcpyAsync(md->gw.p, lp.h_guess, 8 * 16 * (size_t)sc->S, hipMemcpyHostToDevice, c->stream);
-    e = he == hipSuccess ? 0 : fail(NDT_ERR_HIP, std::string("guess upload: ") + hipGetErrorString(he));
-    if (!e) e = predict_enqueue(md, bm, sc, keep);
-    if (!e) e = ndt::align_sub_enqueue(c, g, sc, lp.sub, md->gw.as<double>(), md->refined.as<double>(), c->ev2, c->ev3);
-    hipGraph_t graph = nullptr;
-    he = hipStreamEndCapture(c->stream, &graph);
-    if (e) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return e;
+    for (int k = 0; k < 2; k++) {
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        if (k == 0) {
+            hipError_t he = hipMemcpyAsync(md->gw.p, lp.h_guess, 8 * 16 * (size_t)sc->S, hipMemcpyHostToDevice,
+                                           c->stream);
+            e = he == hipSuccess ? 0 : fail(NDT_ERR_HIP, std::string("guess upload: ") + hipGetErrorString(he));
+            if (!e) e = predict_enqueue(md, bm, sc, keep);
+        } else {
+            e = ndt::align_sub_enqueue(c, g, sc, lp.sub, md->gw.as<double>(), md->refined.as<double>());
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t he = hipStreamEndCapture(c->stream, &graph);
+        if (e) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return e;
+        }
+        if (he != hipSuccess) return fail(NDT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
+        lp.graph[k] = graph;
+        HIP_TRY(hipGraphInstantiate(&lp.exec[k], lp.graph[k], nullptr, nullptr, 0));
     }
-    if (he != hipSuccess) return fail(NDT_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(he));
-    lp.graph = graph;
-    HIP_TRY(hipGraphInstantiate(&lp.exec, lp.graph, nullptr, nullptr, 0));
     lp.valid = true;
     lp.scans = sc->serial;
     lp.grid = g->serial;
@@ -946,17 +956,14 @@ static int learned_plan_run(ndt_ctx* c, ndt_model* md, const ndt_scans* sc, cons
     auto& lp = md->plan;
     std::memcpy(lp.h_guess, guesses, 8 * 16 * (size_t)sc->S);
     HIP_TRY(hipEventRecord(c->ev0, c->stream));
-    HIP_TRY(hipGraphLaunch(lp.exec, c->stream));
+    HIP_TRY(hipGraphLaunch(lp.exec[0], c->stream));
+    HIP_TRY(hipEventRecord(c->ev2, c->stream));
+    HIP_TRY(hipGraphLaunch(lp.exec[1], c->stream));
     HIP_TRY(hipEventRecord(c->ev1, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0.f, align_ms = 0.f;
     HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    /* the align kernel's own time, from the events recorded around it in the graph
-     * (the whole step's when those cannot be read) */
-    if (hipEventElapsedTime(&align_ms, c->ev2, c->ev3) != hipSuccess) {
-        (void)hipGetLastError();
-        align_ms = ms;
-    }
+    HIP_TRY(hipEventElapsedTime(&align_ms, c->ev2, c->ev1));
     std::vector<int32_t> used((size_t)sc->S);
     int e = ndt::align_sub_results(c, sc, lp.sub, results, used.data());
     c->last_ms = align_ms;
