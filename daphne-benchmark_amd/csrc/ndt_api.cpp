@@ -868,11 +868,14 @@ static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const 
     BUF_TRY(dst, sizeof(float4) * t);
     std::vector<float> staging((size_t)(4 * t));
     auto work = [&](int32_t s0, int32_t s1) {
-        std::vector<std::pair<uint32_t, uint32_t>> order;
+        /* stable LSD radix sort of the 30-bit Morton keys, three passes of 10 bits */
+        std::vector<uint32_t> key, idx, tmp, cnt(1025);
         for (int32_t s = s0; s < s1; s++) {
             const float* src = scans[s];
             const int64_t n = counts[s];
-            order.resize((size_t)n);
+            key.resize((size_t)n);
+            idx.resize((size_t)n);
+            tmp.resize((size_t)n);
             for (int64_t i = 0; i < n; i++) {
                 uint32_t q[3];
                 for (int k = 0; k < 3; k++) {
@@ -880,14 +883,18 @@ static int pack_scans(ndt_ctx* c, DevBuf& dst, const float* const* scans, const 
                     double cell = std::isfinite(v) ? std::floor((double)v) + 512.0 : 0.0;
                     q[k] = (uint32_t)std::min(1023.0, std::max(0.0, cell));
                 }
-                order[(size_t)i] = {spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2), (uint32_t)i};
+                key[(size_t)i] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+                idx[(size_t)i] = (uint32_t)i;
             }
-            std::stable_sort(order.begin(), order.end(),
-                             [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-                                 return a.first < b.first;
-                             });
+            for (int sh = 0; sh < 30; sh += 10) {
+                std::fill(cnt.begin(), cnt.end(), 0u);
+                for (int64_t i = 0; i < n; i++) cnt[((key[idx[(size_t)i]] >> sh) & 1023u) + 1]++;
+                for (int b = 0; b < 1024; b++) cnt[(size_t)b + 1] += cnt[(size_t)b];
+                for (int64_t i = 0; i < n; i++) tmp[cnt[(key[idx[(size_t)i]] >> sh) & 1023u]++] = idx[(size_t)i];
+                idx.swap(tmp);
+            }
             float* d = staging.data() + 4 * offsets[s];
-            for (int64_t i = 0; i < n; i++) std::memcpy(d + 4 * i, src + 4 * (size_t)order[(size_t)i].second, 16);
+            for (int64_t i = 0; i < n; i++) std::memcpy(d + 4 * i, src + 4 * (size_t)idx[(size_t)i], 16);
         }
     };
     /* scans are independent: reorder them on up to 16 host threads */

@@ -74,15 +74,6 @@ void correction_params(const double* G, const double* T, double* out6) {
     out6[5] = e[2];
 }
 
-void mat4_mul(const double* A, const double* B, double* C) {
-    for (int r = 0; r < 4; r++)
-        for (int c = 0; c < 4; c++) {
-            double v = 0.0;
-            for (int k = 0; k < 4; k++) v += A[r * 4 + k] * B[k * 4 + c];
-            C[r * 4 + c] = v;
-        }
-}
-
 int pick_splits(int M, int N, int K, int cus) {
     const int tiles = (M / GEMM_BM) * (N / GEMM_BN);
     int s = 1;

@@ -165,7 +165,11 @@ int ndt_align_batch(ndt_ctx* ctx, const ndt_grid* grid, const float* const* scan
                     const int64_t* counts, int32_t S, const double* guesses,
                     const ndt_params* params, ndt_result* results);
 
-/* Device-resident scan batches, so that timed runs start with inputs in HBM. */
+/* Device-resident scan batches, so that timed runs start with inputs in HBM.
+ * ndt_align_scans captures its submission into a graph held by the scans handle (and
+ * pinned guess / result buffers with it) and replays it on later calls with the same
+ * grid and parameters: a scans handle must not be used by two threads at once, even
+ * through the const pointer (each thread uploads its own batch). */
 int ndt_scans_upload(ndt_ctx* ctx, const float* const* scans, const int64_t* counts, int32_t S,
                      ndt_scans** out);
 int ndt_scans_destroy(ndt_scans* scans);

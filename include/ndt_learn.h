@@ -88,7 +88,9 @@ int ndt_features(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, con
                  const double* guesses, int32_t B, uint16_t* out);
 
 /* Creates a model with He-normal hidden weights (seeded), small output weights
- * and zero biases.  f0, f1, f2 must be multiples of 128; features need f0 = 65536. */
+ * and zero biases.  f0, f1, f2 must be multiples of 128; features need f0 = 65536.
+ * A model holds its batch workspace and the captured learned step: it must not be
+ * used by two threads at once. */
 int ndt_model_create(ndt_ctx* ctx, int32_t f0, int32_t f1, int32_t f2, uint64_t seed, ndt_model** out);
 int ndt_model_destroy(ndt_model* model);
 int ndt_model_dims(const ndt_model* model, int32_t dims[3]);
