@@ -74,6 +74,8 @@ def profiled_traffic(args, prm, kind="plain"):
 
 FP64_PEAK_TFLOPS = 78.6        # MI355X vector FP64
 FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
+BF16_PEAK_TFLOPS = 2500.0      # MI355X dense bf16 MFMA
+MLP_FWD_FLOPS = 2.0 * (65536 * 1024 + 1024 * 256 + 256 * 6)  # per sample, 65536-1024-256-6
 
 
 def log(*a):
@@ -520,7 +522,8 @@ def main():
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
                 "algorithmic_bytes": bytes_alg, "kernel_ms": ms,
-                "valu_frac": flops_alg / (ms * 1e-3) / (peak_flops * 1e12), "kernel": kernel}
+                "valu_frac": flops_alg / (ms * 1e-3) / (peak_flops * 1e12), "valu_peak_tflops": peak_flops,
+                "valu_frac_of_fp32_peak": flops_alg / (ms * 1e-3) / (FP32_PEAK_TFLOPS * 1e12), "kernel": kernel}
 
     roof_b = roofline(res_b, pairs_b, avg_ms, "ndt::k_align_wq (work-queue align launch of the 512-scan batch, "
                                               "no proposer)", "plain")
@@ -577,7 +580,14 @@ def main():
         "generation_s": gen_s,
     }
     if learned:
-        out["learned"] = dict(learned["train"], policy=learned["policy"],
+        tr = learned["train"]
+        # training step = forward + backward (3x the forward flops) per sample, over the
+        # whole training call (Adam, map crops and transposes included)
+        train_tflops = 3.0 * MLP_FWD_FLOPS * tr["samples"] * tr["epochs"] / tr["train_s"] / 1e12
+        out["learned"] = dict(tr, policy=learned["policy"],
+                              mlp={"forward_flops_per_sample": MLP_FWD_FLOPS, "train_tflops": train_tflops,
+                                   "train_frac_of_bf16_peak": train_tflops / BF16_PEAK_TFLOPS,
+                                   "bf16_peak_tflops": BF16_PEAK_TFLOPS},
                               used_refined=float(np.mean(learned["used"])),
                               device_ms_per_step=learned["kernel_ms"], align_kernel_ms=learned["align_ms"],
                               with_scan_upload_alignments_per_s=(None if np.isnan(learned["upload_s"])
