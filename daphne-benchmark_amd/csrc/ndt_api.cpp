@@ -410,8 +410,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     int64_t cap = 1ll << log2cap;
     BUF_TRY(g->slots, sizeof(HashSlot) * cap);
     LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
-    BUF_TRY(g->cand_head, sizeof(float4) * (ne + HEAD_PAD));
-    HIP_TRY(hipMemsetAsync(g->cand_head.as<float4>() + ne, 0, sizeof(float4) * HEAD_PAD, c->stream));
+    BUF_TRY(g->cand_head, sizeof(float4) * (ne + 1));
     BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
     BUF_TRY(g->vs64, 5 * sizeof(double2) * (nv + 1));
     LAUNCH_TRY(launch_voxel_s64(g->mean_c.as<double>(), g->icov_c.as<double>(), nv, g->vs64.as<double2>(), c->stream));

@@ -231,10 +231,6 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
  *   vs64  5 x double2 (mx my)(mz c00)(c01 c02)(c11 c12)(c22 0): float64 mean and
  *         inverse covariance, 80 bytes per voxel (27 times less than per
  *         candidate, so that they stay in the caches). */
-/* zeroed candidate heads after the last one: a batch of up to HEAD_PAD head loads
- * starting at any candidate stays inside the array */
-constexpr int HEAD_PAD = 8;
-
 struct NdtGridDev {
     const uint2* dense;      /* query cell -> (begin, count), dense over the map's cell box, or null */
     int32_t dims[3];         /* cells per axis of the dense index */

@@ -1317,7 +1317,6 @@ __device__ inline double bperm_f64(int src_byte, double v) {
 #ifndef HEAD_BATCH
 #define HEAD_BATCH 8
 #endif
-static_assert(HEAD_BATCH <= HEAD_PAD, "a head batch may read past the last candidate by HEAD_BATCH - 1 entries");
 
 /* Inputs of pair n of the wave's current round.  Pairs 0 .. carry-1 are those
  * carried over from the previous round (slot n of the carry table, read only when
@@ -1487,20 +1486,20 @@ __device__ inline int chunk_accumulate(const NdtGridDev& g, const float4* __rest
             lc.begin = ec.x;
             lc.count = ec.y;
             /* prefilter: the passing candidates go to this lane's column of the
-             * table.  Heads are loaded HEAD_BATCH at a time from one address, all
-             * loads issued before the first is used (entries past the run belong to
-             * the next runs or the zeroed padding and are not counted), so a point
+             * table.  Heads are loaded HEAD_BATCH at a time, all loads issued before
+             * the first is used (indices past the run repeat its last entry, whose
+             * cache line is already on its way, and are not counted), so a point
              * waits for one round trip per HEAD_BATCH candidates.  Branch-free:
              * every candidate is written at the next free position, which a
              * failing one leaves free. */
             uint32_t* mine = wp.pick + lane;
             uint32_t cnt = 0;
             for (uint32_t k0 = 0; k0 < lc.count; k0 += HEAD_BATCH) {
+                const uint32_t last = lc.begin + lc.count - 1;
                 const uint32_t i0 = lc.begin + k0;
-                const float4* hp = g.head + i0;
                 float4 h[HEAD_BATCH];
 #pragma unroll
-                for (int q = 0; q < HEAD_BATCH; q++) h[q] = hp[q];
+                for (int q = 0; q < HEAD_BATCH; q++) h[q] = g.head[min(i0 + q, last)];
 #pragma unroll
                 for (int q = 0; q < HEAD_BATCH; q++) {
                     const float e0 = lc.r0 - h[q].x, e1 = lc.r1 - h[q].y, e2 = lc.r2 - h[q].z;
