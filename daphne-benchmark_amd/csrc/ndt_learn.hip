@@ -49,51 +49,75 @@ __device__ inline int bev_index(double qx, double qy, double qz) {
     return ((int)fz * BEV_NY + (int)fy) * BEV_NX + (int)fx;
 }
 
-constexpr int BEV_BLOCK = 512;
-constexpr int BEV_HALF = BEV_CELLS / 2; /* cells counted per pass (z planes 0-3, then 4-7) */
+constexpr int BEV_BLOCK = 1024;
+constexpr int BEV_HALF = BEV_CELLS / 2; /* z planes 0-3, then 4-7 */
+/* points one workgroup may count with 16-bit counters: no count can then carry */
+constexpr int64_t BEV_PACKED_MAX = 65535;
 
+/* A grid's histogram lives in 64 KiB of LDS (BEV_HALF words).  With at most
+ * BEV_PACKED_MAX points (every scan and map crop of the benchmark) the counters are
+ * the 16-bit halves of the words, cell c in the low half of word c and cell
+ * c + BEV_HALF in the high half (cells that share a word lie four z planes apart,
+ * so that nearby points rarely contend for a word), and the points are read once.
+ * Larger inputs take two passes over the points with exact 32-bit counters, one
+ * pass per half of the cells.  Counts saturate at 65535 in the output table. */
 __device__ inline void bev_clear(uint32_t* hist) {
     for (int i = threadIdx.x; i < BEV_HALF; i += BEV_BLOCK) hist[i] = 0u;
 }
-__device__ inline void bev_emit(const uint32_t* hist, int half, const uint16_t* lut, uint16_t* out) {
+__device__ inline void bev_add(uint32_t* hist, const int c, const bool packed, const int half) {
+    if (packed) {
+        if (c >= 0) atomicAdd(&hist[c & (BEV_HALF - 1)], c >= BEV_HALF ? 65536u : 1u);
+    } else {
+        const int h = c - half * BEV_HALF;
+        if (c >= 0 && h >= 0 && h < BEV_HALF) atomicAdd(&hist[h], 1u);
+    }
+}
+__device__ inline void bev_emit(const uint32_t* hist, const bool packed, const int half,
+                                const uint16_t* __restrict__ lut, uint16_t* __restrict__ out) {
     for (int i = threadIdx.x; i < BEV_HALF; i += BEV_BLOCK) {
-        const uint32_t c = hist[i];
-        out[half * BEV_HALF + i] = lut[c > 65535u ? 65535u : c];
+        const uint32_t w = hist[i];
+        if (packed) {
+            out[i] = lut[w & 0xffffu];
+            out[BEV_HALF + i] = lut[w >> 16];
+        } else {
+            out[half * BEV_HALF + i] = lut[w > 65535u ? 65535u : w];
+        }
     }
 }
 
-/* Two workgroups per scan, one per half of the cells (z planes 0-3, then 4-7),
- * so that exact 32-bit counters fit in 64 KiB of LDS: histogram of the scan's
- * points (sensor frame) falling in that half. */
+/* One workgroup per scan: histogram of the scan's points in the sensor frame. */
 __global__ __launch_bounds__(BEV_BLOCK) void k_bev_scan(const float4* __restrict__ pts,
                                                         const int64_t* __restrict__ offsets,
                                                         const int32_t* __restrict__ counts,
                                                         const uint16_t* __restrict__ lut, uint16_t* __restrict__ out) {
     __shared__ uint32_t hist[BEV_HALF];
-    const int s = blockIdx.x, half = blockIdx.y;
-    const int64_t off = offsets[s];
+    const int s = blockIdx.x;
+    const float4* p = pts + offsets[s];
     const int n = counts[s];
     uint16_t* o = out + (int64_t)s * BEV_CELLS;
-    bev_clear(hist);
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += BEV_BLOCK) {
-        const float4 p = pts[off + i];
-        const int c = bev_index((double)p.x, (double)p.y, (double)p.z) - half * BEV_HALF;
-        if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
+    const bool packed = n <= BEV_PACKED_MAX;
+    for (int half = 0; half < (packed ? 1 : 2); half++) {
+        bev_clear(hist);
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += BEV_BLOCK) {
+            const float4 q = p[i];
+            bev_add(hist, bev_index((double)q.x, (double)q.y, (double)q.z), packed, half);
+        }
+        __syncthreads();
+        bev_emit(hist, packed, half, lut, o);
+        __syncthreads();
     }
-    __syncthreads();
-    bev_emit(hist, half, lut, o);
 }
 
 int launch_bev_scan(const float4* pts, const int64_t* offsets, const int32_t* counts, int32_t S, const uint16_t* lut,
                     uint16_t* out, hipStream_t stream) {
     if (S <= 0) return 0;
-    hipLaunchKernelGGL(k_bev_scan, dim3(S, 2), dim3(BEV_BLOCK), 0, stream, pts, offsets, counts, lut, out);
+    hipLaunchKernelGGL(k_bev_scan, dim3(S), dim3(BEV_BLOCK), 0, stream, pts, offsets, counts, lut, out);
     return (int)hipGetLastError();
 }
 
-/* Two workgroups per guess, one per half of the cells: histogram of the map
- * points around the guess, in the guess frame q = R^T (p - t).  Only the tiles under the grid's footprint are
+/* One workgroup per guess: histogram of the map points around the guess, in the
+ * guess frame q = R^T (p - t).  Only the tiles under the grid's footprint are
  * visited: the footprint's corners at both z limits, mapped to the map frame,
  * bound the tile rows and columns. */
 __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const double* __restrict__ guesses,
@@ -101,7 +125,8 @@ __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const do
                                                         int64_t ldx, int64_t col_off) {
     __shared__ uint32_t hist[BEV_HALF];
     __shared__ int s_rng[4];
-    const int b = blockIdx.x, half = blockIdx.y;
+    __shared__ int64_t s_n;
+    const int b = blockIdx.x;
     const double* G = guesses + 16 * (int64_t)b;
     const double r00 = G[0], r01 = G[1], r02 = G[2], t0 = G[3];
     const double r10 = G[4], r11 = G[5], r12 = G[6], t1 = G[7];
@@ -129,37 +154,45 @@ __global__ __launch_bounds__(BEV_BLOCK) void k_bev_crop(TileGridDev tg, const do
             ty0 = (int)fmax(fy0, 0.0);
             ty1 = (int)fmin(fy1, (double)(tg.nty - 1));
         }
+        /* points visited: a bound on every cell's count */
+        int64_t n = 0;
+        for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++)
+            n += (int64_t)tg.start[(int64_t)ty * tg.ntx + tx1 + 1] - (int64_t)tg.start[(int64_t)ty * tg.ntx + tx0];
         s_rng[0] = tx0;
         s_rng[1] = tx1;
         s_rng[2] = ty0;
         s_rng[3] = ty1;
+        s_n = n;
     }
     __syncthreads();
     const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
+    const bool packed = s_n <= BEV_PACKED_MAX;
     uint16_t* o = X + (int64_t)b * ldx + col_off;
-    bev_clear(hist);
-    __syncthreads();
-    for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
-        const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
-        const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
-        for (uint32_t i = pb + threadIdx.x; i < pe; i += BEV_BLOCK) {
-            const float4 p = tg.pts[i];
-            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
-            const double qx = r00 * dx + r10 * dy + r20 * dz;
-            const double qy = r01 * dx + r11 * dy + r21 * dz;
-            const double qz = r02 * dx + r12 * dy + r22 * dz;
-            const int c = bev_index(qx, qy, qz) - half * BEV_HALF;
-            if (c >= 0 && c < BEV_HALF) atomicAdd(&hist[c], 1u);
+    for (int half = 0; half < (packed ? 1 : 2); half++) {
+        bev_clear(hist);
+        __syncthreads();
+        for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
+            const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
+            const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
+            for (uint32_t i = pb + threadIdx.x; i < pe; i += BEV_BLOCK) {
+                const float4 p = tg.pts[i];
+                const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+                const double qx = r00 * dx + r10 * dy + r20 * dz;
+                const double qy = r01 * dx + r11 * dy + r21 * dz;
+                const double qz = r02 * dx + r12 * dy + r22 * dz;
+                bev_add(hist, bev_index(qx, qy, qz), packed, half);
+            }
         }
+        __syncthreads();
+        bev_emit(hist, packed, half, lut, o);
+        __syncthreads();
     }
-    __syncthreads();
-    bev_emit(hist, half, lut, o);
 }
 
 int launch_bev_crop(TileGridDev tg, const double* guesses, int32_t B, const uint16_t* lut, uint16_t* X, int64_t ldx,
                     int64_t col_off, hipStream_t stream) {
     if (B <= 0) return 0;
-    hipLaunchKernelGGL(k_bev_crop, dim3(B, 2), dim3(BEV_BLOCK), 0, stream, tg, guesses, lut, X, ldx, col_off);
+    hipLaunchKernelGGL(k_bev_crop, dim3(B), dim3(BEV_BLOCK), 0, stream, tg, guesses, lut, X, ldx, col_off);
     return (int)hipGetLastError();
 }
 

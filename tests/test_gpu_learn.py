@@ -38,6 +38,38 @@ def test_features_match_oracle(A, ctx, small_scene):
     assert (F[:, :32768] != 0).sum() > 1000 and (F[:, 32768:] != 0).sum() > 10000
 
 
+def test_features_of_dense_inputs_match_oracle(A, ctx):
+    """Scans and map crops of more than 65535 points take the kernels' two-pass
+    32-bit counters instead of the packed 16-bit ones; a cell of more than 65535
+    points saturates.  Both paths, and the packed one beside them, bit-exact."""
+    r = np.random.default_rng(7)
+
+    def cloud(n, lo, hi):
+        p = np.zeros((n, 4), np.float32)
+        p[:, :3] = r.uniform(lo, hi, size=(n, 3))
+        p[:, 3] = 1.0
+        return p
+
+    # one 0.5 m x 0.5 m x 1.5 m cell with 66,000 points plus 5,000 spread ones
+    dense = np.concatenate([cloud(66000, (3.05, -2.45, 0.6), (3.45, -2.05, 1.9)),
+                            cloud(5000, (-15.0, -15.0, -2.0), (15.0, 15.0, 9.0))])
+    small = cloud(20000, (-15.0, -15.0, -2.0), (15.0, 15.0, 9.0))
+    scans = [dense, small]
+    # map: 90,000 points in a 24 m x 24 m patch, and a guess in its middle
+    m = cloud(90000, (-12.0, -12.0, -2.0), (12.0, 12.0, 9.0))
+    sc = A.Scans(ctx, scans)
+    bm = A.BevMap(ctx, m)
+    g = np.stack([A.pose(0.3, -0.2, 0.1, 0.0, 0.0, 0.2), A.pose(40.0, 40.0, 0.0, 0.0, 0.0, 0.0)])
+    F = bm.features(sc, np.array([0, 1], np.int32), g)
+    lut = L.log1p_lut()
+    for b in range(2):
+        assert np.array_equal(F[b, :32768], L.scan_grid(scans[b], lut)), b
+        assert np.array_equal(F[b, 32768:], L.map_grid(m, g[b], lut)), b
+    assert F[0, :32768].max() == lut[65535]          # the saturated cell
+    assert (F[0, 32768:] != 0).sum() > 10000          # the dense crop counted
+    assert (F[1, 32768:] != 0).sum() == 0             # a footprint beside the map
+
+
 def test_features_world_origin(A, ctx, small_scene):
     """Guesses in world coordinates: the map grid is taken relative to the origin."""
     w = small_scene
