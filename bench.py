@@ -242,9 +242,13 @@ def daphne_harness_leg(cases_n: int, extent: float = 100.0, map_points: int = 10
         _harness_time(hip, run)  # warm-up: device initialisation and code loading
         out["hip_p1"], ok_h1 = _harness_time(hip, run)
         out["hip_p8"], ok_h8 = _harness_time(hip, run, ("-p", "8"))
-        out["result_ok"] = {"reference_cpu": ok_cpu, "reference_openmp": ok_omp, "hip_p1": ok_h1, "hip_p8": ok_h8}
-        if out["hip_p8"] and out["reference_openmp"]:
-            out["speedup_p8_vs_openmp"] = out["reference_openmp"] / out["hip_p8"]
+        out["hip_p16"], ok_h16 = _harness_time(hip, run, ("-p", "16"))
+        out["result_ok"] = {"reference_cpu": ok_cpu, "reference_openmp": ok_omp, "hip_p1": ok_h1, "hip_p8": ok_h8,
+                            "hip_p16": ok_h16}
+        if out["reference_openmp"]:
+            for k in ("p1", "p8", "p16"):
+                if out["hip_" + k]:
+                    out[f"speedup_{k}_vs_openmp"] = out["reference_openmp"] / out["hip_" + k]
         return out
     finally:
         shutil.rmtree(d, ignore_errors=True)
@@ -378,7 +382,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
     ap.add_argument("--plain-scans", type=int, default=128, help="scans timed one at a time (subset of the batch)")
     ap.add_argument("--cpu-sample", type=int, default=48, help="scans aligned by the CPU oracle")
-    ap.add_argument("--harness-cases", type=int, default=24,
+    ap.add_argument("--harness-cases", type=int, default=96,
                     help="DAPHNE testcases for the harness comparison with the compiled reference (0: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="batched NDT only (no pose proposer)")
@@ -394,7 +398,7 @@ def main():
     ap.add_argument("--block-size", type=int, default=0, help="fixed launch shape (with --points-per-wg): no tuning")
     ap.add_argument("--points-per-wg", type=int, default=0, help="fixed chunk size (with --block-size): no tuning")
     ap.add_argument("--cfg1-scans", type=int, default=1000, help="sequential scans of the cfg1 leg (0: off)")
-    ap.add_argument("--cfg1-harness-cases", type=int, default=6,
+    ap.add_argument("--cfg1-harness-cases", type=int, default=16,
                     help="cfg1-sized DAPHNE testcases for the reference harness comparison (0: off)")
     ap.add_argument("--precision-study", action="store_true",
                     help="only the cfg3 precision study (four arithmetic modes against the float64 oracle)")
