@@ -175,11 +175,10 @@ NDT_HD double mt_trial_value(double a_l, double f_l, double g_l, double a_u, dou
     }
 }
 
-/* ndt pose: T = [Rx(p3) Ry(p4) Rz(p5) | p0..2], the convention of angle_tables. */
-NDT_HD void pose_matrix(const double* p, double* M) {
-    double cx = cos(p[3]), sx = sin(p[3]);
-    double cy = cos(p[4]), sy = sin(p[4]);
-    double cz = cos(p[5]), sz = sin(p[5]);
+/* ndt pose: T = [Rx(p3) Ry(p4) Rz(p5) | p0..2], the convention of angle_tables,
+ * from the angles' cosines c[0..2] and sines s[0..2]. */
+NDT_HD void pose_matrix_cs(const double* p, const double* c, const double* s, double* M) {
+    const double cx = c[0], sx = s[0], cy = c[1], sy = s[1], cz = c[2], sz = s[2];
     M[0] = cy * cz;
     M[1] = -cy * sz;
     M[2] = sy;
@@ -194,6 +193,12 @@ NDT_HD void pose_matrix(const double* p, double* M) {
     M[11] = p[2];
     M[12] = M[13] = M[14] = 0.0;
     M[15] = 1.0;
+}
+
+NDT_HD void pose_matrix(const double* p, double* M) {
+    const double c[3] = {cos(p[3]), cos(p[4]), cos(p[5])};
+    const double s[3] = {sin(p[3]), sin(p[4]), sin(p[5])};
+    pose_matrix_cs(p, c, s, M);
 }
 
 /* inverse of pose_matrix for |pitch| < pi/2 */

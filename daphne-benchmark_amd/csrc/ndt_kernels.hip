@@ -871,12 +871,25 @@ struct PassParams {
 
 __device__ __attribute__((noinline)) void pass_params_f64(const double* x, const AlignParams& prm, double res,
                                                           double* out /* 32 */) {
+    /* each angle's cosine and sine once (this runs on one lane between passes,
+     * where every transcendental's latency is on the scan's critical path); the
+     * small-angle rule's zeroed angles have cosine 1 and sine 0 exactly */
+    double c[3], sn[3], ca[3], sa[3];
+    for (int k = 0; k < 3; k++) {
+        c[k] = cos(x[3 + k]);
+        sn[k] = sin(x[3 + k]);
+    }
     double M[16];
-    pose_matrix(x, M);
+    pose_matrix_cs(x, c, sn, M);
     double xa[6];
     for (int k = 0; k < 6; k++) xa[k] = (k >= 3 && fabs(x[k]) < 10e-5) ? 0.0 : x[k];
+    for (int k = 0; k < 3; k++) {
+        const bool zero = fabs(x[3 + k]) < 10e-5;
+        ca[k] = zero ? 1.0 : c[k];
+        sa[k] = zero ? 0.0 : sn[k];
+    }
     double Ma[16];
-    pose_matrix(xa, Ma);
+    pose_matrix_cs(xa, ca, sa, Ma);
     for (int r = 0; r < 3; r++) {
         for (int c = 0; c < 3; c++) {
             out[r * 3 + c] = M[r * 4 + c];
@@ -884,7 +897,7 @@ __device__ __attribute__((noinline)) void pass_params_f64(const double* x, const
         }
         out[9 + r] = M[r * 4 + 3];
     }
-    const double c3 = cos(xa[3]), s3 = sin(xa[3]), c4 = cos(xa[4]), s4 = sin(xa[4]);
+    const double c3 = ca[0], s3 = sa[0], c4 = ca[1], s4 = sa[1];
     out[21] = 0.0;
     out[22] = c3;
     out[23] = s3;
