@@ -1981,8 +1981,20 @@ __device__ inline void reduce_slots(const double* partials, int64_t base, int nc
     const int grp = threadIdx.x >> 5, j = threadIdx.x & 31;
     if (threadIdx.x < SLOT_GROUPS * 32) {
         double a = 0.0;
-        if (j < NRED)
-            for (int c = grp; c < nchunks; c += SLOT_GROUPS) a += ld_agent_f64(partials + (base + c) * 32 + j);
+        if (j < NRED) {
+            /* eight slots' loads in flight at once, added in chunk order */
+            for (int c0 = grp; c0 < nchunks; c0 += 8 * SLOT_GROUPS) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int c = c0 + u * SLOT_GROUPS;
+                    v[u] = c < nchunks ? ld_agent_f64(partials + (base + c) * 32 + j) : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (c0 + u * SLOT_GROUPS < nchunks) a += v[u];
+            }
+        }
         gred[grp][j] = a;
     }
     __syncthreads();
