@@ -1681,7 +1681,8 @@ __device__ inline int ctl_end(ScanState& st, const AlignParams& prm, int converg
     return 0;
 }
 
-__device__ __attribute__((noinline)) void ctl_newton_start(ScanState& st, const AlignParams& prm, ScanOut* out) {
+__device__ __attribute__((noinline)) void ctl_newton_start(ScanState& __restrict__ st, const AlignParams& __restrict__ prm,
+                                                           ScanOut* __restrict__ out) {
     const double mu = 1.e-4;
     for (;;) {
         if (st.iter >= prm.max_iterations) {
@@ -1747,7 +1748,8 @@ __device__ inline void ctl_accept(ScanState& st, const AlignParams& prm, ScanOut
     ctl_newton_start(st, prm, out);
 }
 
-__device__ __attribute__((noinline)) void ctl_step(ScanState& st, const double* red, const AlignParams& prm, ScanOut* out) {
+__device__ __attribute__((noinline)) void ctl_step(ScanState& __restrict__ st, const double* __restrict__ red,
+                                                   const AlignParams& __restrict__ prm, ScanOut* __restrict__ out) {
     const double mu = 1.e-4, nu = 0.9;
     st.passes++;
     st.pairs += red[NACC];
