@@ -869,16 +869,12 @@ struct PassParams {
     }
 };
 
-__device__ __attribute__((noinline)) void pass_params_f64(const double* x, const AlignParams& prm, double res,
-                                                          double* out /* 32 */) {
-    /* each angle's cosine and sine once (this runs on one lane between passes,
-     * where every transcendental's latency is on the scan's critical path); the
-     * small-angle rule's zeroed angles have cosine 1 and sine 0 exactly */
-    double c[3], sn[3], ca[3], sa[3];
-    for (int k = 0; k < 3; k++) {
-        c[k] = cos(x[3 + k]);
-        sn[k] = sin(x[3 + k]);
-    }
+/* The 32 pass parameters of pose x from the cosines c[0..2] and sines sn[0..2] of
+ * its angles x[3..5]; the small-angle rule's zeroed angles have cosine 1 and sine 0
+ * exactly. */
+__device__ inline void pass_params_from_trig(const double* x, const double* c, const double* sn,
+                                             const AlignParams& prm, double res, double* out /* 32 */) {
+    double ca[3], sa[3];
     double M[16];
     pose_matrix_cs(x, c, sn, M);
     double xa[6];
@@ -891,9 +887,9 @@ __device__ __attribute__((noinline)) void pass_params_f64(const double* x, const
     double Ma[16];
     pose_matrix_cs(xa, ca, sa, Ma);
     for (int r = 0; r < 3; r++) {
-        for (int c = 0; c < 3; c++) {
-            out[r * 3 + c] = M[r * 4 + c];
-            out[12 + r * 3 + c] = Ma[r * 4 + c];
+        for (int cc = 0; cc < 3; cc++) {
+            out[r * 3 + cc] = M[r * 4 + cc];
+            out[12 + r * 3 + cc] = Ma[r * 4 + cc];
         }
         out[9 + r] = M[r * 4 + 3];
     }
@@ -909,6 +905,32 @@ __device__ __attribute__((noinline)) void pass_params_f64(const double* x, const
     out[29] = prm.d1;
     out[30] = prm.d2;
     out[31] = -prm.d2 / 2.0;
+}
+
+__device__ __attribute__((noinline)) void pass_params_f64(const double* x, const AlignParams& prm, double res,
+                                                          double* out /* 32 */) {
+    double c[3], sn[3];
+    for (int k = 0; k < 3; k++) {
+        c[k] = cos(x[3 + k]);
+        sn[k] = sin(x[3 + k]);
+    }
+    pass_params_from_trig(x, c, sn, prm, res, out);
+}
+
+/* The pass parameters of a scan's trial pose st.x_t into st.pp, by the first wave
+ * of the workgroup after the controller (every lane of wave 0 must call this):
+ * lanes 0-2 take the angles' cosines and lanes 3-5 their sines at once, lane 0
+ * the rest.  Between two passes of a scan this is on its critical path, where
+ * six transcendentals one after the other on one lane took about 2 us. */
+__device__ __attribute__((noinline)) void wave_pass_params(ScanState& st, const AlignParams& prm,
+                                                           double* trig /* LDS, 6 */) {
+    const int lane = threadIdx.x % WAVE;
+    if (lane < 3) trig[lane] = cos(st.x_t[3 + lane]);
+    else if (lane < 6) trig[lane] = sin(st.x_t[lane]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (lane == 0) pass_params_from_trig(st.x_t, trig, trig + 3, prm, prm.resolution_d, st.pp);
 }
 
 template <typename T>
@@ -1731,7 +1753,6 @@ __device__ __attribute__((noinline)) void ctl_newton_start(ScanState& __restrict
         a_t = dmax_std(a_t, prm.trans_eps / 2);
         st.a_t = a_t;
         for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
-        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
         st.phase = PH_MT_FIRST;
         return;
     }
@@ -1764,7 +1785,6 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& __restrict__ st, c
             st.p[k] = st.alt[k];
             st.alt[k] = given;
         }
-        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
         st.fb_stage = FB_REFINED;
         st.phase = PH_INIT;
         st.run_base = st.passes;
@@ -1824,7 +1844,6 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& __restrict__ st, c
         a_t = dmax_std(a_t, prm.trans_eps / 2);
         st.a_t = a_t;
         for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
-        pass_params_f64(st.x_t, prm, prm.resolution_d, st.pp);
         st.phase = PH_MT_LOOP;
         return;
     }
@@ -2016,6 +2035,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double lds_red[BLOCK / WAVE][NRED];
     __shared__ double red[NRED];
     __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ double s_trig[6];
     __shared__ double gred[SLOT_GROUPS][32];
     __shared__ WavePairs wslots[BLOCK / WAVE];
     __shared__ uint32_t s_active[MAX_BATCH];
@@ -2111,6 +2131,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
             __syncthreads();
             reduce_slots(partials, sd.chunk_base, pass_nchunks(sd, run_pass(*(ScanState*)sstate)), gred, red);
             if (tid == 0) ctl_step(*(ScanState*)sstate, red, prm, out + s);
+            __syncthreads();
+            if (tid < WAVE && !((ScanState*)sstate)->done) wave_pass_params(*(ScanState*)sstate, prm, s_trig);
             __syncthreads();
             for (int q = tid; q < STATE_WORDS; q += BLOCK) st_agent_u64((uint64_t*)(states + s) + q, sstate[q]);
             __syncthreads();
@@ -2242,6 +2264,7 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
     __shared__ double red[NRED];
     __shared__ double gred[SLOT_GROUPS][32];
     __shared__ uint64_t sstate[STATE_WORDS];
+    __shared__ double s_trig[6];
     __shared__ WavePairs wslots[BLOCK / WAVE];
     __shared__ uint64_t s_item, s_ppkey;
     __shared__ uint32_t s_base;
@@ -2336,6 +2359,8 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
                 s_done = st.done;
                 s_next_rel = run_pass(st);
             }
+            __syncthreads();
+            if (tid < WAVE && !s_done) wave_pass_params(*(ScanState*)sstate, prm, s_trig);
             __syncthreads();
             for (int w = tid; w < STATE_WORDS; w += BLOCK) st_agent_u64((uint64_t*)(states + s) + w, sstate[w]);
             stores_complete();
