@@ -2306,11 +2306,10 @@ __global__ __launch_bounds__(BLOCK, (sizeof(T) == 8 || SUM == NDT_SUM_FP64) ? AL
         const uint64_t ppkey = item >> ITEM_CHUNK_BITS;
         const ScanDesc sd = scans[s];
         if (ppkey != s_ppkey) {
+            /* the parameters, the run's first pass and the phase in one round trip */
             if (tid < 32) pp.set(tid, ld_agent_f64(&states[s].pp[tid]));
-            if (tid == 0) {
-                s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
-                s_run_base = (uint32_t)ld_agent_i32(&states[s].run_base);
-            }
+            else if (tid == 32) s_run_base = (uint32_t)ld_agent_i32(&states[s].run_base);
+            else if (tid == 33) s_grad_only = HA && grad_only_pass(ld_agent_i32(&states[s].phase));
             __syncthreads();
             if (tid == 0) s_ppkey = ppkey;
             __syncthreads();
