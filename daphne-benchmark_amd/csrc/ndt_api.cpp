@@ -412,7 +412,7 @@ static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) 
     LAUNCH_TRY(launch_hash_clear(g->slots.as<HashSlot>(), cap, c->stream));
     BUF_TRY(g->cand_head, sizeof(float4) * (ne + 1));
     BUF_TRY(g->cand_s32, 2 * sizeof(float4) * (ne + 1));
-    BUF_TRY(g->vs64, 5 * sizeof(double2) * (nv + 1));
+    BUF_TRY(g->vs64, VS64_STRIDE * sizeof(double2) * (nv + 1));
     LAUNCH_TRY(launch_voxel_s64(g->mean_c.as<double>(), g->icov_c.as<double>(), nv, g->vs64.as<double2>(), c->stream));
     /* dense query-cell index when the map's cell box is at most 2^28 cells (2 GiB);
      * the environment variable NDT_FORCE_HASHED_INDEX=1 makes any map use the hashed

@@ -236,6 +236,9 @@ NDT_HD uint64_t hash_slot(uint64_t key, int log2cap) {
  *   vs64  5 x double2 (mx my)(mz c00)(c01 c02)(c11 c12)(c22 0): float64 mean and
  *         inverse covariance, 80 bytes per voxel (27 times less than per
  *         candidate, so that they stay in the caches). */
+/* double2 entries per voxel in vs64 (80 bytes; padding records to 128 bytes, one
+ * cache line each, made the alignment launch 1 % slower) */
+constexpr int VS64_STRIDE = 5;
 struct NdtGridDev {
     const uint2* dense;      /* query cell -> (begin, count), dense over the map's cell box, or null */
     int32_t dims[3];         /* cells per axis of the dense index */

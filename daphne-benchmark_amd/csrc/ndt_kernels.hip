@@ -761,11 +761,12 @@ __global__ void k_voxel_s64(const double* __restrict__ mean, const double* __res
     if (v >= n) return;
     const double* m = mean + 3 * v;
     const double* C = icov + 6 * v;
-    vs64[5 * v] = make_double2(m[0], m[1]);
-    vs64[5 * v + 1] = make_double2(m[2], C[0]);
-    vs64[5 * v + 2] = make_double2(C[1], C[2]);
-    vs64[5 * v + 3] = make_double2(C[3], C[4]);
-    vs64[5 * v + 4] = make_double2(C[5], 0.0);
+    double2* o = vs64 + VS64_STRIDE * v;
+    o[0] = make_double2(m[0], m[1]);
+    o[1] = make_double2(m[2], C[0]);
+    o[2] = make_double2(C[1], C[2]);
+    o[3] = make_double2(C[3], C[4]);
+    o[4] = make_double2(C[5], 0.0);
 }
 
 int launch_voxel_s64(const double* mean, const double* icov, int64_t nvalid, double2* vs64, hipStream_t stream) {
@@ -1165,7 +1166,7 @@ __device__ inline int pair_add(const NdtGridDev& g, const bool valid, const doub
         in = valid && (d0 * d0 + d1 * d1 + d2 * d2 < u.res2);
         c00 = a.x; c01 = a.y; c02 = a.z; c11 = a.w; c12 = b.x; c22 = b.y;
     } else {
-        const double2* r = g.vs64 + 5 * (int64_t)idx;
+        const double2* r = g.vs64 + VS64_STRIDE * (int64_t)idx;
         const double2 m01 = r[0], m2c0 = r[1], c12v = r[2], c34 = r[3], c5p = r[4];
         d0 = xd0 - m01.x;
         d1 = xd1 - m01.y;
