@@ -91,11 +91,13 @@ def algorithmic_work(results, counts, pairs):
     return 16.0 * pts + 36.0 * pairs, 133.0 * pts + 340.0 * pairs
 
 
-def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_decay, seed=5):
+def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_decay, rp=None, seed=5):
     """Training triples from scans of the workload's map at other sensor poses
     (generator seeds disjoint from the batch's), labelled with NDT run from the
     true pose; half the guesses within +-3 m / +-10 deg yaw, half within
-    +-1 m / +-3 deg."""
+    +-1 m / +-3 deg.  With a registration (rp) the guesses are registered first, so
+    that the model learns the correction of the registered guess, and the model
+    gets the same setting."""
     rng = np.random.default_rng(seed)
     ds, t_gen, t_lab = None, 0.0, 0.0
     chunk = 10000
@@ -117,12 +119,15 @@ def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_deca
         g1 = A.perturb_poses(tr_gt[idx], rng)
         g2 = A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0)
         guesses = np.where(wide[:, None, None], g1, g2)
+        if rp is not None:
+            guesses = bm.register(trS, idx, guesses, rp)[0]
         if ds is None:
             ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
         else:
             ds.add(trS, idx, guesses, labels[idx])
         del trS, tr
     model = A.Model(ctx, seed=1)
+    model.set_register(rp)
     t = time.time()
     losses = model.train(ds, A.train_params(epochs=epochs, batch=512, weight_decay=weight_decay))
     t_train = time.time() - t
@@ -391,6 +396,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--weight-decay", type=float, default=0.05)
     ap.add_argument("--policy", choices=["race", "fallback"], default="fallback")
+    ap.add_argument("--no-register", action="store_true",
+                    help="no BEV registration in front of the model (it then corrects the given guess)")
     ap.add_argument("--cfg4-scans", type=int, default=2048, help="held-out scans of the learned evaluation (0: off)")
     ap.add_argument("--full-pose", action="store_true",
                     help="refined guesses take the model's rotation too (default: its translation only)")
@@ -458,8 +465,12 @@ def main():
     learned = None
     if not args.no_learned:
         bm = A.BevMap(ctx, w.map_xyzw, w.origin)
+        rp = None if args.no_register else A.register_params()
         model, tinfo = train_proposer(ctx, w, grid, bm, prm, args.train_scans, args.train_per_scan, args.epochs,
-                                      args.weight_decay)
+                                      args.weight_decay, rp)
+        if rp is not None:
+            tinfo["registration"] = dict(radius_cells=rp.radius, yaws=rp.yaws, yaw_step_deg=float(np.degrees(rp.yaw_step)),
+                                         refine_yaw=bool(rp.refine_yaw))
         log(f"proposer trained: {tinfo}")
         policy = A.LEARNED_RACE if args.policy == "race" else A.LEARNED_FALLBACK
         if not args.full_pose:
@@ -488,8 +499,12 @@ def main():
             upload_s = (time.perf_counter() - t_u) / args.steps
         refined = model.predict(bm, scans, w.guesses)
         if not args.full_pose:
-            refined[:, :3, :3] = w.guesses[:, :3, :3]
-        policy_name = args.policy + ("" if args.full_pose else ", translation from the model")
+            # the translation-only start keeps the rotation of the guess the model
+            # corrects: the registered one with a registration
+            base = w.guesses if rp is None else bm.register(scans, np.arange(S, dtype=np.int32), w.guesses, rp)[0]
+            refined[:, :3, :3] = base[:, :3, :3]
+        policy_name = args.policy + ("" if args.full_pose else ", translation from the model") + \
+            ("" if args.no_register else ", BEV registration before the model")
         learned = dict(model=model, bm=bm, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
                        align_ms=float(np.mean(align_ms_l)), pairs=pairs_l, refined=refined, train=tinfo,
                        policy=policy_name, upload_s=upload_s, policy_flags=policy)

@@ -83,6 +83,42 @@ int pick_splits(int M, int N, int K, int cus) {
 
 } // namespace
 
+/* ------------------------------------------------------------------ registration settings */
+
+static bool reg_on(const ndt_register_params& rp) { return rp.radius > 0; }
+
+static int reg_check(const ndt_register_params* rp) {
+    if (!rp) return fail(NDT_ERR_INVALID, "null registration parameters");
+    if (rp->radius < 0 || rp->radius > NDT_REGISTER_MAX_RADIUS)
+        return fail(NDT_ERR_INVALID, "registration radius must be 0..8 cells");
+    if (rp->radius == 0) return 0;
+    if (rp->yaws < 1 || rp->yaws > NDT_REGISTER_MAX_YAWS || rp->yaws % 2 == 0)
+        return fail(NDT_ERR_INVALID, "registration yaws must be odd, 1..11");
+    if (rp->yaws > 1 && !(rp->yaw_step > 0.0 && rp->yaw_step <= 0.05))
+        return fail(NDT_ERR_INVALID, "registration yaw_step must be in (0, 0.05] rad");
+    if (rp->refine_yaw != 0 && rp->refine_yaw != 1) return fail(NDT_ERR_INVALID, "refine_yaw must be 0 or 1");
+    return 0;
+}
+
+/* the kernel's view: cosine and sine of each searched yaw (k - (yaws - 1) / 2) * step */
+static RegisterDev reg_dev(const ndt_register_params& rp) {
+    RegisterDev d{};
+    d.radius = rp.radius;
+    d.yaws = rp.yaws;
+    d.refine_yaw = rp.refine_yaw;
+    d.yaw_step = rp.yaw_step;
+    for (int k = 0; k < rp.yaws; k++) {
+        const double th = (double)(k - (rp.yaws - 1) / 2) * rp.yaw_step;
+        d.c[k] = std::cos(th);
+        d.s[k] = std::sin(th);
+    }
+    return d;
+}
+
+static bool same_reg(const ndt_register_params& a, const ndt_register_params& b) {
+    return a.radius == b.radius && a.yaws == b.yaws && a.yaw_step == b.yaw_step && a.refine_yaw == b.refine_yaw;
+}
+
 /* ------------------------------------------------------------------ handles */
 
 struct ndt_bevmap {
@@ -108,12 +144,17 @@ struct ndt_model {
      * offsets and counts, the feature rows' scan indices, the world and map-local
      * guesses, and the refined guesses */
     DevBuf sbev, idx, gl, off, cnt, gw, refined;
+    /* the registration before the model (radius 0: none) and the registered world
+     * guesses the refinement starts from */
+    ndt_register_params reg{};
+    DevBuf greg;
     /* the captured learned step of the last (scans, grid, BEV map, parameters) */
     struct LearnedPlan {
         bool valid = false;
         uint64_t scans = 0, grid = 0, bm = 0, wait = 0;
         int device = -1, keep = 0;
         ndt_params params{};
+        ndt_register_params reg{};
         std::vector<const void*> bufs; /* workspace the graphs refer to */
         /* two graphs launched back to back: the prediction, then the alignment
          * (events between them time the align kernel) */
@@ -381,6 +422,29 @@ extern "C" int ndt_model_dims(const ndt_model* md, int32_t dims[3]) {
     return 0;
 }
 
+extern "C" void ndt_register_params_default(ndt_register_params* rp) {
+    if (!rp) return;
+    *rp = ndt_register_params{};
+    rp->radius = 6;
+    rp->yaws = 9;
+    rp->yaw_step = 0.75 * 3.14159265358979323846 / 180.0;
+    rp->refine_yaw = 1;
+}
+
+extern "C" int ndt_model_set_register(ndt_model* md, const ndt_register_params* rp) {
+    if (!md) return fail(NDT_ERR_INVALID, "null model");
+    if (int e = reg_check(rp)) return e;
+    md->reg = *rp;
+    md->reg.pad = 0;
+    return 0;
+}
+
+extern "C" int ndt_model_get_register(const ndt_model* md, ndt_register_params* rp) {
+    if (!md || !rp) return fail(NDT_ERR_INVALID, "null argument");
+    *rp = md->reg;
+    return 0;
+}
+
 extern "C" int ndt_model_get(const ndt_model* md, int32_t which, float* out, int64_t n) {
     if (!md || !out) return fail(NDT_ERR_INVALID, "null argument");
     if (which < 0 || which > 5 || n != tensor_size(md, which)) return fail(NDT_ERR_INVALID, "bad tensor or size");
@@ -412,6 +476,8 @@ extern "C" int ndt_model_set(ndt_model* md, int32_t which, const float* in, int6
 }
 
 static const char MODEL_MAGIC[8] = {'N', 'D', 'T', 'M', 'L', 'P', '0', '1'};
+/* optional trailer after the tensors: the registration setting */
+static const char REG_MAGIC[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '1'};
 
 extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
     if (!md || !path) return fail(NDT_ERR_INVALID, "null argument");
@@ -426,6 +492,11 @@ extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
             return NDT_ERR_HIP;
         }
         ok = std::fwrite(h.data(), 4, h.size(), f) == h.size();
+    }
+    if (ok && reg_on(md->reg)) {
+        const int32_t ri[3] = {md->reg.radius, md->reg.yaws, md->reg.refine_yaw};
+        ok = std::fwrite(REG_MAGIC, 1, 8, f) == 8 && std::fwrite(ri, 4, 3, f) == 3 &&
+             std::fwrite(&md->reg.yaw_step, 8, 1, f) == 1;
     }
     ok = (std::fclose(f) == 0) && ok;
     return ok ? 0 : fail(NDT_ERR_INVALID, std::string("short write to ") + path);
@@ -453,6 +524,19 @@ extern "C" int ndt_model_load(ndt_ctx* c, const char* path, ndt_model** out) {
         std::vector<float> h((size_t)tensor_size(md, k));
         if (std::fread(h.data(), 4, h.size(), f) != h.size()) e = fail(NDT_ERR_INVALID, "truncated model file");
         else e = upload_tensor(md, k, h.data());
+    }
+    char tag[8];
+    if (!e && std::fread(tag, 1, 8, f) == 8) {
+        int32_t ri[3];
+        ndt_register_params rp{};
+        if (std::memcmp(tag, REG_MAGIC, 8) || std::fread(ri, 4, 3, f) != 3 || std::fread(&rp.yaw_step, 8, 1, f) != 1)
+            e = fail(NDT_ERR_INVALID, "bad registration trailer in model file");
+        if (!e) {
+            rp.radius = ri[0];
+            rp.yaws = ri[1];
+            rp.refine_yaw = ri[2];
+            if (!(e = reg_check(&rp))) md->reg = rp;
+        }
     }
     std::fclose(f);
     if (!e) e = refresh_bf16(md);
@@ -654,6 +738,49 @@ extern "C" int ndt_features(ndt_ctx* c, const ndt_bevmap* bm, const ndt_scans* s
     return 0;
 }
 
+extern "C" int ndt_bev_register(ndt_ctx* c, const ndt_bevmap* bm, const ndt_scans* sc, const int32_t* scan_index,
+                                const double* guesses, int32_t B, const ndt_register_params* rp, double* out_guesses,
+                                int32_t* out_score) {
+    if (!c || !bm || !sc || !scan_index || !guesses || !out_guesses) return fail(NDT_ERR_INVALID, "null argument");
+    if (B <= 0) return fail(NDT_ERR_INVALID, "B must be positive");
+    if (int e = reg_check(rp)) return e;
+    for (int32_t b = 0; b < B; b++)
+        if (scan_index[b] < 0 || scan_index[b] >= sc->S) return fail(NDT_ERR_INVALID, "scan index out of range");
+    if (int e = check_poses(guesses, B, "guess")) return e;
+    if (!reg_on(*rp)) {
+        std::memcpy(out_guesses, guesses, 16 * sizeof(double) * (size_t)B);
+        if (out_score) std::memset(out_score, 0, 4 * (size_t)B);
+        return 0;
+    }
+    HIP_TRY(hipSetDevice(c->device));
+    DevBuf sbev, off, cnt, idx, gl, gw, score;
+    hipStream_t st = c->stream;
+    BUF_TRY(sbev, 2 * (size_t)sc->S * BEV_CELLS);
+    BUF_TRY(off, 8 * (size_t)sc->S);
+    BUF_TRY(cnt, 4 * (size_t)sc->S);
+    std::vector<int32_t> counts((size_t)sc->S);
+    for (int32_t s = 0; s < sc->S; s++) counts[(size_t)s] = (int32_t)sc->counts[(size_t)s];
+    HIP_TRY(hipMemcpyAsync(off.p, sc->offsets.data(), 8 * (size_t)sc->S, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(cnt.p, counts.data(), 4 * (size_t)sc->S, hipMemcpyHostToDevice, st));
+    LAUNCH_TRY(launch_bev_scan(sc->pts.as<float4>(), off.as<int64_t>(), cnt.as<int32_t>(), sc->S,
+                               bm->lut.as<uint16_t>(), sbev.as<uint16_t>(), st));
+    std::vector<double> gl_h;
+    guesses_local(bm, guesses, B, gl_h);
+    BUF_TRY(idx, 4 * (size_t)B);
+    BUF_TRY(gl, 8 * gl_h.size());
+    BUF_TRY(gw, 8 * gl_h.size());
+    BUF_TRY(score, 4 * (size_t)B);
+    HIP_TRY(hipMemcpyAsync(idx.p, scan_index, 4 * (size_t)B, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(gl.p, gl_h.data(), 8 * gl_h.size(), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(gw.p, guesses, 8 * gl_h.size(), hipMemcpyHostToDevice, st));
+    LAUNCH_TRY(launch_bev_register(bm->dev, sbev.as<uint16_t>(), idx.as<int32_t>(), gl.as<double>(), gw.as<double>(), B,
+                                   reg_dev(*rp), gl.as<double>(), gw.as<double>(), score.as<int32_t>(), st));
+    HIP_TRY(hipMemcpyAsync(out_guesses, gw.p, 8 * gl_h.size(), hipMemcpyDeviceToHost, st));
+    if (out_score) HIP_TRY(hipMemcpyAsync(out_score, score.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
 /* ------------------------------------------------------------------ training */
 
 extern "C" void ndt_train_params_default(ndt_train_params* tp) {
@@ -840,6 +967,7 @@ static int predict_prepare(ndt_model* md, const ndt_scans* sc) {
     BUF_TRY(md->gl, 8 * 16 * (size_t)Bp);
     BUF_TRY(md->gw, 8 * 16 * (size_t)S);
     BUF_TRY(md->refined, 8 * 16 * (size_t)S);
+    BUF_TRY(md->greg, 8 * 16 * (size_t)S);
     std::vector<int32_t> cnt((size_t)S), idx((size_t)Bp);
     for (int32_t k = 0; k < S; k++) cnt[(size_t)k] = (int32_t)sc->counts[(size_t)k];
     for (int b = 0; b < Bp; b++) idx[(size_t)b] = b < S ? b : 0;
@@ -860,13 +988,20 @@ static int predict_enqueue(ndt_model* md, const ndt_bevmap* bm, const ndt_scans*
                                bm->lut.as<uint16_t>(), md->sbev.as<uint16_t>(), st));
     LAUNCH_TRY(launch_guesses_local(md->gw.as<double>(), S, Bp, bm->origin[0], bm->origin[1], bm->origin[2],
                                     md->gl.as<double>(), st));
+    /* registration: the map-local guesses of the real rows in place, the world
+     * guesses into greg (gw keeps the given guesses, plain NDT's starts) */
+    const bool reg = reg_on(md->reg);
+    if (reg)
+        LAUNCH_TRY(launch_bev_register(bm->dev, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(),
+                                       md->gw.as<double>(), S, reg_dev(md->reg), md->gl.as<double>(),
+                                       md->greg.as<double>(), nullptr, st));
     int e = batch_features(md, bm, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(), Bp);
     if (e) return e;
     if ((e = forward_dev(md, Bp, false))) return e;
     CorrScale scale;
     for (int k = 0; k < NOUT; k++) scale.v[k] = CORR_SCALE[k];
-    LAUNCH_TRY(launch_refine(md->Y.as<float>(), md->gw.as<double>(), S, scale, keep_rotation,
-                             md->refined.as<double>(), st));
+    LAUNCH_TRY(launch_refine(md->Y.as<float>(), reg ? md->greg.as<double>() : md->gw.as<double>(), S, scale,
+                             keep_rotation, md->refined.as<double>(), st));
     return 0;
 }
 
@@ -888,7 +1023,8 @@ static int predict_dev(ndt_ctx* c, ndt_model* md, const ndt_bevmap* bm, const nd
 static std::vector<const void*> plan_bufs(const ndt_model* md) {
     return {md->X.p, md->P1.p, md->H1.p, md->P2.p, md->H2.p, md->Y.p, md->wb1.p, md->wb2.p, md->wb3.p,
             md->w[NDT_B1].p, md->w[NDT_B2].p, md->w[NDT_B3].p, md->sbev.p, md->off.p, md->cnt.p, md->idx.p,
-            md->gl.p, md->gw.p, md->refined.p, (const void*)(intptr_t)md->splits1, (const void*)(intptr_t)md->splits2};
+            md->gl.p, md->gw.p, md->refined.p, md->greg.p, (const void*)(intptr_t)md->splits1,
+            (const void*)(intptr_t)md->splits2};
 }
 
 /* Captures the learned step with the fallback policy for a batch (once per scans,
@@ -901,7 +1037,8 @@ static int learned_plan_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap*
                                 const ndt_scans* sc, const ndt_params* p, int keep) {
     auto& lp = md->plan;
     if (lp.valid && lp.scans == sc->serial && lp.grid == g->serial && lp.bm == bm->serial && lp.device == c->device &&
-        lp.keep == keep && ndt::same_params(lp.params, *p) && lp.wait == ndt::wait_limit() && lp.bufs == plan_bufs(md))
+        lp.keep == keep && ndt::same_params(lp.params, *p) && lp.wait == ndt::wait_limit() && lp.bufs == plan_bufs(md) &&
+        same_reg(lp.reg, md->reg))
         return 0;
     lp.release();
     int e = predict_prepare(md, sc);
@@ -938,6 +1075,7 @@ static int learned_plan_prepare(ndt_ctx* c, const ndt_grid* g, const ndt_bevmap*
     lp.params = *p;
     lp.wait = ndt::wait_limit();
     lp.bufs = plan_bufs(md);
+    lp.reg = md->reg;
     return 0;
 }
 

@@ -196,6 +196,220 @@ int launch_bev_crop(TileGridDev tg, const double* guesses, int32_t B, const uint
     return (int)hipGetLastError();
 }
 
+/* ------------------------------------------------------------------ registration */
+
+constexpr int WAVE = 64;
+constexpr int REG_WORDS = BEV_CELLS / 32;        /* occupancy bits of one grid as 32-bit words */
+constexpr int REG_ROWS = BEV_NZ * BEV_NY;        /* 64-cell rows (z, y) of a grid */
+constexpr int REG_SIDE = 2 * REG_MAX_RADIUS + 1; /* shifts per axis at the largest radius */
+
+/* Offset of the vertex of the parabola through (-1, a), (0, b), (1, c) when it
+ * opens downwards, else 0; with b the largest of the three it lies in [-0.5, 0.5]. */
+__device__ inline double reg_parabola(double a, double b, double c) {
+    const double d = a - 2.0 * b + c;
+    return d < 0.0 ? 0.5 * (a - c) / d : 0.0;
+}
+
+/* Cosine and sine of a small angle (|d| <= 0.025 rad) from their Taylor series up
+ * to d^8 / 8! and d^7 / 7!, term by term (truncation error below 1e-19). */
+__device__ inline void reg_small_cos_sin(double d, double& c, double& s) {
+    const double d2 = d * d;
+    c = 1.0 - d2 / 2.0 + d2 * d2 / 24.0 - d2 * d2 * d2 / 720.0 + d2 * d2 * d2 * d2 / 40320.0;
+    s = d - d * d2 / 6.0 + d * d2 * d2 / 120.0 - d * d2 * d2 * d2 / 5040.0;
+}
+
+/* One workgroup per (scan, guess) row.  The map points under the footprints of the
+ * yaw-rotated grids are binned once into one occupancy bitmap per yaw (LDS): a
+ * point's guess-frame position q = R^T (p - t) is rotated by each yaw,
+ * q_k = (c_k qx + s_k qy, c_k qy - s_k qx, qz).  A 64-cell row (z, y) of a bitmap
+ * is one 64-bit word (bit x = cell x), so the score of a shift (dx, dy) is a sum
+ * of popcounts of scan row AND shifted map row over the scan's non-empty rows.
+ * The best candidate: highest score, then the lowest rank (nearest the guess:
+ * |yaw index offset|, |dy|, |dx|, negative before positive), packed with the
+ * candidate's index into one 64-bit key whose maximum is taken over the block. */
+__global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, const uint16_t* __restrict__ sbev,
+                                                            const int32_t* __restrict__ idx, const double* gl,
+                                                            const double* gw, RegisterDev rp, double* gl_out,
+                                                            double* gw_out, int32_t* __restrict__ score_out) {
+    __shared__ uint32_t occ[REG_MAX_YAWS * REG_WORDS];
+    __shared__ uint64_t srow[REG_ROWS];
+    __shared__ uint16_t rows[REG_ROWS];
+    __shared__ uint32_t sc[REG_MAX_YAWS * REG_SIDE * REG_SIDE];
+    __shared__ uint64_t wbest[REG_BLOCK / WAVE];
+    __shared__ double s_g[12], s_c[REG_MAX_YAWS], s_s[REG_MAX_YAWS];
+    __shared__ int s_rng[4], s_nrows;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    const int K = rp.yaws, R = rp.radius, W = 2 * R + 1, kc = (K - 1) / 2;
+    if (tid < 12) s_g[tid] = gl[16 * (int64_t)b + tid];
+    if (tid < K) {
+        s_c[tid] = rp.c[tid];
+        s_s[tid] = rp.s[tid];
+    }
+    for (int i = tid; i < K * REG_WORDS; i += REG_BLOCK) occ[i] = 0u;
+    /* the scan's occupied cells, one ballot per row */
+    const uint16_t* sg = sbev + (int64_t)idx[b] * BEV_CELLS;
+    for (int r = wave; r < REG_ROWS; r += REG_BLOCK / WAVE) {
+        const uint64_t m = __ballot(sg[r * BEV_NX + lane] != 0);
+        if (lane == 0) srow[r] = m;
+    }
+    __syncthreads();
+    const double r00 = s_g[0], r01 = s_g[1], r02 = s_g[2], t0 = s_g[3];
+    const double r10 = s_g[4], r11 = s_g[5], r12 = s_g[6], t1 = s_g[7];
+    const double r20 = s_g[8], r21 = s_g[9], r22 = s_g[10], t2 = s_g[11];
+    if (tid == 0) {
+        /* tiles under the union of the yaw-rotated footprints (corners at both z
+         * limits, mapped to the map frame), clamped to the tile grid */
+        double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+        for (int k = 0; k < K; k++)
+            for (int v = 0; v < 8; v++) {
+                const double ax = (v & 1) ? -BEV_X0 : BEV_X0;
+                const double ay = (v & 2) ? -BEV_Y0 : BEV_Y0;
+                const double qz = (v & 4) ? BEV_Z0 + BEV_NZ * BEV_DZ : BEV_Z0;
+                const double qx = s_c[k] * ax - s_s[k] * ay, qy = s_s[k] * ax + s_c[k] * ay;
+                const double px = r00 * qx + r01 * qy + r02 * qz + t0;
+                const double py = r10 * qx + r11 * qy + r12 * qz + t1;
+                xmin = fmin(xmin, px);
+                xmax = fmax(xmax, px);
+                ymin = fmin(ymin, py);
+                ymax = fmax(ymax, py);
+            }
+        double fx0 = floor((xmin - 0.5 - tg.x0) / MAP_TILE), fx1 = floor((xmax + 0.5 - tg.x0) / MAP_TILE);
+        double fy0 = floor((ymin - 0.5 - tg.y0) / MAP_TILE), fy1 = floor((ymax + 0.5 - tg.y0) / MAP_TILE);
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        if (fx1 >= 0.0 && fx0 < (double)tg.ntx && fy1 >= 0.0 && fy0 < (double)tg.nty) {
+            tx0 = (int)fmax(fx0, 0.0);
+            tx1 = (int)fmin(fx1, (double)(tg.ntx - 1));
+            ty0 = (int)fmax(fy0, 0.0);
+            ty1 = (int)fmin(fy1, (double)(tg.nty - 1));
+        }
+        s_rng[0] = tx0;
+        s_rng[1] = tx1;
+        s_rng[2] = ty0;
+        s_rng[3] = ty1;
+    }
+    if (wave == 0) {
+        /* the scan's non-empty rows, in order */
+        int n = 0;
+        for (int r0 = 0; r0 < REG_ROWS; r0 += WAVE) {
+            const bool ne = srow[r0 + lane] != 0;
+            const uint64_t bal = __ballot(ne);
+            const int pos = n + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u));
+            if (ne) rows[pos] = (uint16_t)(r0 + lane);
+            n += __popcll(bal);
+        }
+        if (lane == 0) s_nrows = n;
+    }
+    __syncthreads();
+    const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
+    for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
+        const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
+        const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
+        for (uint32_t i = pb + tid; i < pe; i += REG_BLOCK) {
+            const float4 p = tg.pts[i];
+            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+            const double qx = r00 * dx + r10 * dy + r20 * dz;
+            const double qy = r01 * dx + r11 * dy + r21 * dz;
+            const double qz = r02 * dx + r12 * dy + r22 * dz;
+            const double fz = floor((qz - BEV_Z0) / BEV_DZ);
+            if (!(fz >= 0.0 && fz < (double)BEV_NZ)) continue;
+            for (int k = 0; k < K; k++) {
+                const double qxk = s_c[k] * qx + s_s[k] * qy, qyk = s_c[k] * qy - s_s[k] * qx;
+                const double fx = floor((qxk - BEV_X0) * 2.0), fy = floor((qyk - BEV_Y0) * 2.0);
+                if (fx >= 0.0 && fx < (double)BEV_NX && fy >= 0.0 && fy < (double)BEV_NY) {
+                    const int c = ((int)fz * BEV_NY + (int)fy) * BEV_NX + (int)fx;
+                    atomicOr(&occ[k * REG_WORDS + (c >> 5)], 1u << (c & 31));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    /* scores of every (yaw, dy, dx) */
+    const int ncand = K * W * W, nrows = s_nrows;
+    for (int n = tid; n < ncand; n += REG_BLOCK) {
+        const int k = n / (W * W), rem = n % (W * W), dy = rem / W - R, dx = rem % W - R;
+        const uint32_t* ok = occ + k * REG_WORDS;
+        uint32_t acc = 0;
+        for (int j = 0; j < nrows; j++) {
+            const int r = rows[j];
+            const int y = (r & (BEV_NY - 1)) + dy;
+            if (y < 0 || y >= BEV_NY) continue;
+            const int mr = (r & ~(BEV_NY - 1)) + y;
+            uint64_t m = ((uint64_t)ok[2 * mr + 1] << 32) | (uint64_t)ok[2 * mr];
+            m = dx >= 0 ? m >> dx : m << (-dx);
+            acc += (uint32_t)__popcll(srow[r] & m);
+        }
+        sc[n] = acc;
+    }
+    __syncthreads();
+    uint64_t best = 0;
+    for (int n = tid; n < ncand; n += REG_BLOCK) {
+        const int k = n / (W * W), rem = n % (W * W), dy = rem / W - R, dx = rem % W - R;
+        const uint32_t rank =
+            (((((uint32_t)abs(k - kc) * 2u + (k < kc ? 1u : 0u)) * 16u + (uint32_t)abs(dy)) * 2u + (dy < 0 ? 1u : 0u)) *
+                 16u + (uint32_t)abs(dx)) * 2u + (dx < 0 ? 1u : 0u);
+        const uint64_t key = ((uint64_t)sc[n] << 32) | ((uint64_t)(0x3fffu - rank) << 12) | (uint64_t)n;
+        best = key > best ? key : best;
+    }
+    for (int off = WAVE / 2; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+    }
+    if (lane == 0) wbest[wave] = best;
+    __syncthreads();
+    if (tid != 0) return;
+    for (int w = 1; w < REG_BLOCK / WAVE; w++) best = wbest[w] > best ? wbest[w] : best;
+    const uint32_t v = (uint32_t)(best >> 32);
+    const int n = (int)(best & 0xfffu);
+    const int k = n / (W * W), rem = n % (W * W), dy = rem / W - R, dx = rem % W - R;
+    double* outs[2] = {gl_out + 16 * (int64_t)b, gw_out + 16 * (int64_t)b};
+    const double* ins[2] = {gl + 16 * (int64_t)b, gw + 16 * (int64_t)b};
+    double G[2][16];
+    for (int q = 0; q < 2; q++)
+        for (int e = 0; e < 16; e++) G[q][e] = ins[q][e];
+    if (v != 0) {
+        const uint32_t* S = sc + k * W * W;
+        const int i0 = (dy + R) * W + (dx + R);
+        const double ox = abs(dx) < R ? reg_parabola((double)S[i0 - 1], (double)S[i0], (double)S[i0 + 1]) : 0.0;
+        const double oy = abs(dy) < R ? reg_parabola((double)S[i0 - W], (double)S[i0], (double)S[i0 + W]) : 0.0;
+        const double ck = s_c[k], sk = s_s[k];
+        double c = ck, s = sk;
+        if (rp.refine_yaw && k > 0 && k < K - 1) {
+            const double o = reg_parabola((double)sc[i0 + (k - 1) * W * W], (double)S[i0],
+                                          (double)sc[i0 + (k + 1) * W * W]);
+            double cd, sd;
+            reg_small_cos_sin(o * rp.yaw_step, cd, sd);
+            c = ck * cd - sk * sd;
+            s = sk * cd + ck * sd;
+        }
+        const double d0 = ((double)dx + ox) * 0.5, d1 = ((double)dy + oy) * 0.5;
+        for (int q = 0; q < 2; q++) {
+            double* g = G[q];
+            for (int r = 0; r < 3; r++) {
+                const double a = g[4 * r], bb = g[4 * r + 1];
+                /* translation along the axes of the yaw-k frame, then the rotation */
+                const double ak = a * ck + bb * sk, bk = bb * ck - a * sk;
+                g[4 * r + 3] = g[4 * r + 3] + ak * d0 + bk * d1;
+                g[4 * r] = a * c + bb * s;
+                g[4 * r + 1] = bb * c - a * s;
+            }
+        }
+    }
+    for (int q = 0; q < 2; q++)
+        for (int e = 0; e < 16; e++) outs[q][e] = G[q][e];
+    if (score_out) score_out[b] = (int32_t)v;
+}
+
+int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const int32_t* idx, const double* gl, const double* gw,
+                        int32_t B, const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score,
+                        hipStream_t stream) {
+    if (B <= 0) return 0;
+    if (rp.radius < 0 || rp.radius > REG_MAX_RADIUS || rp.yaws < 1 || rp.yaws > REG_MAX_YAWS) return (int)hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bev_register, dim3(B), dim3(REG_BLOCK), 0, stream, tg, sbev, idx, gl, gw, rp, gl_out, gw_out,
+                       score);
+    return (int)hipGetLastError();
+}
+
 /* dst row b = src row idx[b] (ncols u16 values, ncols a multiple of 8) */
 __global__ void k_gather_rows(const uint16_t* __restrict__ src, const int32_t* __restrict__ idx, int64_t ncols,
                               uint16_t* __restrict__ dst, int64_t ldd) {

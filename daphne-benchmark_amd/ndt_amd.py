@@ -410,6 +410,11 @@ class TrainParams(C.Structure):
                 ("weight_decay", C.c_float), ("cosine", C.c_int32)]
 
 
+class RegisterParams(C.Structure):
+    _fields_ = [("radius", C.c_int32), ("yaws", C.c_int32), ("yaw_step", C.c_double), ("refine_yaw", C.c_int32),
+                ("pad", C.c_int32)]
+
+
 class NdtError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"ndt error {code}: {msg}")
@@ -475,6 +480,11 @@ def hip_lib():
         L.ndt_model_train.argtypes = [P, P, P, C.POINTER(TrainParams), P]
         L.ndt_model_predict.argtypes = [P, P, P, P, P, P]
         L.ndt_align_scans_learned.argtypes = [P, P, P, P, P, P, C.POINTER(NdtParams), C.c_int32, P, P]
+        L.ndt_register_params_default.argtypes = [C.POINTER(RegisterParams)]
+        L.ndt_register_params_default.restype = None
+        L.ndt_bev_register.argtypes = [P, P, P, P, P, C.c_int32, C.POINTER(RegisterParams), P, P]
+        L.ndt_model_set_register.argtypes = [P, C.POINTER(RegisterParams)]
+        L.ndt_model_get_register.argtypes = [P, C.POINTER(RegisterParams)]
         _hip_lib = L
     return _hip_lib
 
@@ -489,7 +499,9 @@ LEARN_SYMBOLS = ("ndt_bevmap_build", "ndt_bevmap_destroy", "ndt_features", "ndt_
                  "ndt_model_forward", "ndt_model_backward", "ndt_model_grad", "ndt_model_adam_step",
                  "ndt_train_params_default", "ndt_dataset_create", "ndt_dataset_destroy", "ndt_dataset_add",
                  "ndt_dataset_size", "ndt_model_train",
-                 "ndt_model_predict", "ndt_align_scans_learned", "ndt_ctx_last_step_time")
+                 "ndt_model_predict", "ndt_align_scans_learned", "ndt_ctx_last_step_time",
+                 "ndt_register_params_default", "ndt_bev_register", "ndt_model_set_register",
+                 "ndt_model_get_register")
 
 
 def _check(rc):
@@ -809,6 +821,18 @@ def train_params(**kw) -> TrainParams:
     return tp
 
 
+def register_params(**kw) -> RegisterParams:
+    """BEV registration settings (include/ndt_learn.h): the defaults, overridden by kw."""
+    rp = RegisterParams()
+    hip_lib().ndt_register_params_default(C.byref(rp))
+    for k, v in kw.items():
+        setattr(rp, k, v)
+    return rp
+
+
+REGISTER_OFF = dict(radius=0)
+
+
 class _Handle:
     _destroy = None
 
@@ -845,6 +869,18 @@ class BevMap(_Handle):
         _check(hip_lib().ndt_features(self.ctx.h, self.h, scans.h, idx.ctypes.data, g.ctypes.data, B,
                                       out.ctypes.data))
         return out
+
+    def register(self, scans: "Scans", scan_index, guesses, rp: RegisterParams | None = None):
+        """Registered guesses (B, 4, 4) and best scores (B,) of (scan, guess) pairs."""
+        rp = rp or register_params()
+        idx = np.ascontiguousarray(scan_index, dtype=np.int32)
+        B = idx.shape[0]
+        g = np.ascontiguousarray(guesses, dtype=np.float64).reshape(B, 16)
+        out = np.zeros_like(g)
+        score = np.zeros(B, np.int32)
+        _check(hip_lib().ndt_bev_register(self.ctx.h, self.h, scans.h, idx.ctypes.data, g.ctypes.data, B,
+                                          C.byref(rp), out.ctypes.data, score.ctypes.data))
+        return out.reshape(B, 4, 4), score
 
 
 class Model(_Handle):
@@ -883,6 +919,16 @@ class Model(_Handle):
 
     def save(self, path):
         _check(hip_lib().ndt_model_save(self.h, os.fsencode(path)))
+
+    def set_register(self, rp: RegisterParams | None):
+        """The BEV registration in front of the model (None: none)."""
+        rp = rp if rp is not None else register_params(**REGISTER_OFF)
+        _check(hip_lib().ndt_model_set_register(self.h, C.byref(rp)))
+
+    def get_register(self) -> RegisterParams:
+        rp = RegisterParams()
+        _check(hip_lib().ndt_model_get_register(self.h, C.byref(rp)))
+        return rp
 
     def forward(self, x_bf16) -> np.ndarray:
         x = np.ascontiguousarray(x_bf16, dtype=np.uint16)

@@ -74,8 +74,33 @@ typedef struct ndt_train_params {
  *                        of the two guesses (passes count every pass spent).
  * Either may be combined (bitwise or) with
  *   NDT_LEARNED_KEEP_ROTATION  the refined guess takes only the model's translation
- *                        and keeps the given guess's rotation. */
+ *                        and keeps the rotation of the guess the model corrects
+ *                        (the given guess, or the registered one when the model
+ *                        has a registration set). */
 enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1, NDT_LEARNED_KEEP_ROTATION = 0x10 };
+
+/* BEV registration, the stage before the model when the model has it set
+ * (ndt_model_set_register): the occupancy of the scan grid (cells with a count)
+ * is correlated with the occupancy of the map around the guess, for `yaws` yaw
+ * angles (yaw_step apart, centred on the guess's yaw) and for every shift of
+ * -radius .. radius cells on x and y.  The best (score = occupied cells in common)
+ * gives the registered guess guess * Rz(yaw) * T(dx, dy, 0), the shifts refined by
+ * a parabola through the neighbouring scores (and the yaw too with refine_yaw);
+ * ties go to the candidate nearest the guess, and a guess with no cell in common
+ * at any candidate stays as it is.  The model then sees the map around the
+ * registered guess and corrects it. */
+typedef struct ndt_register_params {
+    int32_t radius;     /* cells (0.5 m) searched each way on x and y, 0..8; 0 = off */
+    int32_t yaws;       /* yaw angles searched: odd, 1..11 */
+    double yaw_step;    /* radians between them, 0 < yaw_step <= 0.05 */
+    int32_t refine_yaw; /* 1: the yaw refined by a parabola through its neighbours */
+    int32_t pad;
+} ndt_register_params;
+
+enum { NDT_REGISTER_MAX_RADIUS = 8, NDT_REGISTER_MAX_YAWS = 11 };
+
+/* radius 6 (+-3 m), 9 yaws 0.75 degrees apart (+-3 degrees), refine_yaw 1 */
+void ndt_register_params_default(ndt_register_params* rp);
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m
  * tiles for the map grids; origin = world position of the map-local frame. */
@@ -87,6 +112,13 @@ int ndt_bevmap_destroy(ndt_bevmap* bm);
 int ndt_features(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
                  const double* guesses, int32_t B, uint16_t* out);
 
+/* Registered guesses of B (scan, guess) pairs (see ndt_register_params):
+ * out_guesses[b] for scan scans[scan_index[b]] from guesses[b]; out_score
+ * (optional) receives the best score.  radius 0 copies the guesses. */
+int ndt_bev_register(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, const int32_t* scan_index,
+                     const double* guesses, int32_t B, const ndt_register_params* rp, double* out_guesses,
+                     int32_t* out_score);
+
 /* Creates a model with He-normal hidden weights (seeded), small output weights
  * and zero biases.  f0, f1, f2 must be multiples of 128; features need f0 = 65536.
  * A model holds its batch workspace and the captured learned step: it must not be
@@ -94,7 +126,14 @@ int ndt_features(ndt_ctx* ctx, const ndt_bevmap* bm, const ndt_scans* scans, con
 int ndt_model_create(ndt_ctx* ctx, int32_t f0, int32_t f1, int32_t f2, uint64_t seed, ndt_model** out);
 int ndt_model_destroy(ndt_model* model);
 int ndt_model_dims(const ndt_model* model, int32_t dims[3]);
-/* Float32 weights to / from a file (Adam state is not stored). */
+/* The BEV registration the model runs behind (radius 0: none, the default).  A
+ * model trained on crops around registered guesses needs the same setting:
+ * ndt_model_predict and ndt_align_scans_learned register each guess first and
+ * refine the registered guess. */
+int ndt_model_set_register(ndt_model* model, const ndt_register_params* rp);
+int ndt_model_get_register(const ndt_model* model, ndt_register_params* rp);
+/* Float32 weights to / from a file, with the registration setting (Adam state is
+ * not stored). */
 int ndt_model_save(const ndt_model* model, const char* path);
 int ndt_model_load(ndt_ctx* ctx, const char* path, ndt_model** out);
 /* Copies parameter tensor `which` (NDT_W1 ... NDT_B3) to / from host float32. */
@@ -134,7 +173,8 @@ int ndt_dataset_destroy(ndt_dataset* ds);
 int ndt_model_train(ndt_ctx* ctx, ndt_model* model, const ndt_dataset* ds, const ndt_train_params* tp,
                     float* epoch_loss);
 
-/* Refined guesses of S scans: out_guesses[s] = guesses[s] * C(model output). */
+/* Refined guesses of S scans: out_guesses[s] = G * C(model output), G = guesses[s]
+ * or, with a registration set on the model, the registered guesses[s]. */
 int ndt_model_predict(ndt_ctx* ctx, ndt_model* model, const ndt_bevmap* bm, const ndt_scans* scans,
                       const double* guesses, double* out_guesses);
 

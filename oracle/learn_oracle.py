@@ -14,12 +14,18 @@ definitions (include/ndt_learn.h):
                                       arithmetic with the engine's bf16 roundings
                                       (or none, for finite-difference checks)
   adam_step                           Adam (Kingma & Ba 2015)
+  register                            the BEV registration before the model: yaw-
+                                      rotated occupancy of the map, popcount scores
+                                      of every shift, the best candidate and its
+                                      parabolic refinement
 
 Pins (tests/test_learn_oracle.py): bf16 rounding against torch's bfloat16
 conversion; grids against a per-axis linear search on small inputs and against
 the invariance scan grid(G^-1 map) = map grid(G); gradients against central
-finite differences in float64; the correction against its round trip.  No
-function is left unpinned.
+finite differences in float64; the correction against its round trip; the
+registration's scores against a cell-by-cell count, its tie rule on constructed
+score tables, its parabola and small-angle series against closed forms and libm,
+and its result against planted shifts and yaws.  No function is left unpinned.
 """
 from __future__ import annotations
 
@@ -182,3 +188,149 @@ def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
     mh = m / (1 - beta1 ** t)
     vh = v / (1 - beta2 ** t)
     return w - lr * (mh / (np.sqrt(vh) + eps) + wd * w), m, v
+
+
+# ---------------------------------------------------------------- registration
+
+def register_params_default() -> dict:
+    """radius 6 cells (+-3 m), 9 yaws 0.75 degrees apart, yaw refined"""
+    return dict(radius=6, yaws=9, yaw_step=0.75 * 3.14159265358979323846 / 180.0, refine_yaw=1)
+
+
+def yaw_table(rp) -> tuple:
+    """cosine and sine of yaw k = (k - (yaws - 1) / 2) * yaw_step (the C library's)"""
+    import math
+    K = rp["yaws"]
+    th = [float(k - (K - 1) // 2) * rp["yaw_step"] for k in range(K)]
+    return [math.cos(t) for t in th], [math.sin(t) for t in th]
+
+
+def occupancy_rows(grid_bits) -> np.ndarray:
+    """(512,) uint64: row (z, y) of a grid's occupied cells, bit x = cell x"""
+    occ = (np.asarray(grid_bits) != 0).reshape(NZ * NY, NX)
+    w = np.zeros(NZ * NY, np.uint64)
+    for x in range(NX):
+        w |= occ[:, x].astype(np.uint64) << np.uint64(x)
+    return w
+
+
+def map_occupancy_yaws(map_xyzw, guess_local, cs) -> np.ndarray:
+    """(K, 512) uint64: occupancy of the map points around the guess in the frame
+    rotated by yaw k: q = R^T (p - t) as map_grid computes it, then
+    q_k = (c_k qx + s_k qy, c_k qy - s_k qx, qz)."""
+    c_, s_ = cs
+    m = np.asarray(map_xyzw, np.float32)
+    G = np.asarray(guess_local, np.float64).reshape(4, 4)
+    dx = m[:, 0].astype(np.float64) - G[0, 3]
+    dy = m[:, 1].astype(np.float64) - G[1, 3]
+    dz = m[:, 2].astype(np.float64) - G[2, 3]
+    qx = G[0, 0] * dx + G[1, 0] * dy + G[2, 0] * dz
+    qy = G[0, 1] * dx + G[1, 1] * dy + G[2, 1] * dz
+    qz = G[0, 2] * dx + G[1, 2] * dy + G[2, 2] * dz
+    fz = np.floor((qz - Z0) / DZ)
+    out = np.zeros((len(c_), NZ * NY), np.uint64)
+    for k in range(len(c_)):
+        qxk = c_[k] * qx + s_[k] * qy
+        qyk = c_[k] * qy - s_[k] * qx
+        idx = bev_index_xyz_floor(np.floor((qxk - X0) * 2.0), np.floor((qyk - Y0) * 2.0), fz)
+        occ = np.zeros(CELLS, bool)
+        occ[idx[idx >= 0]] = True
+        out[k] = occupancy_rows(occ)
+    return out
+
+
+def bev_index_xyz_floor(fx, fy, fz) -> np.ndarray:
+    with np.errstate(invalid="ignore"):
+        ok = (fx >= 0) & (fx < NX) & (fy >= 0) & (fy < NY) & (fz >= 0) & (fz < NZ)
+    idx = np.full(fx.shape, -1, np.int64)
+    idx[ok] = (fz[ok].astype(np.int64) * NY + fy[ok].astype(np.int64)) * NX + fx[ok].astype(np.int64)
+    return idx
+
+
+def register_scores(srows, mrows, R) -> np.ndarray:
+    """(K, 2R+1, 2R+1) int64: score[k, dy+R, dx+R] = sum over cells c of
+    scan(c) AND map_k(c + (dx, dy)), cells outside the grid empty."""
+    S = np.asarray(srows, np.uint64).reshape(NZ, NY)
+    W = 2 * R + 1
+    out = np.zeros((mrows.shape[0], W, W), np.int64)
+    for k in range(mrows.shape[0]):
+        M = mrows[k].reshape(NZ, NY)
+        for dy in range(-R, R + 1):
+            Mr = np.zeros_like(M)
+            if dy >= 0:
+                Mr[:, :NY - dy] = M[:, dy:]
+            else:
+                Mr[:, -dy:] = M[:, :NY + dy]
+            for dx in range(-R, R + 1):
+                Ms = (Mr >> np.uint64(dx)) if dx >= 0 else (Mr << np.uint64(-dx))
+                out[k, dy + R, dx + R] = int(np.bitwise_count(S & Ms).sum())
+    return out
+
+
+def _parabola(a, b, c) -> float:
+    d = a - 2.0 * b + c
+    return 0.5 * (a - c) / d if d < 0.0 else 0.0
+
+
+def _small_cos_sin(d) -> tuple:
+    d2 = d * d
+    c = 1.0 - d2 / 2.0 + d2 * d2 / 24.0 - d2 * d2 * d2 / 720.0 + d2 * d2 * d2 * d2 / 40320.0
+    s = d - d * d2 / 6.0 + d * d2 * d2 / 120.0 - d * d2 * d2 * d2 / 5040.0
+    return c, s
+
+
+def register_pick(sc, rp) -> tuple:
+    """(score, k, dy, dx) of the best candidate: highest score, then the lowest
+    rank (|k - kc|, |dy|, |dx|, each negative before positive)."""
+    K, R = sc.shape[0], rp["radius"]
+    kc = (K - 1) // 2
+    best = None
+    for k in range(K):
+        for dy in range(-R, R + 1):
+            for dx in range(-R, R + 1):
+                rank = ((((abs(k - kc) * 2 + (k < kc)) * 16 + abs(dy)) * 2 + (dy < 0)) * 16 + abs(dx)) * 2 + (dx < 0)
+                key = (int(sc[k, dy + R, dx + R]), -rank)
+                if best is None or key > best[0]:
+                    best = (key, k, dy, dx)
+    return best[0][0], best[1], best[2], best[3]
+
+
+def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tuple:
+    """Registered (map-local guess, world guess, best score) of one (scan, guess)
+    pair: G' = G Rz(yaw_k) T(d) [Rz(delta)], d the best shift refined by parabolas
+    through its neighbours' scores (half a cell at most), delta likewise between
+    the neighbouring yaws when refine_yaw; unchanged when the best score is 0."""
+    GL = np.asarray(guess_local, np.float64).reshape(4, 4).copy()
+    GW = np.asarray(guess_world, np.float64).reshape(4, 4).copy()
+    if rp["radius"] == 0:
+        return GL, GW, 0
+    cs = yaw_table(rp)
+    srows = occupancy_rows(scan_grid(scan_xyzw, lut))
+    sc = register_scores(srows, map_occupancy_yaws(map_xyzw, GL, cs), rp["radius"])
+    v, k, dy, dx = register_pick(sc, rp)
+    if v == 0:
+        return GL, GW, 0
+    R, K = rp["radius"], len(cs[0])
+    S = sc[k].astype(np.float64)
+    b = S[dy + R, dx + R]
+    ox = _parabola(S[dy + R, dx + R - 1], b, S[dy + R, dx + R + 1]) if abs(dx) < R else 0.0
+    oy = _parabola(S[dy + R - 1, dx + R], b, S[dy + R + 1, dx + R]) if abs(dy) < R else 0.0
+    ck, sk = cs[0][k], cs[1][k]
+    c, s = ck, sk
+    if rp["refine_yaw"] and 0 < k < K - 1:
+        o = _parabola(float(sc[k - 1, dy + R, dx + R]), b, float(sc[k + 1, dy + R, dx + R]))
+        cd, sd = _small_cos_sin(o * rp["yaw_step"])
+        c = ck * cd - sk * sd
+        s = sk * cd + ck * sd
+    d0, d1 = (float(dx) + ox) * 0.5, (float(dy) + oy) * 0.5
+    out = []
+    for G in (GL, GW):
+        g = [float(v_) for v_ in G.reshape(-1)]
+        for r in range(3):
+            a, bb = g[4 * r], g[4 * r + 1]
+            ak, bk = a * ck + bb * sk, bb * ck - a * sk
+            g[4 * r + 3] = g[4 * r + 3] + ak * d0 + bk * d1
+            g[4 * r] = a * c + bb * s
+            g[4 * r + 1] = bb * c - a * s
+        out.append(np.array(g).reshape(4, 4))
+    return out[0], out[1], int(v)

@@ -84,6 +84,50 @@ def test_features_world_origin(A, ctx, small_scene):
         assert np.array_equal(F[b, 32768:], L.map_grid(w.map_xyzw, g[b]))
 
 
+def _rp_dict(rp):
+    return dict(radius=rp.radius, yaws=rp.yaws, yaw_step=rp.yaw_step, refine_yaw=rp.refine_yaw)
+
+
+@pytest.mark.parametrize("kw,origin", [(dict(), (0.0, 0.0, 0.0)), (dict(refine_yaw=0, radius=8, yaws=11),
+                                                                    (4.2e5, 5.6e6, 80.0)),
+                                       (dict(yaws=1, radius=3), (0.0, 0.0, 0.0))])
+def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
+    """Registered guesses and best scores bit for bit equal to the oracle's, for
+    guesses off by up to +-1 m / +-3 degrees, a guess beside the map (no cell in
+    common: the guess comes back unchanged with score 0) and a scan above the grid's
+    height range, with a world origin far from the map-local frame."""
+    w = small_scene
+    high = w.scans[0].copy()
+    high[:, 2] += 40.0
+    scans = list(w.scans) + [high]
+    sc = A.Scans(ctx, scans)
+    o = np.array(origin)
+    bm = A.BevMap(ctx, w.map_xyzw, o)
+    g = A.perturb_poses(w.gt, np.random.default_rng(4), xy=1.0, z=0.3, yaw_deg=3.0)
+    gl = np.concatenate([g, g[:1], w.gt[:1]])
+    gl[len(w.scans), 0, 3] += 500.0
+    idx = np.array(list(range(len(w.scans))) + [0, len(w.scans)], np.int32)
+    gw = gl.copy()
+    gw[:, :3, 3] += o
+    rp = A.register_params(**kw)
+    out, score = bm.register(sc, idx, gw, rp)
+    lut = L.log1p_lut()
+    for b, s in enumerate(idx):
+        gl_b = gw[b].copy()
+        gl_b[:3, 3] -= o
+        el, ew, v = L.register(scans[s], w.map_xyzw, gl_b, gw[b], _rp_dict(rp), lut)
+        assert score[b] == v and np.array_equal(out[b], ew), (b, score[b], v)
+    assert score[len(w.scans)] == 0 and np.array_equal(out[len(w.scans)], gw[len(w.scans)])
+    assert score[-1] == 0 and np.array_equal(out[-1], gw[-1])
+    assert (score[:len(w.scans)] > 100).all()
+    off = bm.register(sc, idx, gw, A.register_params(radius=0))[0]
+    assert np.array_equal(off, gw)
+    with pytest.raises(A.NdtError):
+        bm.register(sc, idx, gw, A.register_params(yaws=4))
+    with pytest.raises(A.NdtError):
+        bm.register(sc, idx, gw, A.register_params(radius=9))
+
+
 def _random_model(A, ctx, f0, f1, f2, seed):
     m = A.Model(ctx, f0, f1, f2, seed=seed)
     r = np.random.default_rng(seed)
@@ -127,7 +171,11 @@ def test_model_save_load(A, ctx, tmp_path):
     p = tmp_path / "m.bin"
     m.save(str(p))
     m2 = A.Model(ctx, path=str(p))
-    assert m2.dims == (256, 128, 128)
+    assert m2.dims == (256, 128, 128) and m2.get_register().radius == 0
+    m.set_register(A.register_params(radius=5, yaws=7))
+    m.save(str(p))
+    r3 = A.Model(ctx, path=str(p)).get_register()
+    assert (r3.radius, r3.yaws, r3.yaw_step, r3.refine_yaw) == (5, 7, A.register_params().yaw_step, 1)
     xb = L.bf16_bits(np.abs(r.normal(size=(128, 256))).astype(np.float32))
     assert np.array_equal(m.forward(xb), m2.forward(xb))
 
@@ -151,7 +199,8 @@ def trained(A, ctx):
     grid = A.Grid(ctx, w.map_xyzw, prm)
     bm = A.BevMap(ctx, w.map_xyzw, w.origin)
     rng = np.random.default_rng(5)
-    ds = None
+    ds = ds_reg = None
+    rp = A.register_params()
     for c0 in range(0, 24000, 8000):
         tr, tr_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 8000, 99 + c0),
                                     100000 + c0 + np.arange(8000))
@@ -163,17 +212,26 @@ def trained(A, ctx):
         wide = rng.uniform(size=len(idx)) < 0.5
         guesses = np.where(wide[:, None, None], A.perturb_poses(tr_gt[idx], rng),
                            A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0))
+        reg, _ = bm.register(trS, idx, guesses, rp)
         if ds is None:
             ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
+            ds_reg = A.Dataset(ctx, bm, trS, idx, reg, labels[idx])
         else:
             ds.add(trS, idx, guesses, labels[idx])
+            ds_reg.add(trS, idx, reg, labels[idx])
         del tr, trS
     model = A.Model(ctx, seed=1)
     losses = model.train(ds, A.train_params(epochs=8, batch=512))
+    model_reg = A.Model(ctx, seed=1)
+    model_reg.set_register(rp)
+    losses_reg = model_reg.train(ds_reg, A.train_params(epochs=8, batch=512))
+    ds.close()
+    ds_reg.close()
     ev, ev_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 256, 77), 900000 + np.arange(256))
     evS = A.Scans(ctx, ev)
     eg = A.perturb_poses(ev_gt, np.random.default_rng(9))
-    return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg, ev=ev)
+    return dict(w=w, prm=prm, grid=grid, bm=bm, model=model, losses=losses, evS=evS, ev_gt=ev_gt, eg=eg, ev=ev,
+                model_reg=model_reg, losses_reg=losses_reg, rp=rp)
 
 
 def test_predictions_depend_only_on_weights_and_inputs(A, ctx, trained):
@@ -238,27 +296,65 @@ def test_learned_warm_start_gains(A, trained):
     assert np.mean([r["converged"] for r in race]) >= np.mean([r["converged"] for r in plain])
 
 
-def test_policies_return_plain_or_learned_runs(A, trained):
+@pytest.mark.parametrize("which", ["model", "model_reg"])
+def test_policies_return_plain_or_learned_runs(A, trained, which):
     """Every result of either policy, with the model's full pose or only its
     translation, is bit for bit the NDT run from the given guess or the one from the
     refined guess, as used_learned says; the race keeps the better final score; an
-    unknown policy is rejected."""
+    unknown policy is rejected.  With a registration the refined guess corrects the
+    registered guess, whose rotation the translation-only variant keeps."""
     t = trained
-    ev, eg = t["evS"], t["eg"]
-    pred = t["model"].predict(t["bm"], ev, eg)
+    ev, eg, m = t["evS"], t["eg"], t[which]
+    pred = m.predict(t["bm"], ev, eg)
+    base = eg if which == "model" else t["bm"].register(ev, np.arange(ev.S, dtype=np.int32), eg, t["rp"])[0]
     plain = ev.align(t["grid"], eg, t["prm"])
     from_pred = ev.align(t["grid"], pred, t["prm"])
     pred_t = pred.copy()
-    pred_t[:, :3, :3] = eg[:, :3, :3]
+    pred_t[:, :3, :3] = base[:, :3, :3]
     from_pred_t = ev.align(t["grid"], pred_t, t["prm"])
     for policy in (A.LEARNED_RACE, A.LEARNED_FALLBACK, A.LEARNED_RACE | A.LEARNED_KEEP_ROTATION,
                    A.LEARNED_FALLBACK | A.LEARNED_KEEP_ROTATION):
         learned = from_pred_t if policy & A.LEARNED_KEEP_ROTATION else from_pred
-        res, used = t["model"].align(t["grid"], t["bm"], ev, eg, t["prm"], policy)
+        res, used = m.align(t["grid"], t["bm"], ev, eg, t["prm"], policy)
         for s, r in enumerate(res):
             ref = learned[s] if used[s] else plain[s]
             assert np.array_equal(r["final_f64"], ref["final_f64"]) and r["iterations"] == ref["iterations"], (policy, s)
             if (policy & 1) == A.LEARNED_RACE and plain[s]["converged"] == learned[s]["converged"]:
                 assert r["score"] == max(plain[s]["score"], learned[s]["score"])
     with pytest.raises(A.NdtError):
-        t["model"].align(t["grid"], t["bm"], ev, eg, t["prm"], 7)
+        m.align(t["grid"], t["bm"], ev, eg, t["prm"], 7)
+
+
+def test_registered_prediction_corrects_the_registered_guess(A, trained):
+    """With a registration set, the model sees the map around the registered guess
+    and its correction applies to that guess: predictions equal registered guess *
+    C(model output on the features at the registered guess), to rounding."""
+    t = trained
+    m, ev, eg = t["model_reg"], t["evS"], t["eg"]
+    idx = np.arange(ev.S, dtype=np.int32)
+    reg, _ = t["bm"].register(ev, idx, eg, t["rp"])
+    y = m.forward(t["bm"].features(ev, idx, reg)).astype(np.float64)
+    pred = m.predict(t["bm"], ev, eg)
+    for s in range(ev.S):
+        ref = reg[s] @ L.pose_matrix(y[s] * L.CORR_SCALE)
+        assert np.abs(pred[s] - ref).max() < 1e-9, s
+
+
+def test_registered_warm_start_saves_passes(A, trained):
+    """Held-out scans with +-1 m / +-3 degree guesses: the registered model's refined
+    guesses lie much closer to the truth, and the fallback policy (translation from
+    the model) spends at least a fifth fewer derivative passes than plain NDT, probe
+    included, converging and reaching the truth at least as often."""
+    t = trained
+    g = A.perturb_poses(t["ev_gt"], np.random.default_rng(12), xy=1.0, z=1.0, yaw_deg=3.0)
+    pred = t["model_reg"].predict(t["bm"], t["evS"], g)
+    raw = np.linalg.norm(g[:, :3, 3] - t["ev_gt"][:, :3, 3], axis=1)
+    ref = np.linalg.norm(pred[:, :3, 3] - t["ev_gt"][:, :3, 3], axis=1)
+    assert ref.mean() < 0.5 * raw.mean(), (raw.mean(), ref.mean())
+    plain = t["evS"].align(t["grid"], g, t["prm"])
+    res, used = t["model_reg"].align(t["grid"], t["bm"], t["evS"], g, t["prm"],
+                                     A.LEARNED_FALLBACK | A.LEARNED_KEEP_ROTATION)
+    pp, pl = np.mean([r["passes"] for r in plain]), np.mean([r["passes"] for r in res])
+    assert pl < 0.8 * pp, (pp, pl)
+    assert np.mean([r["converged"] for r in res]) >= np.mean([r["converged"] for r in plain])
+    assert _success(res, t["ev_gt"]).mean() >= _success(plain, t["ev_gt"]).mean()
