@@ -100,7 +100,7 @@ static int reg_check(const ndt_register_params* rp) {
     return 0;
 }
 
-/* the kernel's view: cosine and sine of each searched yaw (k - (yaws - 1) / 2) * step */
+/* the kernel's view: cosine, sine and tangent of each searched yaw (k - (yaws - 1) / 2) * step */
 static RegisterDev reg_dev(const ndt_register_params& rp) {
     RegisterDev d{};
     d.radius = rp.radius;
@@ -111,6 +111,7 @@ static RegisterDev reg_dev(const ndt_register_params& rp) {
         const double th = (double)(k - (rp.yaws - 1) / 2) * rp.yaw_step;
         d.c[k] = std::cos(th);
         d.s[k] = std::sin(th);
+        d.t[k] = std::tan(th);
     }
     return d;
 }
@@ -426,8 +427,8 @@ extern "C" void ndt_register_params_default(ndt_register_params* rp) {
     if (!rp) return;
     *rp = ndt_register_params{};
     rp->radius = 6;
-    rp->yaws = 9;
-    rp->yaw_step = 0.75 * 3.14159265358979323846 / 180.0;
+    rp->yaws = 7;
+    rp->yaw_step = 3.14159265358979323846 / 180.0;
     rp->refine_yaw = 1;
 }
 

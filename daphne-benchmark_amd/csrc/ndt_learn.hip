@@ -202,6 +202,8 @@ constexpr int WAVE = 64;
 constexpr int REG_WORDS = BEV_CELLS / 32;        /* occupancy bits of one grid as 32-bit words */
 constexpr int REG_ROWS = BEV_NZ * BEV_NY;        /* 64-cell rows (z, y) of a grid */
 constexpr int REG_SIDE = 2 * REG_MAX_RADIUS + 1; /* shifts per axis at the largest radius */
+constexpr int REG_UNROLL = 4;                    /* map points a thread loads at once */
+constexpr int REG_TROWS = 32;                    /* tile rows indexed together */
 
 /* Offset of the vertex of the parabola through (-1, a), (0, b), (1, c) when it
  * opens downwards, else 0; with b the largest of the three it lies in [-0.5, 0.5]. */
@@ -220,37 +222,62 @@ __device__ inline void reg_small_cos_sin(double d, double& c, double& s) {
 
 /* One workgroup per (scan, guess) row.  The map points under the footprints of the
  * yaw-rotated grids are binned once into one occupancy bitmap per yaw (LDS): a
- * point's guess-frame position q = R^T (p - t) is rotated by each yaw,
- * q_k = (c_k qx + s_k qy, c_k qy - s_k qx, qz).  A 64-cell row (z, y) of a bitmap
+ * point's guess-frame position q = R^T (p - t) is turned by each yaw as
+ * q_k = (qx + t_k qy, qy - t_k qx, qz), t_k = tan(yaw_k) (the rotation by -yaw_k
+ * scaled by 1 / cos(yaw_k), at most 1.0014 for 3 degrees: 2 cm at the grid's
+ * edge), so that its cell on x is floor((2 qx + 32) + t_k (2 qy)) and on y
+ * floor((2 qy + 32) - t_k (2 qx)): two products and two sums per yaw.  A 64-cell row (z, y) of a bitmap
  * is one 64-bit word (bit x = cell x), so the score of a shift (dx, dy) is a sum
  * of popcounts of scan row AND shifted map row over the scan's non-empty rows.
- * The best candidate: highest score, then the lowest rank (nearest the guess:
- * |yaw index offset|, |dy|, |dx|, negative before positive), packed with the
- * candidate's index into one 64-bit key whose maximum is taken over the block. */
+ * A thread scores one (yaw, dy) over a quarter of the scan's rows for every dx at
+ * once (the radius is a template parameter, so the shifts are constants), and the
+ * quarters add into the score table.  The best candidate: highest score, then the
+ * lowest rank (nearest the guess: |yaw index offset|, |dy|, |dx|, negative before
+ * positive), packed with the candidate's index into one 64-bit key whose maximum
+ * is taken over the block. */
+template <int R>
 __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, const uint16_t* __restrict__ sbev,
                                                             const int32_t* __restrict__ idx, const double* gl,
                                                             const double* gw, RegisterDev rp, double* gl_out,
                                                             double* gw_out, int32_t* __restrict__ score_out) {
-    __shared__ uint32_t occ[REG_MAX_YAWS * REG_WORDS];
+    constexpr int W = 2 * R + 1;
+    __shared__ __align__(16) uint32_t occ[REG_MAX_YAWS * REG_WORDS];
     __shared__ uint64_t srow[REG_ROWS];
     __shared__ uint16_t rows[REG_ROWS];
     __shared__ uint32_t sc[REG_MAX_YAWS * REG_SIDE * REG_SIDE];
     __shared__ uint64_t wbest[REG_BLOCK / WAVE];
-    __shared__ double s_g[12], s_c[REG_MAX_YAWS], s_s[REG_MAX_YAWS];
+    __shared__ double s_g[12], s_c[REG_MAX_YAWS], s_s[REG_MAX_YAWS], s_t[REG_MAX_YAWS];
     __shared__ int s_rng[4], s_nrows;
+    __shared__ uint32_t s_rbeg[REG_TROWS], s_rcnt[REG_TROWS], s_pre[REG_TROWS + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
-    const int K = rp.yaws, R = rp.radius, W = 2 * R + 1, kc = (K - 1) / 2;
+    const int K = rp.yaws, kc = (K - 1) / 2;
     if (tid < 12) s_g[tid] = gl[16 * (int64_t)b + tid];
     if (tid < K) {
         s_c[tid] = rp.c[tid];
         s_s[tid] = rp.s[tid];
+        s_t[tid] = rp.t[tid];
     }
     for (int i = tid; i < K * REG_WORDS; i += REG_BLOCK) occ[i] = 0u;
-    /* the scan's occupied cells, one ballot per row */
-    const uint16_t* sg = sbev + (int64_t)idx[b] * BEV_CELLS;
-    for (int r = wave; r < REG_ROWS; r += REG_BLOCK / WAVE) {
-        const uint64_t m = __ballot(sg[r * BEV_NX + lane] != 0);
-        if (lane == 0) srow[r] = m;
+    for (int i = tid; i < K * W * W; i += REG_BLOCK) sc[i] = 0u;
+    /* the scan's occupied cells: a thread reads 8 cells (16 bytes) of a row at a
+     * time, all its loads in flight together, and writes their occupancy as byte
+     * x / 8 of the row's word (little-endian: bit x of the word is cell x) */
+    {
+        const uint4* sg = (const uint4*)(sbev + (int64_t)idx[b] * BEV_CELLS);
+        constexpr int PER = REG_ROWS * (BEV_NX / 8) / REG_BLOCK;
+        uint4 v[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) v[u] = sg[tid + u * REG_BLOCK];
+        uint8_t* bytes = (uint8_t*)srow;
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+            uint32_t m8 = 0;
+#pragma unroll
+            for (int h = 0; h < 4; h++)
+                m8 |= (((w[h] & 0xffffu) != 0u) ? 1u : 0u) << (2 * h) | ((w[h] >> 16) != 0u ? 1u : 0u) << (2 * h + 1);
+            bytes[tid + u * REG_BLOCK] = (uint8_t)m8;
+        }
     }
     __syncthreads();
     const double r00 = s_g[0], r01 = s_g[1], r02 = s_g[2], t0 = s_g[3];
@@ -302,44 +329,105 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
     }
     __syncthreads();
     const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
-    for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
-        const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
-        const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
-        for (uint32_t i = pb + tid; i < pe; i += REG_BLOCK) {
-            const float4 p = tg.pts[i];
-            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
-            const double qx = r00 * dx + r10 * dy + r20 * dz;
-            const double qy = r01 * dx + r11 * dy + r21 * dz;
-            const double qz = r02 * dx + r12 * dy + r22 * dz;
-            const double fz = floor((qz - BEV_Z0) / BEV_DZ);
-            if (!(fz >= 0.0 && fz < (double)BEV_NZ)) continue;
-            for (int k = 0; k < K; k++) {
-                const double qxk = s_c[k] * qx + s_s[k] * qy, qyk = s_c[k] * qy - s_s[k] * qx;
-                const double fx = floor((qxk - BEV_X0) * 2.0), fy = floor((qyk - BEV_Y0) * 2.0);
-                if (fx >= 0.0 && fx < (double)BEV_NX && fy >= 0.0 && fy < (double)BEV_NY) {
-                    const int c = ((int)fz * BEV_NY + (int)fy) * BEV_NX + (int)fx;
-                    atomicOr(&occ[k * REG_WORDS + (c >> 5)], 1u << (c & 31));
+    double tk[REG_MAX_YAWS];
+#pragma unroll
+    for (int k = 0; k < REG_MAX_YAWS; k++) tk[k] = k < K ? s_t[k] : 0.0;
+    /* a point with |qx| or |qy| above 16 (1 + max |t_k|) lies outside every yaw's
+     * grid; half a metre more keeps the test clear of rounding */
+    const double reach = -BEV_X0 * (1.0 + fmax(fabs(s_t[0]), fabs(s_t[K - 1]))) + 0.5;
+    /* the points of up to REG_TROWS tile rows at a time as one index range (each
+     * row's run of tiles tx0..tx1 is contiguous), so that every thread has points
+     * and REG_UNROLL loads in flight however the points spread over the rows */
+    for (int rb = ty0; rb <= ty1 && tx0 <= tx1; rb += REG_TROWS) {
+        const int nr = min(REG_TROWS, ty1 - rb + 1);
+        __syncthreads(); /* the previous rows' table is no longer read */
+        if (tid < nr) {
+            const int64_t row = (int64_t)(rb + tid) * tg.ntx;
+            const uint32_t pb = tg.start[row + tx0];
+            s_rbeg[tid] = pb;
+            s_rcnt[tid] = tg.start[row + tx1 + 1] - pb;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            uint32_t acc = 0;
+            for (int r = 0; r < nr; r++) {
+                s_pre[r] = acc;
+                acc += s_rcnt[r];
+            }
+            s_pre[nr] = acc;
+        }
+        __syncthreads();
+        const uint32_t total = s_pre[nr];
+        int r = 0;
+        for (uint32_t i0 = tid; i0 < total; i0 += REG_UNROLL * REG_BLOCK) {
+            /* REG_UNROLL points per thread, their loads issued first; the row of an
+             * index only moves forward (indices past the end repeat the last) */
+            float4 pp[REG_UNROLL];
+#pragma unroll
+            for (int u = 0; u < REG_UNROLL; u++) {
+                const uint32_t gi = min(i0 + u * REG_BLOCK, total - 1);
+                while (s_pre[r + 1] <= gi) r++;
+                pp[u] = tg.pts[s_rbeg[r] + (gi - s_pre[r])];
+            }
+#pragma unroll
+            for (int u = 0; u < REG_UNROLL; u++) {
+                if (i0 + u * REG_BLOCK >= total) continue;
+                const float4 p = pp[u];
+                const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+                const double qx = r00 * dx + r10 * dy + r20 * dz;
+                const double qy = r01 * dx + r11 * dy + r21 * dz;
+                const double qz = r02 * dx + r12 * dy + r22 * dz;
+                if (!(fabs(qx) <= reach && fabs(qy) <= reach)) continue;
+                const double fz = floor((qz - BEV_Z0) / BEV_DZ);
+                if (!(fz >= 0.0 && fz < (double)BEV_NZ)) continue;
+                const int zoff = (int)fz * (BEV_NY * BEV_NX);
+                /* a point that passed the z test is finite (a NaN or infinite
+                 * coordinate makes qz NaN), so the cells convert exactly or
+                 * saturate, and one unsigned compare per axis bounds them */
+                const double ax = 2.0 * qx + 32.0, bx = 2.0 * qy, ay = 2.0 * qy + 32.0, by = 2.0 * qx;
+#pragma unroll
+                for (int k = 0; k < REG_MAX_YAWS; k++) {
+                    if (k < K) {
+                        const int ix = (int)floor(ax + tk[k] * bx);
+                        const int iy = (int)floor(ay - tk[k] * by);
+                        if ((unsigned)ix < (unsigned)BEV_NX && (unsigned)iy < (unsigned)BEV_NY) {
+                            const int c = zoff + iy * BEV_NX + ix;
+                            atomicOr(&occ[k * REG_WORDS + (c >> 5)], 1u << (c & 31));
+                        }
+                    }
                 }
             }
         }
     }
     __syncthreads();
-    /* scores of every (yaw, dy, dx) */
-    const int ncand = K * W * W, nrows = s_nrows;
-    for (int n = tid; n < ncand; n += REG_BLOCK) {
-        const int k = n / (W * W), rem = n % (W * W), dy = rem / W - R, dx = rem % W - R;
-        const uint32_t* ok = occ + k * REG_WORDS;
-        uint32_t acc = 0;
-        for (int j = 0; j < nrows; j++) {
+    /* scores of every (yaw, dy, dx): a thread takes one (yaw, dy) over every
+     * `parts`-th non-empty scan row and counts all dx at once */
+    const int ncand = K * W * W, nrows = s_nrows, nkd = K * W;
+    const int parts = nkd >= REG_BLOCK ? 1 : REG_BLOCK / nkd;
+    for (int it = tid; it < nkd * parts; it += REG_BLOCK) {
+        const int kd = it % nkd, part = it / nkd;
+        const int k = kd / W, dy = kd % W - R;
+        const uint2* ok = (const uint2*)(occ + k * REG_WORDS);
+        uint32_t cnt[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) cnt[q] = 0u;
+        for (int j = part; j < nrows; j += parts) {
             const int r = rows[j];
             const int y = (r & (BEV_NY - 1)) + dy;
             if (y < 0 || y >= BEV_NY) continue;
-            const int mr = (r & ~(BEV_NY - 1)) + y;
-            uint64_t m = ((uint64_t)ok[2 * mr + 1] << 32) | (uint64_t)ok[2 * mr];
-            m = dx >= 0 ? m >> dx : m << (-dx);
-            acc += (uint32_t)__popcll(srow[r] & m);
+            const uint2 mw = ok[(r & ~(BEV_NY - 1)) + y];
+            const uint64_t m = ((uint64_t)mw.y << 32) | (uint64_t)mw.x;
+            const uint64_t sr = srow[r];
+#pragma unroll
+            for (int q = 0; q < W; q++) {
+                const int dx = q - R;
+                cnt[q] += (uint32_t)__popcll(sr & (dx >= 0 ? m >> dx : m << (-dx)));
+            }
         }
-        sc[n] = acc;
+        uint32_t* out = sc + k * W * W + (dy + R) * W;
+#pragma unroll
+        for (int q = 0; q < W; q++)
+            if (cnt[q]) atomicAdd(&out[q], cnt[q]);
     }
     __syncthreads();
     uint64_t best = 0;
@@ -404,9 +492,16 @@ int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const int32_t* idx
                         int32_t B, const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score,
                         hipStream_t stream) {
     if (B <= 0) return 0;
-    if (rp.radius < 0 || rp.radius > REG_MAX_RADIUS || rp.yaws < 1 || rp.yaws > REG_MAX_YAWS) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_bev_register, dim3(B), dim3(REG_BLOCK), 0, stream, tg, sbev, idx, gl, gw, rp, gl_out, gw_out,
-                       score);
+    if (rp.radius < 1 || rp.radius > REG_MAX_RADIUS || rp.yaws < 1 || rp.yaws > REG_MAX_YAWS) return (int)hipErrorInvalidValue;
+#define REG_LAUNCH(r)                                                                                              \
+    case r:                                                                                                        \
+        hipLaunchKernelGGL(k_bev_register<r>, dim3(B), dim3(REG_BLOCK), 0, stream, tg, sbev, idx, gl, gw, rp, gl_out,  \
+                           gw_out, score);                                                                         \
+        break;
+    switch (rp.radius) {
+        REG_LAUNCH(1) REG_LAUNCH(2) REG_LAUNCH(3) REG_LAUNCH(4) REG_LAUNCH(5) REG_LAUNCH(6) REG_LAUNCH(7) REG_LAUNCH(8)
+    }
+#undef REG_LAUNCH
     return (int)hipGetLastError();
 }
 

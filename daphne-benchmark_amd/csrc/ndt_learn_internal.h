@@ -69,13 +69,14 @@ int launch_sum_ordered(const float* x, int n, float* out, hipStream_t stream);
 int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream);
 
 /* BEV registration of a guess (ndt_register_params): yaw k of the search is
- * (k - (yaws - 1) / 2) * yaw_step, its cosine and sine computed on the host */
+ * (k - (yaws - 1) / 2) * yaw_step, its cosine, sine and tangent computed on the
+ * host */
 constexpr int REG_MAX_RADIUS = 8, REG_MAX_YAWS = 11;
-constexpr int REG_BLOCK = 512;
+constexpr int REG_BLOCK = 1024;
 struct RegisterDev {
     int32_t radius, yaws, refine_yaw;
     double yaw_step;
-    double c[REG_MAX_YAWS], s[REG_MAX_YAWS];
+    double c[REG_MAX_YAWS], s[REG_MAX_YAWS], t[REG_MAX_YAWS];
 };
 /* Registers row b (B rows): scan grid sbev[idx[b]] (bf16 bits, 0 = no point)
  * against the map around the map-local guess gl[b]; writes the registered

@@ -99,7 +99,7 @@ typedef struct ndt_register_params {
 
 enum { NDT_REGISTER_MAX_RADIUS = 8, NDT_REGISTER_MAX_YAWS = 11 };
 
-/* radius 6 (+-3 m), 9 yaws 0.75 degrees apart (+-3 degrees), refine_yaw 1 */
+/* radius 6 (+-3 m), 7 yaws 1 degree apart (+-3 degrees), refine_yaw 1 */
 void ndt_register_params_default(ndt_register_params* rp);
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m

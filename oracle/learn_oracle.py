@@ -193,16 +193,17 @@ def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
 # ---------------------------------------------------------------- registration
 
 def register_params_default() -> dict:
-    """radius 6 cells (+-3 m), 9 yaws 0.75 degrees apart, yaw refined"""
-    return dict(radius=6, yaws=9, yaw_step=0.75 * 3.14159265358979323846 / 180.0, refine_yaw=1)
+    """radius 6 cells (+-3 m), 7 yaws 1 degree apart, yaw refined"""
+    return dict(radius=6, yaws=7, yaw_step=3.14159265358979323846 / 180.0, refine_yaw=1)
 
 
 def yaw_table(rp) -> tuple:
-    """cosine and sine of yaw k = (k - (yaws - 1) / 2) * yaw_step (the C library's)"""
+    """cosine, sine and tangent of yaw k = (k - (yaws - 1) / 2) * yaw_step (the C
+    library's)"""
     import math
     K = rp["yaws"]
     th = [float(k - (K - 1) // 2) * rp["yaw_step"] for k in range(K)]
-    return [math.cos(t) for t in th], [math.sin(t) for t in th]
+    return [math.cos(t) for t in th], [math.sin(t) for t in th], [math.tan(t) for t in th]
 
 
 def occupancy_rows(grid_bits) -> np.ndarray:
@@ -216,9 +217,11 @@ def occupancy_rows(grid_bits) -> np.ndarray:
 
 def map_occupancy_yaws(map_xyzw, guess_local, cs) -> np.ndarray:
     """(K, 512) uint64: occupancy of the map points around the guess in the frame
-    rotated by yaw k: q = R^T (p - t) as map_grid computes it, then
-    q_k = (c_k qx + s_k qy, c_k qy - s_k qx, qz)."""
-    c_, s_ = cs
+    turned by yaw k: q = R^T (p - t) as map_grid computes it, then
+    q_k = (qx + t_k qy, qy - t_k qx, qz) with t_k = tan(yaw_k) (the rotation by
+    -yaw_k scaled by 1 / cos(yaw_k)); cells floor((2 qx + 32) + t_k (2 qy)) and
+    floor((2 qy + 32) - t_k (2 qx)), z as bev_index."""
+    t_ = cs[2]
     m = np.asarray(map_xyzw, np.float32)
     G = np.asarray(guess_local, np.float64).reshape(4, 4)
     dx = m[:, 0].astype(np.float64) - G[0, 3]
@@ -228,11 +231,10 @@ def map_occupancy_yaws(map_xyzw, guess_local, cs) -> np.ndarray:
     qy = G[0, 1] * dx + G[1, 1] * dy + G[2, 1] * dz
     qz = G[0, 2] * dx + G[1, 2] * dy + G[2, 2] * dz
     fz = np.floor((qz - Z0) / DZ)
-    out = np.zeros((len(c_), NZ * NY), np.uint64)
-    for k in range(len(c_)):
-        qxk = c_[k] * qx + s_[k] * qy
-        qyk = c_[k] * qy - s_[k] * qx
-        idx = bev_index_xyz_floor(np.floor((qxk - X0) * 2.0), np.floor((qyk - Y0) * 2.0), fz)
+    ax, bx, ay, by = 2.0 * qx + 32.0, 2.0 * qy, 2.0 * qy + 32.0, 2.0 * qx
+    out = np.zeros((len(t_), NZ * NY), np.uint64)
+    for k in range(len(t_)):
+        idx = bev_index_xyz_floor(np.floor(ax + t_[k] * bx), np.floor(ay - t_[k] * by), fz)
         occ = np.zeros(CELLS, bool)
         occ[idx[idx >= 0]] = True
         out[k] = occupancy_rows(occ)

@@ -195,11 +195,12 @@ def _scene(A):
     return mp, G, scan
 
 
-@pytest.mark.parametrize("shift,steps", [((1.0, -1.5), 0), ((0.0, 0.0), 2), ((-2.0, 0.5), -3), ((2.5, 2.0), 4),
-                                         ((-2.5, -3.0), -4)])
+@pytest.mark.parametrize("shift,steps", [((1.0, -1.5), 0), ((0.0, 0.0), 2), ((-2.0, 0.5), -3), ((2.5, 2.0), 3),
+                                         ((-2.5, -3.0), 1)])
 def test_register_recovers_planted_offsets(A, shift, steps):
     """A scan made of the map's own points seen from the true pose T, and the guess
-    G = T T(-d) Rz(-steps yaw_step), d a whole number of cells: the registration
+    G = T T(-d) Rz(-steps yaw_step), d a whole number of cells and the yaw within
+    the searched ones (the outermost without a neighbour to refine with): the registration
     must return T within 3 cm and 0.05 degrees (observed: 1 cm, 0.008 degrees; the
     shift applied along the guess's axes instead of the searched yaw's would miss
     by 8 cm at 3 steps)."""
