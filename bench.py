@@ -91,13 +91,25 @@ def algorithmic_work(results, counts, pairs):
     return 16.0 * pts + 36.0 * pairs, 133.0 * pts + 340.0 * pairs
 
 
+def coarse_register_params():
+    """the first registration of training guesses: +-4 m, +-10 deg in 2 deg steps"""
+    return A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)))
+
+
+REGISTERED_LOSS_WEIGHTS = [0.1, 0.1, 1.0, 1.0, 1.0, 1.0]
+
+
 def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_decay, rp=None, seed=5):
     """Training triples from scans of the workload's map at other sensor poses
     (generator seeds disjoint from the batch's), labelled with NDT run from the
     true pose; half the guesses within +-3 m / +-10 deg yaw, half within
     +-1 m / +-3 deg.  With a registration (rp) the guesses are registered first, so
     that the model learns the correction of the registered guess, and the model
-    gets the same setting."""
+    gets the same setting: a coarse search (+-4 m, +-10 deg in 2 deg steps) and
+    then rp, so that the wide guesses' residuals are as small as the narrow ones'
+    (the model otherwise learns corrections from the wide guesses' failed
+    registrations and moves good registered guesses); the x and y terms of the loss
+    weigh 0.1, the registration having set x and y."""
     rng = np.random.default_rng(seed)
     ds, t_gen, t_lab = None, 0.0, 0.0
     chunk = 10000
@@ -120,7 +132,7 @@ def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_deca
         g2 = A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0)
         guesses = np.where(wide[:, None, None], g1, g2)
         if rp is not None:
-            guesses = bm.register(trS, idx, guesses, rp)[0]
+            guesses = bm.register(trS, idx, bm.register(trS, idx, guesses, coarse_register_params())[0], rp)[0]
         if ds is None:
             ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
         else:
@@ -129,7 +141,10 @@ def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_deca
     model = A.Model(ctx, seed=1)
     model.set_register(rp)
     t = time.time()
-    losses = model.train(ds, A.train_params(epochs=epochs, batch=512, weight_decay=weight_decay))
+    tp = dict(epochs=epochs, batch=512, weight_decay=weight_decay)
+    if rp is not None:
+        tp["weights"] = REGISTERED_LOSS_WEIGHTS
+    losses = model.train(ds, A.train_params(**tp))
     t_train = time.time() - t
     samples = int(ds.n)
     ds.close()
@@ -470,7 +485,9 @@ def main():
                                       args.weight_decay, rp)
         if rp is not None:
             tinfo["registration"] = dict(radius_cells=rp.radius, yaws=rp.yaws, yaw_step_deg=float(np.degrees(rp.yaw_step)),
-                                         refine_yaw=bool(rp.refine_yaw))
+                                         refine_yaw=bool(rp.refine_yaw), z_search=bool(rp.z_search),
+                                         training_guesses="registered +-4 m / +-10 deg in 2 deg steps, then as above",
+                                         loss_weights=REGISTERED_LOSS_WEIGHTS)
         log(f"proposer trained: {tinfo}")
         policy = A.LEARNED_RACE if args.policy == "race" else A.LEARNED_FALLBACK
         if not args.full_pose:

@@ -97,6 +97,7 @@ static int reg_check(const ndt_register_params* rp) {
     if (rp->yaws > 1 && !(rp->yaw_step > 0.0 && rp->yaw_step <= 0.05))
         return fail(NDT_ERR_INVALID, "registration yaw_step must be in (0, 0.05] rad");
     if (rp->refine_yaw != 0 && rp->refine_yaw != 1) return fail(NDT_ERR_INVALID, "refine_yaw must be 0 or 1");
+    if (rp->z_search != 0 && rp->z_search != 1) return fail(NDT_ERR_INVALID, "z_search must be 0 or 1");
     return 0;
 }
 
@@ -106,6 +107,7 @@ static RegisterDev reg_dev(const ndt_register_params& rp) {
     d.radius = rp.radius;
     d.yaws = rp.yaws;
     d.refine_yaw = rp.refine_yaw;
+    d.z_search = rp.z_search;
     d.yaw_step = rp.yaw_step;
     for (int k = 0; k < rp.yaws; k++) {
         const double th = (double)(k - (rp.yaws - 1) / 2) * rp.yaw_step;
@@ -117,7 +119,8 @@ static RegisterDev reg_dev(const ndt_register_params& rp) {
 }
 
 static bool same_reg(const ndt_register_params& a, const ndt_register_params& b) {
-    return a.radius == b.radius && a.yaws == b.yaws && a.yaw_step == b.yaw_step && a.refine_yaw == b.refine_yaw;
+    return a.radius == b.radius && a.yaws == b.yaws && a.yaw_step == b.yaw_step && a.refine_yaw == b.refine_yaw &&
+           a.z_search == b.z_search;
 }
 
 /* ------------------------------------------------------------------ handles */
@@ -430,13 +433,13 @@ extern "C" void ndt_register_params_default(ndt_register_params* rp) {
     rp->yaws = 7;
     rp->yaw_step = 3.14159265358979323846 / 180.0;
     rp->refine_yaw = 1;
+    rp->z_search = 1;
 }
 
 extern "C" int ndt_model_set_register(ndt_model* md, const ndt_register_params* rp) {
     if (!md) return fail(NDT_ERR_INVALID, "null model");
     if (int e = reg_check(rp)) return e;
     md->reg = *rp;
-    md->reg.pad = 0;
     return 0;
 }
 
@@ -478,7 +481,7 @@ extern "C" int ndt_model_set(ndt_model* md, int32_t which, const float* in, int6
 
 static const char MODEL_MAGIC[8] = {'N', 'D', 'T', 'M', 'L', 'P', '0', '1'};
 /* optional trailer after the tensors: the registration setting */
-static const char REG_MAGIC[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '1'};
+static const char REG_MAGIC[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '2'};
 
 extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
     if (!md || !path) return fail(NDT_ERR_INVALID, "null argument");
@@ -495,8 +498,8 @@ extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
         ok = std::fwrite(h.data(), 4, h.size(), f) == h.size();
     }
     if (ok && reg_on(md->reg)) {
-        const int32_t ri[3] = {md->reg.radius, md->reg.yaws, md->reg.refine_yaw};
-        ok = std::fwrite(REG_MAGIC, 1, 8, f) == 8 && std::fwrite(ri, 4, 3, f) == 3 &&
+        const int32_t ri[4] = {md->reg.radius, md->reg.yaws, md->reg.refine_yaw, md->reg.z_search};
+        ok = std::fwrite(REG_MAGIC, 1, 8, f) == 8 && std::fwrite(ri, 4, 4, f) == 4 &&
              std::fwrite(&md->reg.yaw_step, 8, 1, f) == 1;
     }
     ok = (std::fclose(f) == 0) && ok;
@@ -528,14 +531,15 @@ extern "C" int ndt_model_load(ndt_ctx* c, const char* path, ndt_model** out) {
     }
     char tag[8];
     if (!e && std::fread(tag, 1, 8, f) == 8) {
-        int32_t ri[3];
+        int32_t ri[4];
         ndt_register_params rp{};
-        if (std::memcmp(tag, REG_MAGIC, 8) || std::fread(ri, 4, 3, f) != 3 || std::fread(&rp.yaw_step, 8, 1, f) != 1)
+        if (std::memcmp(tag, REG_MAGIC, 8) || std::fread(ri, 4, 4, f) != 4 || std::fread(&rp.yaw_step, 8, 1, f) != 1)
             e = fail(NDT_ERR_INVALID, "bad registration trailer in model file");
         if (!e) {
             rp.radius = ri[0];
             rp.yaws = ri[1];
             rp.refine_yaw = ri[2];
+            rp.z_search = ri[3];
             if (!(e = reg_check(&rp))) md->reg = rp;
         }
     }
@@ -774,7 +778,8 @@ extern "C" int ndt_bev_register(ndt_ctx* c, const ndt_bevmap* bm, const ndt_scan
     HIP_TRY(hipMemcpyAsync(idx.p, scan_index, 4 * (size_t)B, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(gl.p, gl_h.data(), 8 * gl_h.size(), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(gw.p, guesses, 8 * gl_h.size(), hipMemcpyHostToDevice, st));
-    LAUNCH_TRY(launch_bev_register(bm->dev, sbev.as<uint16_t>(), idx.as<int32_t>(), gl.as<double>(), gw.as<double>(), B,
+    LAUNCH_TRY(launch_bev_register(bm->dev, sbev.as<uint16_t>(), sc->pts.as<float4>(), off.as<int64_t>(),
+                                   cnt.as<int32_t>(), idx.as<int32_t>(), gl.as<double>(), gw.as<double>(), B,
                                    reg_dev(*rp), gl.as<double>(), gw.as<double>(), score.as<int32_t>(), st));
     HIP_TRY(hipMemcpyAsync(out_guesses, gw.p, 8 * gl_h.size(), hipMemcpyDeviceToHost, st));
     if (out_score) HIP_TRY(hipMemcpyAsync(out_score, score.p, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
@@ -993,7 +998,8 @@ static int predict_enqueue(ndt_model* md, const ndt_bevmap* bm, const ndt_scans*
      * guesses into greg (gw keeps the given guesses, plain NDT's starts) */
     const bool reg = reg_on(md->reg);
     if (reg)
-        LAUNCH_TRY(launch_bev_register(bm->dev, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(),
+        LAUNCH_TRY(launch_bev_register(bm->dev, md->sbev.as<uint16_t>(), sc->pts.as<float4>(), md->off.as<int64_t>(),
+                                       md->cnt.as<int32_t>(), md->idx.as<int32_t>(), md->gl.as<double>(),
                                        md->gw.as<double>(), S, reg_dev(md->reg), md->gl.as<double>(),
                                        md->greg.as<double>(), nullptr, st));
     int e = batch_features(md, bm, md->sbev.as<uint16_t>(), md->idx.as<int32_t>(), md->gl.as<double>(), Bp);

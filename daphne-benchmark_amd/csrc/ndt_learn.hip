@@ -204,6 +204,17 @@ constexpr int REG_ROWS = BEV_NZ * BEV_NY;        /* 64-cell rows (z, y) of a gri
 constexpr int REG_SIDE = 2 * REG_MAX_RADIUS + 1; /* shifts per axis at the largest radius */
 constexpr int REG_UNROLL = 4;                    /* map points a thread loads at once */
 constexpr int REG_TROWS = 32;                    /* tile rows indexed together */
+/* height search: 5 cm bins over [-4, 12) m (bin floor((z + 4) * 20)), shifts of
+ * -1.5 .. 1.5 m; each wave counts into its own copy of a histogram (the ground's
+ * few bins take most points), the copies summed before the scores */
+constexpr int ZH_BINS = 320, ZH_RANGE = 30, ZH_COPIES = REG_BLOCK / WAVE;
+constexpr double ZH_Z0 = -4.0, ZH_BIN = 0.05, ZH_PER_M = 20.0;
+
+/* height bin of z, or -1 outside (NaN included) */
+__device__ inline int zh_bin(double z) {
+    const double f = floor((z - ZH_Z0) * ZH_PER_M);
+    return (f >= 0.0 && f < (double)ZH_BINS) ? (int)f : -1;
+}
 
 /* Offset of the vertex of the parabola through (-1, a), (0, b), (1, c) when it
  * opens downwards, else 0; with b the largest of the three it lies in [-0.5, 0.5]. */
@@ -237,6 +248,9 @@ __device__ inline void reg_small_cos_sin(double d, double& c, double& s) {
  * is taken over the block. */
 template <int R>
 __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, const uint16_t* __restrict__ sbev,
+                                                            const float4* __restrict__ spts,
+                                                            const int64_t* __restrict__ soff,
+                                                            const int32_t* __restrict__ scnt,
                                                             const int32_t* __restrict__ idx, const double* gl,
                                                             const double* gw, RegisterDev rp, double* gl_out,
                                                             double* gw_out, int32_t* __restrict__ score_out) {
@@ -249,6 +263,8 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
     __shared__ double s_g[12], s_c[REG_MAX_YAWS], s_s[REG_MAX_YAWS], s_t[REG_MAX_YAWS];
     __shared__ int s_rng[4], s_nrows;
     __shared__ uint32_t s_rbeg[REG_TROWS], s_rcnt[REG_TROWS], s_pre[REG_TROWS + 1];
+    __shared__ uint32_t zh[2][ZH_COPIES][ZH_BINS]; /* heights: 0 scan, 1 map */
+    __shared__ uint64_t zsc[2 * ZH_RANGE + 1];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const int K = rp.yaws, kc = (K - 1) / 2;
     if (tid < 12) s_g[tid] = gl[16 * (int64_t)b + tid];
@@ -259,6 +275,8 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
     }
     for (int i = tid; i < K * REG_WORDS; i += REG_BLOCK) occ[i] = 0u;
     for (int i = tid; i < K * W * W; i += REG_BLOCK) sc[i] = 0u;
+    const bool zs = rp.z_search != 0;
+    for (int i = tid; i < 2 * ZH_COPIES * ZH_BINS; i += REG_BLOCK) (&zh[0][0][0])[i] = 0u;
     /* the scan's occupied cells: a thread reads 8 cells (16 bytes) of a row at a
      * time, all its loads in flight together, and writes their occupancy as byte
      * x / 8 of the row's word (little-endian: bit x of the word is cell x) */
@@ -280,6 +298,23 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
         }
     }
     __syncthreads();
+    if (zs) {
+        /* the scan's point heights, eight loads in flight per thread */
+        const int s = idx[b];
+        const float* sz = (const float*)(spts + soff[s]) + 2;
+        const int n = scnt[s];
+        uint32_t* h = zh[0][wave];
+        for (int i0 = tid; i0 < n; i0 += 8 * REG_BLOCK) {
+            float z[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) z[u] = sz[4 * (int64_t)min(i0 + u * REG_BLOCK, n - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int bz = zh_bin((double)z[u]);
+                if (i0 + u * REG_BLOCK < n && bz >= 0) atomicAdd(&h[bz], 1u);
+            }
+        }
+    }
     const double r00 = s_g[0], r01 = s_g[1], r02 = s_g[2], t0 = s_g[3];
     const double r10 = s_g[4], r11 = s_g[5], r12 = s_g[6], t1 = s_g[7];
     const double r20 = s_g[8], r21 = s_g[9], r22 = s_g[10], t2 = s_g[11];
@@ -378,6 +413,12 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
                 const double qy = r01 * dx + r11 * dy + r21 * dz;
                 const double qz = r02 * dx + r12 * dy + r22 * dz;
                 if (!(fabs(qx) <= reach && fabs(qy) <= reach)) continue;
+                if (zs && fabs(qx) < -BEV_X0 && fabs(qy) < -BEV_Y0) {
+                    /* the map's heights under the grid's footprint (a yaw turn and
+                     * a shift in the plane leave them unchanged) */
+                    const int bz = zh_bin(qz);
+                    if (bz >= 0) atomicAdd(&zh[1][wave][bz], 1u);
+                }
                 const double fz = floor((qz - BEV_Z0) / BEV_DZ);
                 if (!(fz >= 0.0 && fz < (double)BEV_NZ)) continue;
                 const int zoff = (int)fz * (BEV_NY * BEV_NX);
@@ -400,6 +441,16 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
         }
     }
     __syncthreads();
+    if (zs) {
+        /* the histograms' copies into copy 0 */
+        for (int i = tid; i < 2 * ZH_BINS; i += REG_BLOCK) {
+            uint32_t* h = &zh[i / ZH_BINS][0][i % ZH_BINS];
+            uint32_t a = 0;
+            for (int c = 0; c < ZH_COPIES; c++) a += h[c * ZH_BINS];
+            h[0] = a;
+        }
+        __syncthreads();
+    }
     /* scores of every (yaw, dy, dx): a thread takes one (yaw, dy) over every
      * `parts`-th non-empty scan row and counts all dx at once */
     const int ncand = K * W * W, nrows = s_nrows, nkd = K * W;
@@ -428,6 +479,13 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
 #pragma unroll
         for (int q = 0; q < W; q++)
             if (cnt[q]) atomicAdd(&out[q], cnt[q]);
+    }
+    if (zs && tid >= REG_BLOCK - (2 * ZH_RANGE + 1)) {
+        /* height scores sum_b scan(b) map(b + k), by the last threads */
+        const int j = tid - (REG_BLOCK - (2 * ZH_RANGE + 1)), k = j - ZH_RANGE;
+        uint64_t acc = 0;
+        for (int q = max(0, -k); q < ZH_BINS - max(0, k); q++) acc += (uint64_t)zh[0][0][q] * zh[1][0][q + k];
+        zsc[j] = acc;
     }
     __syncthreads();
     uint64_t best = 0;
@@ -483,20 +541,39 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
             }
         }
     }
+    if (zs) {
+        /* the best height shift: highest score, then the smallest |k|, negative
+         * first; refined by a parabola; applied along the guess's z axis (the yaw
+         * turn leaves that column as it is) */
+        int jb = ZH_RANGE;
+        for (int j = 0; j < 2 * ZH_RANGE + 1; j++) {
+            const int k = j - ZH_RANGE, kb = jb - ZH_RANGE;
+            const int rk = 2 * abs(k) + (k < 0 ? 1 : 0), rb = 2 * abs(kb) + (kb < 0 ? 1 : 0);
+            if (zsc[j] > zsc[jb] || (zsc[j] == zsc[jb] && rk < rb)) jb = j;
+        }
+        if (zsc[jb] != 0) {
+            const double o = (jb > 0 && jb < 2 * ZH_RANGE)
+                                 ? reg_parabola((double)zsc[jb - 1], (double)zsc[jb], (double)zsc[jb + 1])
+                                 : 0.0;
+            const double dz = ((double)(jb - ZH_RANGE) + o) * ZH_BIN;
+            for (int q = 0; q < 2; q++)
+                for (int r = 0; r < 3; r++) G[q][4 * r + 3] = G[q][4 * r + 3] + G[q][4 * r + 2] * dz;
+        }
+    }
     for (int q = 0; q < 2; q++)
         for (int e = 0; e < 16; e++) outs[q][e] = G[q][e];
     if (score_out) score_out[b] = (int32_t)v;
 }
 
-int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const int32_t* idx, const double* gl, const double* gw,
-                        int32_t B, const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score,
-                        hipStream_t stream) {
+int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const float4* pts, const int64_t* offsets,
+                        const int32_t* counts, const int32_t* idx, const double* gl, const double* gw, int32_t B,
+                        const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score, hipStream_t stream) {
     if (B <= 0) return 0;
     if (rp.radius < 1 || rp.radius > REG_MAX_RADIUS || rp.yaws < 1 || rp.yaws > REG_MAX_YAWS) return (int)hipErrorInvalidValue;
 #define REG_LAUNCH(r)                                                                                              \
     case r:                                                                                                        \
-        hipLaunchKernelGGL(k_bev_register<r>, dim3(B), dim3(REG_BLOCK), 0, stream, tg, sbev, idx, gl, gw, rp, gl_out,  \
-                           gw_out, score);                                                                         \
+        hipLaunchKernelGGL(k_bev_register<r>, dim3(B), dim3(REG_BLOCK), 0, stream, tg, sbev, pts, offsets, counts,    \
+                           idx, gl, gw, rp, gl_out, gw_out, score);                                                \
         break;
     switch (rp.radius) {
         REG_LAUNCH(1) REG_LAUNCH(2) REG_LAUNCH(3) REG_LAUNCH(4) REG_LAUNCH(5) REG_LAUNCH(6) REG_LAUNCH(7) REG_LAUNCH(8)

@@ -87,19 +87,24 @@ enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1, NDT_LEARNED_KEEP_ROTATION
  * gives the registered guess guess * Rz(yaw) * T(dx, dy, 0), the shifts refined by
  * a parabola through the neighbouring scores (and the yaw too with refine_yaw);
  * ties go to the candidate nearest the guess, and a guess with no cell in common
- * at any candidate stays as it is.  The model then sees the map around the
+ * at any candidate stays as it is.  With z_search the height follows: histograms
+ * of point heights in 5 cm bins over [-4, 12) m (bin floor((z + 4) * 20)), the
+ * scan's in its sensor frame
+ * and the map's under the grid's footprint in the guess frame, correlated over
+ * shifts of -1.5 .. 1.5 m; the best shift, refined by a parabola, moves the
+ * registered guess along its z axis.  The model then sees the map around the
  * registered guess and corrects it. */
 typedef struct ndt_register_params {
     int32_t radius;     /* cells (0.5 m) searched each way on x and y, 0..8; 0 = off */
     int32_t yaws;       /* yaw angles searched: odd, 1..11 */
     double yaw_step;    /* radians between them, 0 < yaw_step <= 0.05 */
     int32_t refine_yaw; /* 1: the yaw refined by a parabola through its neighbours */
-    int32_t pad;
+    int32_t z_search;   /* 1: the height registered too */
 } ndt_register_params;
 
 enum { NDT_REGISTER_MAX_RADIUS = 8, NDT_REGISTER_MAX_YAWS = 11 };
 
-/* radius 6 (+-3 m), 7 yaws 1 degree apart (+-3 degrees), refine_yaw 1 */
+/* radius 6 (+-3 m), 7 yaws 1 degree apart (+-3 degrees), refine_yaw 1, z_search 1 */
 void ndt_register_params_default(ndt_register_params* rp);
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m

@@ -193,8 +193,43 @@ def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
 # ---------------------------------------------------------------- registration
 
 def register_params_default() -> dict:
-    """radius 6 cells (+-3 m), 7 yaws 1 degree apart, yaw refined"""
-    return dict(radius=6, yaws=7, yaw_step=3.14159265358979323846 / 180.0, refine_yaw=1)
+    """radius 6 cells (+-3 m), 7 yaws 1 degree apart, yaw refined, height searched"""
+    return dict(radius=6, yaws=7, yaw_step=3.14159265358979323846 / 180.0, refine_yaw=1, z_search=1)
+
+
+ZH_BINS, ZH_RANGE, ZH_Z0, ZH_BIN = 320, 30, -4.0, 0.05
+
+
+def z_histogram(z) -> np.ndarray:
+    """(320,) int64 counts of heights in 5 cm bins over [-4, 12) m, bin
+    floor((z + 4) * 20) (NaN outside)"""
+    with np.errstate(invalid="ignore"):
+        f = np.floor((np.asarray(z, np.float64) - ZH_Z0) * 20.0)
+        ok = (f >= 0) & (f < ZH_BINS)
+    return np.bincount(f[ok].astype(np.int64), minlength=ZH_BINS).astype(np.int64)
+
+
+def z_scores(hs, hm) -> np.ndarray:
+    """(61,) score[k + 30] = sum_b scan(b) map(b + k), k = -30 .. 30"""
+    out = np.zeros(2 * ZH_RANGE + 1, np.int64)
+    for k in range(-ZH_RANGE, ZH_RANGE + 1):
+        lo, hi = max(0, -k), ZH_BINS - max(0, k)
+        out[k + ZH_RANGE] = int(np.sum(hs[lo:hi] * hm[lo + k:hi + k]))
+    return out
+
+
+def z_shift(sc) -> float:
+    """height shift (m) of the best score: highest, then smallest |k|, negative
+    first; refined by a parabola through its neighbours; 0 when every score is 0"""
+    jb = ZH_RANGE
+    for j in range(2 * ZH_RANGE + 1):
+        k, kb = j - ZH_RANGE, jb - ZH_RANGE
+        if sc[j] > sc[jb] or (sc[j] == sc[jb] and 2 * abs(k) + (k < 0) < 2 * abs(kb) + (kb < 0)):
+            jb = j
+    if sc[jb] == 0:
+        return 0.0
+    o = _parabola(float(sc[jb - 1]), float(sc[jb]), float(sc[jb + 1])) if 0 < jb < 2 * ZH_RANGE else 0.0
+    return (float(jb - ZH_RANGE) + o) * ZH_BIN
 
 
 def yaw_table(rp) -> tuple:
@@ -297,21 +332,54 @@ def register_pick(sc, rp) -> tuple:
     return best[0][0], best[1], best[2], best[3]
 
 
+def map_heights(map_xyzw, guess_local) -> np.ndarray:
+    """height histogram of the map points under the grid's footprint
+    (|qx| < 16, |qy| < 16 in the guess frame, q as map_grid computes it)"""
+    m = np.asarray(map_xyzw, np.float32)
+    G = np.asarray(guess_local, np.float64).reshape(4, 4)
+    dx = m[:, 0].astype(np.float64) - G[0, 3]
+    dy = m[:, 1].astype(np.float64) - G[1, 3]
+    dz = m[:, 2].astype(np.float64) - G[2, 3]
+    qx = G[0, 0] * dx + G[1, 0] * dy + G[2, 0] * dz
+    qy = G[0, 1] * dx + G[1, 1] * dy + G[2, 1] * dz
+    qz = G[0, 2] * dx + G[1, 2] * dy + G[2, 2] * dz
+    with np.errstate(invalid="ignore"):
+        foot = (np.abs(qx) < -X0) & (np.abs(qy) < -Y0)
+    return z_histogram(qz[foot])
+
+
 def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tuple:
     """Registered (map-local guess, world guess, best score) of one (scan, guess)
     pair: G' = G Rz(yaw_k) T(d) [Rz(delta)], d the best shift refined by parabolas
     through its neighbours' scores (half a cell at most), delta likewise between
-    the neighbouring yaws when refine_yaw; unchanged when the best score is 0."""
+    the neighbouring yaws when refine_yaw; unchanged when the best score is 0.
+    With z_search the result then moves along its z axis by the best height shift
+    (z_scores of the scan's heights and the map's under the footprint of G)."""
     GL = np.asarray(guess_local, np.float64).reshape(4, 4).copy()
     GW = np.asarray(guess_world, np.float64).reshape(4, 4).copy()
     if rp["radius"] == 0:
         return GL, GW, 0
+    dz = 0.0
+    if rp.get("z_search", 0):
+        dz = z_shift(z_scores(z_histogram(np.asarray(scan_xyzw, np.float32)[:, 2].astype(np.float64)),
+                              map_heights(map_xyzw, GL)))
+
+    def lift(Gs):
+        out = []
+        for G in Gs:
+            g = G.reshape(-1).copy()
+            if dz != 0.0:
+                for r in range(3):
+                    g[4 * r + 3] = g[4 * r + 3] + g[4 * r + 2] * dz
+            out.append(g.reshape(4, 4))
+        return out
     cs = yaw_table(rp)
     srows = occupancy_rows(scan_grid(scan_xyzw, lut))
     sc = register_scores(srows, map_occupancy_yaws(map_xyzw, GL, cs), rp["radius"])
     v, k, dy, dx = register_pick(sc, rp)
     if v == 0:
-        return GL, GW, 0
+        gl, gw = lift([GL, GW])
+        return gl, gw, 0
     R, K = rp["radius"], len(cs[0])
     S = sc[k].astype(np.float64)
     b = S[dy + R, dx + R]
@@ -335,4 +403,5 @@ def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tup
             g[4 * r] = a * c + bb * s
             g[4 * r + 1] = bb * c - a * s
         out.append(np.array(g).reshape(4, 4))
+    out = lift(out)
     return out[0], out[1], int(v)

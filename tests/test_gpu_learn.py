@@ -85,12 +85,12 @@ def test_features_world_origin(A, ctx, small_scene):
 
 
 def _rp_dict(rp):
-    return dict(radius=rp.radius, yaws=rp.yaws, yaw_step=rp.yaw_step, refine_yaw=rp.refine_yaw)
+    return dict(radius=rp.radius, yaws=rp.yaws, yaw_step=rp.yaw_step, refine_yaw=rp.refine_yaw, z_search=rp.z_search)
 
 
 @pytest.mark.parametrize("kw,origin", [(dict(), (0.0, 0.0, 0.0)), (dict(refine_yaw=0, radius=8, yaws=11),
                                                                     (4.2e5, 5.6e6, 80.0)),
-                                       (dict(yaws=1, radius=3), (0.0, 0.0, 0.0))])
+                                       (dict(yaws=1, radius=3, z_search=0), (0.0, 0.0, 0.0))])
 def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
     """Registered guesses and best scores bit for bit equal to the oracle's, for
     guesses off by up to +-1 m / +-3 degrees, a guess beside the map (no cell in
@@ -117,8 +117,10 @@ def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
         gl_b[:3, 3] -= o
         el, ew, v = L.register(scans[s], w.map_xyzw, gl_b, gw[b], _rp_dict(rp), lut)
         assert score[b] == v and np.array_equal(out[b], ew), (b, score[b], v)
+    # no cell in common: the guess stays, apart from the height search (the map
+    # beside it is empty, the scan above the grid still has heights)
     assert score[len(w.scans)] == 0 and np.array_equal(out[len(w.scans)], gw[len(w.scans)])
-    assert score[-1] == 0 and np.array_equal(out[-1], gw[-1])
+    assert score[-1] == 0 and np.array_equal(out[-1][:, :3], gw[-1][:, :3])
     assert (score[:len(w.scans)] > 100).all()
     off = bm.register(sc, idx, gw, A.register_params(radius=0))[0]
     assert np.array_equal(off, gw)
@@ -175,7 +177,10 @@ def test_model_save_load(A, ctx, tmp_path):
     m.set_register(A.register_params(radius=5, yaws=7))
     m.save(str(p))
     r3 = A.Model(ctx, path=str(p)).get_register()
-    assert (r3.radius, r3.yaws, r3.yaw_step, r3.refine_yaw) == (5, 7, A.register_params().yaw_step, 1)
+    assert (r3.radius, r3.yaws, r3.yaw_step, r3.refine_yaw, r3.z_search) == (5, 7, A.register_params().yaw_step, 1, 1)
+    m.set_register(A.register_params(z_search=0))
+    m.save(str(p))
+    assert A.Model(ctx, path=str(p)).get_register().z_search == 0
     xb = L.bf16_bits(np.abs(r.normal(size=(128, 256))).astype(np.float32))
     assert np.array_equal(m.forward(xb), m2.forward(xb))
 
@@ -212,7 +217,9 @@ def trained(A, ctx):
         wide = rng.uniform(size=len(idx)) < 0.5
         guesses = np.where(wide[:, None, None], A.perturb_poses(tr_gt[idx], rng),
                            A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0))
-        reg, _ = bm.register(trS, idx, guesses, rp)
+        # registered as bench.py does: a coarse search (+-4 m, +-10 deg), then rp
+        coarse = A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)))
+        reg, _ = bm.register(trS, idx, bm.register(trS, idx, guesses, coarse)[0], rp)
         if ds is None:
             ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])
             ds_reg = A.Dataset(ctx, bm, trS, idx, reg, labels[idx])
@@ -224,7 +231,7 @@ def trained(A, ctx):
     losses = model.train(ds, A.train_params(epochs=8, batch=512))
     model_reg = A.Model(ctx, seed=1)
     model_reg.set_register(rp)
-    losses_reg = model_reg.train(ds_reg, A.train_params(epochs=8, batch=512))
+    losses_reg = model_reg.train(ds_reg, A.train_params(epochs=8, batch=512, weights=[0.1, 0.1, 1, 1, 1, 1]))
     ds.close()
     ds_reg.close()
     ev, ev_gt, _ = A.make_scans(w.scene, w.map_xyzw, A.scan_sizes(1000, 60000, 256, 77), 900000 + np.arange(256))

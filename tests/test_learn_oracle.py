@@ -195,24 +195,41 @@ def _scene(A):
     return mp, G, scan
 
 
-@pytest.mark.parametrize("shift,steps", [((1.0, -1.5), 0), ((0.0, 0.0), 2), ((-2.0, 0.5), -3), ((2.5, 2.0), 3),
-                                         ((-2.5, -3.0), 1)])
+@pytest.mark.parametrize("shift,steps", [((1.0, -1.5, 0.0), 0), ((0.0, 0.0, 0.75), 2), ((-2.0, 0.5, -0.4), -3),
+                                         ((2.5, 2.0, 1.2), 3), ((-2.5, -3.0, 0.0), 1)])
 def test_register_recovers_planted_offsets(A, shift, steps):
     """A scan made of the map's own points seen from the true pose T, and the guess
-    G = T T(-d) Rz(-steps yaw_step), d a whole number of cells and the yaw within
-    the searched ones (the outermost without a neighbour to refine with): the registration
-    must return T within 3 cm and 0.05 degrees (observed: 1 cm, 0.008 degrees; the
-    shift applied along the guess's axes instead of the searched yaw's would miss
-    by 8 cm at 3 steps)."""
+    G = T T(-d) Rz(-steps yaw_step), d whole cells on x and y and whole height bins
+    on z, the yaw within the searched ones (the outermost without a neighbour to
+    refine with): the registration must return T within 3 cm on x and y, 1 cm on z
+    and 0.05 degrees (observed: 1 cm, 0.2 mm, 0.008 degrees; the shift applied
+    along the guess's axes instead of the searched yaw's would miss by 8 cm at 3
+    steps, a height shift of the wrong sign by twice d_z).  Without the height
+    search the height stays as it is."""
     mp, T, scan = _scene(A)
     rp = L.register_params_default()
-    d = np.array([shift[0], shift[1], 0.0, 0.0, 0.0, 0.0])
+    d = np.array([shift[0], shift[1], shift[2], 0.0, 0.0, 0.0])
     G = T @ L.pose_matrix(-d) @ L.pose_matrix([0, 0, 0, 0, 0, -steps * rp["yaw_step"]])
     gl, gw, v = L.register(scan, mp, G, G, rp)
     assert np.array_equal(gl, gw) and v > 1000
     p = L.correction_params(gl, T)
-    assert np.hypot(p[0], p[1]) < 0.03 and abs(p[2]) < 1e-9, p
+    assert np.hypot(p[0], p[1]) < 0.03 and abs(p[2]) < 0.01, p
     assert abs(p[5]) < np.radians(0.05), p
+    gl2, _, _ = L.register(scan, mp, G, G, dict(rp, z_search=0))
+    assert abs(L.correction_params(gl2, T)[2] - shift[2]) < 1e-9
+
+
+def test_height_shift_rules():
+    sc = np.zeros(2 * L.ZH_RANGE + 1, np.int64)
+    assert L.z_shift(sc) == 0.0
+    sc[L.ZH_RANGE + 4] = sc[L.ZH_RANGE - 4] = 10     # a tie: the positive shift
+    assert L.z_shift(sc) == 4 * L.ZH_BIN
+    sc[L.ZH_RANGE + 3] = 10                          # the same score nearer zero
+    assert abs(L.z_shift(sc) - 3.5 * L.ZH_BIN) < 1e-15   # parabola through 0, 10, 10: +0.5
+    hs = np.zeros(L.ZH_BINS, np.int64)
+    hm = np.zeros(L.ZH_BINS, np.int64)
+    hs[100], hm[107] = 3, 5                          # map heights 7 bins above the scan's
+    assert L.z_scores(hs, hm)[L.ZH_RANGE + 7] == 15 and L.z_scores(hs, hm).sum() == 15
 
 
 def test_register_without_common_cells_keeps_the_guess(A):
