@@ -153,7 +153,7 @@ def train_proposer(ctx, w, grid, bm, prm, n_scans, per_scan, epochs, weight_deca
                        loss_first=float(losses[0]), loss_last=float(losses[-1]))
 
 
-def cfg4_leg(ctx, w, grid, bm, model, prm, policy, n_scans, steps) -> dict:
+def cfg4_leg(ctx, w, grid, bm, model, prm, prm_l, policy, n_scans, steps) -> dict:
     """Held-out evaluation of the learned warm start: n_scans scans of the batched
     kind (1 k - 60 k points, log-uniform; guesses +-1 m / +-3 deg yaw / +-0.5 deg
     roll-pitch) of the same map, from generator seeds disjoint from the training
@@ -171,10 +171,10 @@ def cfg4_leg(ctx, w, grid, bm, model, prm, policy, n_scans, steps) -> dict:
     for _ in range(steps):
         plain = evS.align(grid, ev_g, prm)
     plain_s = (time.perf_counter() - t) / steps
-    model.align(grid, bm, evS, ev_g, prm, policy)
+    model.align(grid, bm, evS, ev_g, prm_l, policy)
     t = time.perf_counter()
     for _ in range(steps):
-        res, used = model.align(grid, bm, evS, ev_g, prm, policy)
+        res, used = model.align(grid, bm, evS, ev_g, prm_l, policy)
     learned_s = (time.perf_counter() - t) / steps
     evS.close()
     sys.path.insert(0, os.path.join(REPO, "oracle"))
@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--weight-decay", type=float, default=0.05)
     ap.add_argument("--policy", choices=["race", "fallback"], default="fallback")
+    ap.add_argument("--learned-fine-pass", type=int, default=12,
+                    help="pass of a run from which the learned step's launch cuts scans into quarter-size chunks")
     ap.add_argument("--no-register", action="store_true",
                     help="no BEV registration in front of the model (it then corrects the given guess)")
     ap.add_argument("--cfg4-scans", type=int, default=2048, help="held-out scans of the learned evaluation (0: off)")
@@ -490,15 +492,19 @@ def main():
                                          loss_weights=REGISTERED_LOSS_WEIGHTS)
         log(f"proposer trained: {tinfo}")
         policy = A.LEARNED_RACE if args.policy == "race" else A.LEARNED_FALLBACK
+        # the learned step's launch shape: the batch's, with fine chunks from pass
+        # learned_fine_pass of a run on (its runs start near the optimum and are short)
+        prm_l = A.NdtParams.from_buffer_copy(prm)
+        prm_l.fine_pass = args.learned_fine_pass
         if not args.full_pose:
             policy |= A.LEARNED_KEEP_ROTATION
         for _ in range(args.warmup):
-            model.align(grid, bm, scans, w.guesses, prm, policy)
+            model.align(grid, bm, scans, w.guesses, prm_l, policy)
         # the timed step: scan and map grids, the MLP, the alignment launch, results
         align_ms_l, step_ms_l = [], []
         t_l = time.perf_counter()
         for _ in range(args.steps):
-            res_l, used = model.align(grid, bm, scans, w.guesses, prm, policy)
+            res_l, used = model.align(grid, bm, scans, w.guesses, prm_l, policy)
             ms_l, _, pairs_l = ctx.last_timing()
             align_ms_l.append(ms_l)
             step_ms_l.append(ctx.last_step_ms())
@@ -511,7 +517,7 @@ def main():
             t_u = time.perf_counter()
             for _ in range(args.steps):
                 fresh = A.Scans(ctx, w.scans)
-                model.align(grid, bm, fresh, w.guesses, prm, policy)
+                model.align(grid, bm, fresh, w.guesses, prm_l, policy)
                 fresh.close()
             upload_s = (time.perf_counter() - t_u) / args.steps
         refined = model.predict(bm, scans, w.guesses)
@@ -522,7 +528,7 @@ def main():
             refined[:, :3, :3] = base[:, :3, :3]
         policy_name = args.policy + ("" if args.full_pose else ", translation from the model") + \
             ("" if args.no_register else ", BEV registration before the model")
-        learned = dict(model=model, bm=bm, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
+        learned = dict(model=model, bm=bm, prm=prm_l, res=res_l, used=used, step_s=learned_s, kernel_ms=learned_ms,
                        align_ms=float(np.mean(align_ms_l)), pairs=pairs_l, refined=refined, train=tinfo,
                        policy=policy_name, upload_s=upload_s, policy_flags=policy)
         log(f"batched+learned {S / learned_s:.0f}/s ({S / upload_s:.0f}/s with the scan upload in the step)")
@@ -608,7 +614,8 @@ def main():
         "mean_passes_batched_plain": float(np.mean([r["passes"] for r in res_b])),
         "grid_build_ms": ginfo["build_ms"],
         "launch_shape": {"block_size": prm.block_size or 256, "points_per_wg": prm.points_per_wg or 2048,
-                         "autotuned": tuned_ms is not None},
+                         "fine_pass": prm.fine_pass, "autotuned": tuned_ms is not None,
+                         "learned_fine_pass": learned["prm"].fine_pass if learned else None},
         "profile_us_batched_plain": {k: float(v) for k, v in zip(("lists", "compute", "wait", "control"), profile)},
         "roofline": roof_l or roof_b,
         "roofline_batched_no_proposer": roof_b,
@@ -629,7 +636,7 @@ def main():
                               with_scan_upload_alignments_per_s=(None if np.isnan(learned["upload_s"])
                                                                  else S / learned["upload_s"]))
         if args.cfg4_scans > 0:
-            out["cfg4_held_out"] = cfg4_leg(ctx, w, grid, learned["bm"], learned["model"], prm,
+            out["cfg4_held_out"] = cfg4_leg(ctx, w, grid, learned["bm"], learned["model"], prm, learned["prm"],
                                             learned["policy_flags"], args.cfg4_scans, max(1, min(args.steps, 5)))
             log(f"cfg4: {out['cfg4_held_out']['passes_saved_per_scan']:.2f} passes saved per scan")
     # the reference's order of passes (Hessian only at accepted steps): same poses,

@@ -83,7 +83,11 @@ extern "C" void ndt_params_default(ndt_params* p, int32_t mode) {
      * than a centimetre; float32 modes remain selectable */
     p->point_precision = NDT_POINT_FP64;
     p->sum_precision = NDT_SUM_FP64;
+    p->fine_pass = DEFAULT_FINE_PASS;
 }
+
+/* first pass of a run in fine chunks */
+static int32_t fine_pass_of(const ndt_params* p) { return p->fine_pass > 0 ? p->fine_pass : DEFAULT_FINE_PASS; }
 
 static int check_params(const ndt_params* p) {
     if (!p) return fail(NDT_ERR_INVALID, "params is null");
@@ -101,6 +105,7 @@ static int check_params(const ndt_params* p) {
     if (p->scheduler < NDT_SCHED_DEFAULT || p->scheduler > NDT_SCHED_QUEUE) return fail(NDT_ERR_INVALID, "bad scheduler");
     if (p->hessian_at_accept != 0 && p->hessian_at_accept != 1)
         return fail(NDT_ERR_INVALID, "hessian_at_accept must be 0 or 1");
+    if (p->fine_pass < 0 || p->fine_pass > (1 << 20)) return fail(NDT_ERR_INVALID, "fine_pass must be in 0 .. 2^20");
     return 0;
 }
 
@@ -720,6 +725,8 @@ static void make_descs(const int64_t* offsets, const int64_t* counts, int32_t S,
         desc[s].chunk_base = slots;
         /* first-pass items are packed in scan order; slots cover the finer chunking */
         desc[s].qbase = first;
+        desc[s].fine_pass = fine_pass_of(p);
+        desc[s].pad = 0;
         slots += desc[s].fnchunks;
         first += desc[s].nchunks;
         max_chunks = std::max<int64_t>(max_chunks, desc[s].fnchunks);
@@ -1055,7 +1062,7 @@ extern "C" int ndt_align_scans_noalloc(ndt_ctx* c, const ndt_grid* g, const ndt_
     HIP_TRY(hipMemcpyAsync(d_counts, sc->counts.data(), sizeof(int64_t) * (size_t)S, hipMemcpyHostToDevice,
                            c->stream));
     HIP_TRY(hipMemcpyAsync(d_guess, guesses, sizeof(double) * 16 * (size_t)S, hipMemcpyHostToDevice, c->stream));
-    LAUNCH_TRY(launch_make_descs(d_counts, S, chunk, d_desc, c->stream));
+    LAUNCH_TRY(launch_make_descs(d_counts, S, chunk, fine_pass_of(p), d_desc, c->stream));
     LAUNCH_TRY(launch_zero_u32(d_out, (int64_t)(sizeof(ScanOut) * (size_t)S / 4), c->stream));
     LAUNCH_TRY(launch_zero_u32(d_items, (int64_t)(2 * cap), c->stream));
     LAUNCH_TRY(launch_zero_u32(d_prof, 16, c->stream));
@@ -1155,7 +1162,7 @@ static bool same_align_params(const ndt_params& a, const ndt_params& b) {
            a.point_precision == b.point_precision && a.sum_precision == b.sum_precision &&
            a.origin[0] == b.origin[0] && a.origin[1] == b.origin[1] && a.origin[2] == b.origin[2] &&
            a.block_size == b.block_size && a.points_per_wg == b.points_per_wg && a.scheduler == b.scheduler &&
-           a.hessian_at_accept == b.hessian_at_accept;
+           a.hessian_at_accept == b.hessian_at_accept && fine_pass_of(&a) == fine_pass_of(&b);
 }
 
 bool ndt::same_params(const ndt_params& a, const ndt_params& b) { return same_align_params(a, b); }
@@ -1303,9 +1310,9 @@ static std::string tune_key(ndt_ctx* c, const ndt_scans* sc, const ndt_params* p
     double total = (double)sc->total, mag = 1.0;
     while (total >= 1000.0 * mag) mag *= 10.0;
     const double rounded = std::round(total / mag) * mag;
-    std::snprintf(buf, sizeof(buf), "dev=%.120s;cus=%d;pp=%d;sp=%d;ha=%d;S=%d;N=%.0f", c->name, c->cus,
+    std::snprintf(buf, sizeof(buf), "dev=%.120s;cus=%d;pp=%d;sp=%d;ha=%d;fp=%d;S=%d;N=%.0f", c->name, c->cus,
                   p->point_precision, p->point_precision == NDT_POINT_FP64 ? NDT_SUM_FP64 : p->sum_precision,
-                  p->hessian_at_accept, sc->S, rounded);
+                  p->hessian_at_accept, fine_pass_of(p), sc->S, rounded);
     std::string k(buf);
     for (auto& ch : k)
         if (ch == ' ' || ch == '\n' || ch == '\t') ch = '_';

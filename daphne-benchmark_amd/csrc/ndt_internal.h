@@ -335,31 +335,35 @@ struct ScanDesc {
     int32_t n;          /* points */
     int32_t nchunks;    /* chunks of `chunk` points */
     int32_t chunk;      /* points per partial sum */
-    int32_t fnchunks;   /* chunks of fine_chunk(chunk) points (late passes) */
+    int32_t fnchunks;   /* chunks of fine_chunk(chunk) points (passes from fine_pass on) */
     int64_t qbase;      /* queue position of the scan's first-pass chunks */
+    int32_t fine_pass;  /* first pass of a run in fine chunks */
+    int32_t pad;
 };
 
-/* Passes from this one on cut a scan into chunks a quarter the size (at least 256
- * points): the scans that need many passes set a batch's finish time, and finer
- * chunks spread each of their late passes over more workgroups (on the 512-scan,
- * 1 M-point-map batch: 6.45 ms against 7.0 ms with one chunk size; passes from 20,
- * 24, 32 or 36 on, chunks an eighth or half the size, or a further level of
- * sixteenth-size chunks from pass 36, 40 or 44 on, gave 6.45-7.0 ms).  The
- * chunking is a function of the pass number only, so results stay independent of
- * scheduling. */
-#ifndef NDT_FINE_PASS
-#define NDT_FINE_PASS 28
-#endif
+/* Passes of a run from fine_pass on (default DEFAULT_FINE_PASS) cut a scan into
+ * chunks a quarter the size (at least 256 points): the scans that need many passes
+ * set a batch's finish time, and finer chunks spread each of their late passes
+ * over more workgroups (on the 512-scan, 1 M-point-map batch from its given
+ * guesses: 6.45 ms against 7.0 ms with one chunk size; passes from 20, 24, 32 or 36
+ * on, chunks an eighth or half the size, or a further level of sixteenth-size
+ * chunks from pass 36, 40 or 44 on, gave 6.45-7.0 ms; with starts a registration
+ * put near the optimum, from pass 12 on).  The chunking is a function of the pass
+ * number only, so results stay independent of scheduling. */
+constexpr int32_t DEFAULT_FINE_PASS = 28;
 #ifndef NDT_FINE_DIV
 #define NDT_FINE_DIV 4
 #endif
-constexpr uint32_t FINE_PASS = NDT_FINE_PASS;
 NDT_HD int32_t fine_chunk(int32_t chunk) {
     const int32_t q = chunk / NDT_FINE_DIV;
     return q >= 256 ? q : (chunk < 256 ? chunk : 256);
 }
-NDT_HD int32_t pass_chunk(const ScanDesc& d, uint32_t pass) { return pass >= FINE_PASS ? fine_chunk(d.chunk) : d.chunk; }
-NDT_HD int32_t pass_nchunks(const ScanDesc& d, uint32_t pass) { return pass >= FINE_PASS ? d.fnchunks : d.nchunks; }
+NDT_HD int32_t pass_chunk(const ScanDesc& d, uint32_t pass) {
+    return pass >= (uint32_t)d.fine_pass ? fine_chunk(d.chunk) : d.chunk;
+}
+NDT_HD int32_t pass_nchunks(const ScanDesc& d, uint32_t pass) {
+    return pass >= (uint32_t)d.fine_pass ? d.fnchunks : d.nchunks;
+}
 
 struct AlignParams {
     double step_size, trans_eps, d1, d2;
@@ -510,7 +514,8 @@ constexpr int NOALLOC_MAX_DEVICES = 64;
 /* device address of this device's workspace array (call with `device` current) */
 int noalloc_scratch(int device, void** ptr);
 /* chunk descriptors of S scans from their point counts (device memory) */
-int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, ScanDesc* desc, hipStream_t stream);
+int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, int32_t fine_pass, ScanDesc* desc,
+                      hipStream_t stream);
 /* ndt_result records from the outputs of a launch; prof[5] receives the pair total */
 int launch_noalloc_results(const ScanOut* out, const ScanDesc* desc, int32_t S, double* prof, ndt_result* res,
                            hipStream_t stream);

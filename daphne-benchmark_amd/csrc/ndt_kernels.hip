@@ -2833,7 +2833,8 @@ constexpr int DESC_BLOCK = 1024;
  * and first-pass queue items laid out in scan order.  One workgroup; exclusive
  * prefix sums tile by tile with a running carry. */
 __global__ __launch_bounds__(DESC_BLOCK) void k_make_descs(const int64_t* __restrict__ counts, int32_t S,
-                                                           int32_t chunk, ScanDesc* __restrict__ desc) {
+                                                           int32_t chunk, int32_t fine_pass,
+                                                           ScanDesc* __restrict__ desc) {
     __shared__ int64_t wsum[3][DESC_BLOCK / WAVE];
     __shared__ int64_t carry[3];
     const int tid = threadIdx.x, lane = tid & (WAVE - 1), wave = tid / WAVE;
@@ -2867,6 +2868,8 @@ __global__ __launch_bounds__(DESC_BLOCK) void k_make_descs(const int64_t* __rest
             d.fnchunks = (int32_t)v[1];
             d.chunk_base = ex[1];
             d.qbase = ex[2];
+            d.fine_pass = fine_pass;
+            d.pad = 0;
             desc[s] = d;
         }
         __syncthreads();
@@ -2879,8 +2882,9 @@ __global__ __launch_bounds__(DESC_BLOCK) void k_make_descs(const int64_t* __rest
     }
 }
 
-int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, ScanDesc* desc, hipStream_t stream) {
-    hipLaunchKernelGGL(k_make_descs, dim3(1), dim3(DESC_BLOCK), 0, stream, counts, S, chunk, desc);
+int launch_make_descs(const int64_t* counts, int32_t S, int32_t chunk, int32_t fine_pass, ScanDesc* desc,
+                      hipStream_t stream) {
+    hipLaunchKernelGGL(k_make_descs, dim3(1), dim3(DESC_BLOCK), 0, stream, counts, S, chunk, fine_pass, desc);
     return (int)hipGetLastError();
 }
 

@@ -388,7 +388,7 @@ class NdtParams(C.Structure):
                 ("max_iterations", C.c_int32), ("outlier_ratio", C.c_double), ("min_points", C.c_int32),
                 ("min_eig_ratio", C.c_double), ("point_precision", C.c_int32), ("sum_precision", C.c_int32),
                 ("origin", C.c_double * 3), ("block_size", C.c_int32), ("points_per_wg", C.c_int32),
-                ("scheduler", C.c_int32), ("hessian_at_accept", C.c_int32)]
+                ("scheduler", C.c_int32), ("hessian_at_accept", C.c_int32), ("fine_pass", C.c_int32)]
 
 
 class NdtResult(C.Structure):

@@ -79,6 +79,11 @@ typedef struct ndt_params {
                                   reference's order (computeDerivatives(..., false) at
                                   kernel.cpp:1123, computeHessian at :1141-1142).  Same poses,
                                   more passes. */
+    int32_t fine_pass;        /* ndt align: passes of a run from this one on cut a scan into
+                                 chunks a quarter the size (at least 256 points), spreading the
+                                 late passes of the slowest scans over more workgroups; 0 = 28.
+                                 Part of the launch shape: results depend on it at the level of
+                                 rounding, as on points_per_wg */
 } ndt_params;
 
 typedef struct ndt_result {

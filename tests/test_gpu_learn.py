@@ -303,14 +303,17 @@ def test_learned_warm_start_gains(A, trained):
     assert np.mean([r["converged"] for r in race]) >= np.mean([r["converged"] for r in plain])
 
 
-@pytest.mark.parametrize("which", ["model", "model_reg"])
-def test_policies_return_plain_or_learned_runs(A, trained, which):
+@pytest.mark.parametrize("which,fine_pass", [("model", 0), ("model_reg", 0), ("model_reg", 12)])
+def test_policies_return_plain_or_learned_runs(A, trained, which, fine_pass):
     """Every result of either policy, with the model's full pose or only its
     translation, is bit for bit the NDT run from the given guess or the one from the
     refined guess, as used_learned says; the race keeps the better final score; an
     unknown policy is rejected.  With a registration the refined guess corrects the
     registered guess, whose rotation the translation-only variant keeps."""
-    t = trained
+    t = dict(trained)
+    if fine_pass:
+        t["prm"] = A.NdtParams.from_buffer_copy(t["prm"])
+        t["prm"].fine_pass = fine_pass
     ev, eg, m = t["evS"], t["eg"], t[which]
     pred = m.predict(t["bm"], ev, eg)
     base = eg if which == "model" else t["bm"].register(ev, np.arange(ev.S, dtype=np.int32), eg, t["rp"])[0]

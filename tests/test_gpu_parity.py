@@ -153,26 +153,32 @@ def test_schedulers_and_batch_composition_agree(A, gpu_grid, cfg0):
         assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"]
 
 
-def test_late_pass_chunking_agrees(A, gpu_grid, cfg0):
-    """From the 28th pass on a scan's chunks are a quarter the size.  With guesses
-    +-3 m / +-10 deg off, some scans run that long: queue and rounds still give
-    bit-identical results (both derive the chunking from the pass number), and a
-    scan's result does not depend on its batch."""
+@pytest.mark.parametrize("fine_pass", [0, 6])
+def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
+    """From pass fine_pass of a run on (0: the 28th) a scan's chunks are a quarter
+    the size.  With guesses +-3 m / +-10 deg off, some scans run past it: queue and
+    rounds still give bit-identical results (both derive the chunking from the pass
+    number), a scan's result does not depend on its batch, and another fine_pass
+    moves results only at the level of rounding."""
     scene, mp, _ = cfg0
     sizes = [2600, 4100, 1800, 3300, 2200, 5200, 1500, 3900]
     made = [A.make_scan(scene, mp, A.scan_cfg(n), 300 + k) for k, n in enumerate(sizes)]
     pts = [m[0] for m in made]
     gs = A.perturb_poses(np.stack([m[1] for m in made]), np.random.default_rng(5), xy=3.0, yaw_deg=10.0)
     for ppw in (0, 1024):
-        q = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, points_per_wg=ppw))
-        r = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_ROUNDS, points_per_wg=ppw))
-        assert max(x["passes"] for x in q) > 28, [x["passes"] for x in q]
+        kw = dict(points_per_wg=ppw, fine_pass=fine_pass)
+        q = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, **kw))
+        r = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_ROUNDS, **kw))
+        assert max(x["passes"] for x in q) > (fine_pass or 28), [x["passes"] for x in q]
         for x, y in zip(q, r):
             assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"], ppw
         k = int(np.argmax([x["passes"] for x in q]))
         alone = gpu_grid.align_batch(pts[k:k + 1], gs[k:k + 1],
-                                     prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, points_per_wg=ppw))
+                                     prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, **kw))
         assert np.array_equal(alone[0]["final_f64"], q[k]["final_f64"]) and alone[0]["passes"] == q[k]["passes"]
+        if fine_pass:
+            d = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw))
+            assert max(np.abs(x["final_f64"][:3, 3] - y["final_f64"][:3, 3]).max() for x, y in zip(q, d)) < 1e-3
 
 
 def test_deterministic(A, gpu_grid, cfg0):
