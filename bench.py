@@ -62,11 +62,11 @@ def profiled_traffic(args, prm, kind="plain"):
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch " + kind + r": (\d+) \(workload (\w+), (\d+) scans, (\w+), block (\d+), "
-                  r"chunk (\d+)\)", text)
+                  r"chunk (\d+), fine_pass (\d+)\)", text)
     if not m:
         return None
-    shape = (prm.block_size or 256, prm.points_per_wg or 2048)
-    if (m.group(2), int(m.group(3)), m.group(4), (int(m.group(5)), int(m.group(6)))) != (
+    shape = (prm.block_size or 256, prm.points_per_wg or 2048, prm.fine_pass or 28)
+    if (m.group(2), int(m.group(3)), m.group(4), (int(m.group(5)), int(m.group(6)), int(m.group(7)))) != (
             args.workload, args.scans, args.precision, shape):
         return None
     return int(m.group(1))
@@ -560,7 +560,7 @@ def main():
     def roofline(results, pairs, ms, kernel, traffic_kind):
         bytes_alg, flops_alg = algorithmic_work(results, counts, pairs)
         achieved = bytes_alg / (ms * 1e-3) / 1e9
-        traffic = profiled_traffic(args, prm, traffic_kind)
+        traffic = profiled_traffic(args, learned["prm"] if traffic_kind == "learned" else prm, traffic_kind)
         return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": TRAFFIC_PROFILE if traffic else None,
                 "algorithmic_bytes": bytes_alg, "kernel_ms": ms,
