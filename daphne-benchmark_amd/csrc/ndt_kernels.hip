@@ -1815,27 +1815,36 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& __restrict__ st, c
         return;
     }
     /* a line-search evaluation at x_t = p + a_t dir */
+    const bool red_has_hess = !(prm.hess_at_accept && st.phase == PH_MT_LOOP);
     double g_t[6];
     for (int i = 0; i < 6; i++) g_t[i] = red[1 + i];
-    double phi_t = -red[0];
-    double d_phi_t = -dot6(g_t, st.dir);
-    double psi_t = psi_mt(st.a_t, phi_t, st.phi_0, st.d_phi_0, mu);
-    double d_psi_t = dpsi_mt(d_phi_t, st.d_phi_0, mu);
-    if (st.phase == PH_MT_LOOP) {
-        if (st.open_interval && (psi_t <= 0 && d_psi_t >= 0)) {
-            st.open_interval = 0;
-            st.f_l = st.f_l + st.phi_0 - mu * st.d_phi_0 * st.a_l;
-            st.g_l = st.g_l + mu * st.d_phi_0;
-            st.f_u = st.f_u + st.phi_0 - mu * st.d_phi_0 * st.a_u;
-            st.g_u = st.g_u + mu * st.d_phi_0;
+    const double phi_t = -red[0];
+    const double d_phi_t = -dot6(g_t, st.dir);
+    const double psi_t = psi_mt(st.a_t, phi_t, st.phi_0, st.d_phi_0, mu);
+    const double d_psi_t = dpsi_mt(d_phi_t, st.d_phi_0, mu);
+    /* A trial at the step of the trial before it (the trial value clamped to the
+     * same bound again) is at the same pose, whose pass would reproduce `red` bit
+     * for bit: the search takes `red` again without a pass, for as long as it
+     * repeats the step. */
+    for (;;) {
+        if (st.phase == PH_MT_LOOP) {
+            if (st.open_interval && (psi_t <= 0 && d_psi_t >= 0)) {
+                st.open_interval = 0;
+                st.f_l = st.f_l + st.phi_0 - mu * st.d_phi_0 * st.a_l;
+                st.g_l = st.g_l + mu * st.d_phi_0;
+                st.f_u = st.f_u + st.phi_0 - mu * st.d_phi_0 * st.a_u;
+                st.g_u = st.g_u + mu * st.d_phi_0;
+            }
+            if (st.open_interval)
+                st.interval_converged =
+                    mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, psi_t, d_psi_t);
+            else
+                st.interval_converged =
+                    mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, phi_t, d_phi_t);
+            st.step_iterations++;
         }
-        if (st.open_interval)
-            st.interval_converged = mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, psi_t, d_psi_t);
-        else
-            st.interval_converged = mt_update_interval(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, phi_t, d_phi_t);
-        st.step_iterations++;
-    }
-    if (!st.interval_converged && st.step_iterations < 10 && !(psi_t <= 0 && d_phi_t <= -nu * st.d_phi_0)) {
+        if (st.interval_converged || st.step_iterations >= 10 || (psi_t <= 0 && d_phi_t <= -nu * st.d_phi_0))
+            break;
         double a_t;
         if (st.open_interval)
             a_t = mt_trial_value(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, psi_t, d_psi_t);
@@ -1843,16 +1852,19 @@ __device__ __attribute__((noinline)) void ctl_step(ScanState& __restrict__ st, c
             a_t = mt_trial_value(st.a_l, st.f_l, st.g_l, st.a_u, st.f_u, st.g_u, st.a_t, phi_t, d_phi_t);
         a_t = dmin_std(a_t, prm.step_size);
         a_t = dmax_std(a_t, prm.trans_eps / 2);
+        const bool repeat = a_t == st.a_t;
+        st.phase = PH_MT_LOOP;
+        if (repeat) continue;
         st.a_t = a_t;
         for (int i = 0; i < 6; i++) st.x_t[i] = st.p[i] + st.dir[i] * a_t;
-        st.phase = PH_MT_LOOP;
         return;
     }
     /* accept x_t: its pass supplies score and gradient, and the Hessian unless the
-     * loop's passes skipped it (then one more pass at x_t computes it) */
+     * loop's passes skipped it (then one more pass at x_t computes it, unless the
+     * pass in `red`, at the same pose, did) */
     st.score = red[0];
     for (int i = 0; i < 6; i++) st.grad[i] = g_t[i];
-    if (prm.hess_at_accept && st.phase == PH_MT_LOOP) {
+    if (prm.hess_at_accept && st.phase == PH_MT_LOOP && !red_has_hess) {
         st.phase = PH_MT_HESS;
         return;
     }

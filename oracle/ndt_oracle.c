@@ -716,7 +716,7 @@ void orc_ndt_params_default(orc_ndt_params* p) {
     p->min_points = 6;
     p->min_eig_ratio = 0.01;
     p->hessian_at_accept = 0;
-    p->pad = 0;
+    p->repeat_passes = 0;
 }
 
 void orc_sym3_eig(const double Ain[9], double evals[3], double evecs[9]) {
@@ -1120,17 +1120,29 @@ typedef struct ndt_line {
     double* score;
     double* grad;
     double* H;
+    double last_t; /* line parameter of the previous evaluation (NaN before the first) */
+    int last_h;    /* the pass whose sums are in the state computed the Hessian */
 } ndt_line;
 
 /* phi(t) = -score(x + t dir), phi'(t) = -grad . dir: one derivative pass at the trial
  * pose (kernel.cpp:1058-1066, :1114-1128), leaving score, gradient and (when asked
- * for) Hessian of the last trial in the driver's state. */
+ * for) Hessian of the last trial in the driver's state.  An evaluation at the same
+ * t as the one before it (the search clamps successive trials to the same bound,
+ * step_min or step_max) is at the same pose, and its pass would reproduce the sums
+ * still in the state bit for bit: unless prm->repeat_passes is set, or it asks for
+ * a Hessian that pass did not compute, it takes them without a pass. */
 static double ndt_line_eval(void* vctx, double t, int with_hessian, double* dphi) {
     ndt_line* L = (ndt_line*)vctx;
+    if (t == L->last_t && (!with_hessian || L->last_h) && !L->c->prm->repeat_passes) {
+        *dphi = -(dot6(L->grad, L->dir));
+        return -*L->score;
+    }
     for (int i = 0; i < 6; i++) L->x_t[i] = L->x[i] + L->dir[i] * t;
     *L->score = ndt_pass(L->c->g, L->c->prm, L->c->d1, L->c->d2, L->c->scan4, L->c->n, L->x_t, L->grad,
                          with_hessian ? L->H : NULL, 1, with_hessian, NULL);
     L->c->passes++;
+    L->last_t = t;
+    L->last_h = with_hessian;
     *dphi = -(dot6(L->grad, L->dir));
     return -*L->score;
 }
@@ -1171,7 +1183,7 @@ int orc_ndt_align(const orc_ndt_grid* g, const orc_ndt_params* prm, const float*
         }
         for (int i = 0; i < 6; i++) delta_p[i] /= norm;
         /* computeStepLengthMT (kernel.cpp:1015-1144), phi(0) = -score, phi'(0) = -grad . dir */
-        ndt_line L = {&c, p, delta_p, x_t, &score, grad, H};
+        ndt_line L = {&c, p, delta_p, x_t, &score, grad, H, NAN, 0};
         orc_mt_out mt;
         orc_mt_search(ndt_line_eval, &L, -score, -(dot6(grad, delta_p)), norm, prm->step_size, prm->trans_eps / 2,
                       prm->hessian_at_accept ? ORC_MT_HESSIAN_AT_ACCEPT : 0, &mt);

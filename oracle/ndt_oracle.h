@@ -143,7 +143,11 @@ typedef struct orc_ndt_params {
                                   one more pass computes at the accepted step (the
                                   reference's order, kernel.cpp:1123,1141-1142); 0: every
                                   pass computes it (fewer passes, same poses) */
-    int32_t pad;
+    int32_t repeat_passes;     /* 1: a line-search trial at the step of the trial before it
+                                  (the trial value clamped to the same bound again) makes
+                                  its pass again, as the reference's loop does; 0: it takes
+                                  that pass's sums, which the pass would reproduce bit for
+                                  bit (fewer passes, same poses) */
 } orc_ndt_params;
 
 void orc_ndt_params_default(orc_ndt_params* p);

@@ -38,7 +38,7 @@ class OrcCompatGrid(C.Structure):
 class OrcNdtParams(C.Structure):
     _fields_ = [("resolution", D), ("step_size", D), ("trans_eps", D), ("outlier_ratio", D),
                 ("max_iterations", C.c_int32), ("min_points", C.c_int32), ("min_eig_ratio", D),
-                ("hessian_at_accept", C.c_int32), ("pad", C.c_int32)]
+                ("hessian_at_accept", C.c_int32), ("repeat_passes", C.c_int32)]
 
 
 class OrcMtOut(C.Structure):

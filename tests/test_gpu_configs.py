@@ -73,24 +73,28 @@ def cfg3(A, orc, ctx):
 def test_cfg1_sequential_scans_match_oracle(A, orc, ctx):
     """16 sequential 20 k-point scans on the 1 M-point map: within 1e-3 m / 1e-4 rad
     with the same iterations, passes and flag on every reproducible scan, within
-    1 cm / 0.05 deg on all of them."""
+    1 cm / 0.05 deg on all of them; with the default chunk of 2048 points and with
+    the 256-point chunks that minimise a lone scan's latency."""
     w = A.make_workload("cfg1", seed=1, scans=16)
     prm = A.params(A.NDT_MODE_NDT)
     grid = A.Grid(ctx, w.map_xyzw, prm)
     assert grid.info()["map_points"] == 1_000_000
-    res = [grid.align(s, g, prm=prm) for s, g in zip(w.scans, w.guesses)]
     og = orc.NdtGrid(w.map_xyzw)
     ref, stable = _oracle_runs(orc, og, w.scans, w.guesses, w.origin)
-    err = _errors(orc, res, ref)
-    print(f"cfg1: {stable.sum()} of {len(stable)} scans reproducible; max error {err[:, 0].max():.2e} m "
-          f"{err[:, 1].max():.2e} rad", flush=True)
     assert stable.sum() >= 12
-    for k, (r, o) in enumerate(zip(res, ref)):
-        assert r["status"] == 0
-        assert err[k, 0] <= 0.01 and err[k, 1] <= np.deg2rad(0.05), (k, err[k])
-        if stable[k]:
-            assert err[k, 0] <= 1e-3 and err[k, 1] <= 1e-4, (k, err[k])
-            assert (r["iterations"], r["passes"], r["converged"]) == (o["iterations"], o["passes"], o["converged"]), k
+    for chunk in (0, 256):
+        q = A.params(A.NDT_MODE_NDT, points_per_wg=chunk)
+        res = [grid.align(s, g, prm=q) for s, g in zip(w.scans, w.guesses)]
+        err = _errors(orc, res, ref)
+        print(f"cfg1, chunk {chunk}: {stable.sum()} of {len(stable)} scans reproducible; max error "
+              f"{err[:, 0].max():.2e} m {err[:, 1].max():.2e} rad", flush=True)
+        for k, (r, o) in enumerate(zip(res, ref)):
+            assert r["status"] == 0
+            assert err[k, 0] <= 0.01 and err[k, 1] <= np.deg2rad(0.05), (chunk, k, err[k])
+            if stable[k]:
+                assert err[k, 0] <= 1e-3 and err[k, 1] <= 1e-4, (chunk, k, err[k])
+                assert (r["iterations"], r["passes"], r["converged"]) == \
+                    (o["iterations"], o["passes"], o["converged"]), (chunk, k)
 
 
 def test_cfg2_batch_matches_oracle_sample(A, orc, cfg2):

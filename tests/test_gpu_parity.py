@@ -153,18 +153,22 @@ def test_schedulers_and_batch_composition_agree(A, gpu_grid, cfg0):
         assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"]
 
 
-@pytest.mark.parametrize("fine_pass", [0, 6])
+@pytest.mark.parametrize("fine_pass", [0, 12, 6])
 def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
     """From pass fine_pass of a run on (0: the 28th) a scan's chunks are a quarter
-    the size.  With guesses +-3 m / +-10 deg off, some scans run past it: queue and
-    rounds still give bit-identical results (both derive the chunking from the pass
-    number), a scan's result does not depend on its batch, and another fine_pass
-    moves results only at the level of rounding."""
+    the size.  Some scans run past it (guesses +-4 m / +-15 deg off for the default,
+    +-3 m / +-10 deg otherwise): queue and rounds still give bit-identical results
+    (both derive the chunking from the pass number), a scan's result does not
+    depend on its batch, 0 is 28, and another fine_pass moves results only at the
+    level of rounding (on starts whose results are stable to that level)."""
     scene, mp, _ = cfg0
     sizes = [2600, 4100, 1800, 3300, 2200, 5200, 1500, 3900]
     made = [A.make_scan(scene, mp, A.scan_cfg(n), 300 + k) for k, n in enumerate(sizes)]
     pts = [m[0] for m in made]
-    gs = A.perturb_poses(np.stack([m[1] for m in made]), np.random.default_rng(5), xy=3.0, yaw_deg=10.0)
+    if fine_pass:
+        gs = A.perturb_poses(np.stack([m[1] for m in made]), np.random.default_rng(5), xy=3.0, yaw_deg=10.0)
+    else:
+        gs = A.perturb_poses(np.stack([m[1] for m in made]), np.random.default_rng(19), xy=4.0, yaw_deg=15.0)
     for ppw in (0, 1024):
         kw = dict(points_per_wg=ppw, fine_pass=fine_pass)
         q = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, **kw))
@@ -179,6 +183,10 @@ def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
         if fine_pass:
             d = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw))
             assert max(np.abs(x["final_f64"][:3, 3] - y["final_f64"][:3, 3]).max() for x, y in zip(q, d)) < 1e-3
+        else:
+            e = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw, fine_pass=28))
+            for x, y in zip(q, e):
+                assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"], ppw
 
 
 def test_deterministic(A, gpu_grid, cfg0):

@@ -199,3 +199,31 @@ def test_ndt_hessian_at_accept_keeps_poses(A, orc):
         assert 0 <= y["passes"] - x["passes"] <= x["iterations"]
         extra += y["passes"] - x["passes"]
     assert extra > 0  # some loops ran
+
+
+@pytest.mark.parametrize("hessian_at_accept", [0, 1])
+def test_ndt_repeated_trials_keep_poses(A, orc, hessian_at_accept):
+    """A line-search trial clamped to the same step as the trial before it (step_min
+    or step_max) is at the same pose: taking that pass's sums instead of making the
+    pass again gives the same poses, iterations and scores bit for bit, with fewer
+    passes; from starts near the optimum most loop trials are such repeats (a Newton
+    step shorter than step_min is clamped up to it and overshoots)."""
+    scene = A.scene_cfg(1)
+    mp = A.make_map(scene, 60000)
+    again = orc.NdtGrid(mp, orc.ndt_params(hessian_at_accept=hessian_at_accept, repeat_passes=1))
+    reuse = orc.NdtGrid(mp, orc.ndt_params(hessian_at_accept=hessian_at_accept))
+    scans = [A.make_scan(scene, mp, A.scan_cfg(1500, 1.0, 3.0), 900 + s) for s in range(12)]
+    near = [A.make_scan(scene, mp, A.scan_cfg(1500, 0.01, 0.1), 950 + s) for s in range(12)]
+    saved = {}
+    for name, group in (("far", scans), ("near", near)):
+        gs = np.stack([s[2] for s in group])
+        r0 = again.align_many([s[0] for s in group], gs, threads=8)
+        r1 = reuse.align_many([s[0] for s in group], gs, threads=8)
+        for x, y in zip(r0, r1):
+            assert np.array_equal(x["final_f64"], y["final_f64"])
+            assert (x["iterations"], x["converged"], x["score"]) == (y["iterations"], y["converged"], y["score"])
+            assert y["passes"] <= x["passes"]
+        saved[name] = (sum(x["passes"] for x in r0), sum(y["passes"] for y in r1))
+    print(saved)
+    assert saved["far"][1] < saved["far"][0]
+    assert saved["near"][1] < 0.8 * saved["near"][0], saved
