@@ -413,7 +413,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--weight-decay", type=float, default=0.05)
     ap.add_argument("--policy", choices=["race", "fallback"], default="fallback")
-    ap.add_argument("--learned-fine-pass", type=int, default=12,
+    ap.add_argument("--learned-fine-pass", type=int, default=10,
                     help="pass of a run from which the learned step's launch cuts scans into quarter-size chunks")
     ap.add_argument("--no-register", action="store_true",
                     help="no BEV registration in front of the model (it then corrects the given guess)")

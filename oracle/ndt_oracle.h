@@ -27,7 +27,10 @@
  *            derivatives, score/gradient terms, solve and the More-Thuente helpers
  *            (those are pinned by the reference); the remainder is pinned by finite
  *            differences, closed forms and ground-truth recovery
- *            (tests/test_oracle_ndt.py).
+ *            (tests/test_oracle_ndt.py).  A line-search trial at the step of the
+ *            trial before it takes that pass's sums without a new pass (the pose is
+ *            the same); repeat_passes = 1 makes the pass again, and both give the
+ *            same poses bit for bit (tests/test_oracle_linesearch.py).
  */
 #ifndef NDT_ORACLE_H
 #define NDT_ORACLE_H
