@@ -93,7 +93,7 @@ def algorithmic_work(results, counts, pairs):
 
 def coarse_register_params():
     """the first registration of training guesses: +-4 m, +-10 deg in 2 deg steps"""
-    return A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)))
+    return A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)), level=0)
 
 
 REGISTERED_LOSS_WEIGHTS = [0.1, 0.1, 1.0, 1.0, 1.0, 1.0]
@@ -413,7 +413,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--weight-decay", type=float, default=0.05)
     ap.add_argument("--policy", choices=["race", "fallback"], default="fallback")
-    ap.add_argument("--learned-fine-pass", type=int, default=10,
+    ap.add_argument("--learned-fine-pass", type=int, default=8,
                     help="pass of a run from which the learned step's launch cuts scans into quarter-size chunks")
     ap.add_argument("--no-register", action="store_true",
                     help="no BEV registration in front of the model (it then corrects the given guess)")
@@ -490,6 +490,7 @@ def main():
         if rp is not None:
             tinfo["registration"] = dict(radius_cells=rp.radius, yaws=rp.yaws, yaw_step_deg=float(np.degrees(rp.yaw_step)),
                                          refine_yaw=bool(rp.refine_yaw), z_search=bool(rp.z_search),
+                                         level=bool(rp.level),
                                          training_guesses="registered +-4 m / +-10 deg in 2 deg steps, then as above",
                                          loss_weights=REGISTERED_LOSS_WEIGHTS)
         log(f"proposer trained: {tinfo}")

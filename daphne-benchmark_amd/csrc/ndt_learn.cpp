@@ -98,6 +98,7 @@ static int reg_check(const ndt_register_params* rp) {
         return fail(NDT_ERR_INVALID, "registration yaw_step must be in (0, 0.05] rad");
     if (rp->refine_yaw != 0 && rp->refine_yaw != 1) return fail(NDT_ERR_INVALID, "refine_yaw must be 0 or 1");
     if (rp->z_search != 0 && rp->z_search != 1) return fail(NDT_ERR_INVALID, "z_search must be 0 or 1");
+    if (rp->level != 0 && rp->level != 1) return fail(NDT_ERR_INVALID, "level must be 0 or 1");
     return 0;
 }
 
@@ -108,6 +109,7 @@ static RegisterDev reg_dev(const ndt_register_params& rp) {
     d.yaws = rp.yaws;
     d.refine_yaw = rp.refine_yaw;
     d.z_search = rp.z_search;
+    d.level = rp.level;
     d.yaw_step = rp.yaw_step;
     for (int k = 0; k < rp.yaws; k++) {
         const double th = (double)(k - (rp.yaws - 1) / 2) * rp.yaw_step;
@@ -120,7 +122,7 @@ static RegisterDev reg_dev(const ndt_register_params& rp) {
 
 static bool same_reg(const ndt_register_params& a, const ndt_register_params& b) {
     return a.radius == b.radius && a.yaws == b.yaws && a.yaw_step == b.yaw_step && a.refine_yaw == b.refine_yaw &&
-           a.z_search == b.z_search;
+           a.z_search == b.z_search && a.level == b.level;
 }
 
 /* ------------------------------------------------------------------ handles */
@@ -434,6 +436,7 @@ extern "C" void ndt_register_params_default(ndt_register_params* rp) {
     rp->yaw_step = 3.14159265358979323846 / 180.0;
     rp->refine_yaw = 1;
     rp->z_search = 1;
+    rp->level = 1;
 }
 
 extern "C" int ndt_model_set_register(ndt_model* md, const ndt_register_params* rp) {
@@ -480,8 +483,11 @@ extern "C" int ndt_model_set(ndt_model* md, int32_t which, const float* in, int6
 }
 
 static const char MODEL_MAGIC[8] = {'N', 'D', 'T', 'M', 'L', 'P', '0', '1'};
-/* optional trailer after the tensors: the registration setting */
-static const char REG_MAGIC[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '2'};
+/* optional trailer after the tensors: the registration setting (radius, yaws,
+ * refine_yaw, z_search and level as int32, then yaw_step; files with the earlier
+ * trailer, without level, load with level 0) */
+static const char REG_MAGIC[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '3'};
+static const char REG_MAGIC_02[8] = {'N', 'D', 'T', 'R', 'E', 'G', '0', '2'};
 
 extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
     if (!md || !path) return fail(NDT_ERR_INVALID, "null argument");
@@ -498,8 +504,8 @@ extern "C" int ndt_model_save(const ndt_model* md, const char* path) {
         ok = std::fwrite(h.data(), 4, h.size(), f) == h.size();
     }
     if (ok && reg_on(md->reg)) {
-        const int32_t ri[4] = {md->reg.radius, md->reg.yaws, md->reg.refine_yaw, md->reg.z_search};
-        ok = std::fwrite(REG_MAGIC, 1, 8, f) == 8 && std::fwrite(ri, 4, 4, f) == 4 &&
+        const int32_t ri[5] = {md->reg.radius, md->reg.yaws, md->reg.refine_yaw, md->reg.z_search, md->reg.level};
+        ok = std::fwrite(REG_MAGIC, 1, 8, f) == 8 && std::fwrite(ri, 4, 5, f) == 5 &&
              std::fwrite(&md->reg.yaw_step, 8, 1, f) == 1;
     }
     ok = (std::fclose(f) == 0) && ok;
@@ -531,15 +537,17 @@ extern "C" int ndt_model_load(ndt_ctx* c, const char* path, ndt_model** out) {
     }
     char tag[8];
     if (!e && std::fread(tag, 1, 8, f) == 8) {
-        int32_t ri[4];
+        int32_t ri[5] = {0, 0, 0, 0, 0};
         ndt_register_params rp{};
-        if (std::memcmp(tag, REG_MAGIC, 8) || std::fread(ri, 4, 4, f) != 4 || std::fread(&rp.yaw_step, 8, 1, f) != 1)
+        const int nint = !std::memcmp(tag, REG_MAGIC, 8) ? 5 : !std::memcmp(tag, REG_MAGIC_02, 8) ? 4 : 0;
+        if (!nint || std::fread(ri, 4, (size_t)nint, f) != (size_t)nint || std::fread(&rp.yaw_step, 8, 1, f) != 1)
             e = fail(NDT_ERR_INVALID, "bad registration trailer in model file");
         if (!e) {
             rp.radius = ri[0];
             rp.yaws = ri[1];
             rp.refine_yaw = ri[2];
             rp.z_search = ri[3];
+            rp.level = ri[4];
             if (!(e = reg_check(&rp))) md->reg = rp;
         }
     }

@@ -565,6 +565,184 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
     if (score_out) score_out[b] = (int32_t)v;
 }
 
+/* ------------------------------------------------------------------ leveling
+ * One 1024-thread workgroup per registered guess, one thread per 1 m cell of
+ * [-16, 16)^2 (cell (j, i) = (floor(y + 16), floor(x + 16)), index j * 32 + i).
+ * The lowest scan point (sensor frame) and the lowest map point in the guess frame
+ * (q = R^T (p - t), as the registration computes it) of each cell, heights in
+ * [-8, 8) only, are kept in LDS as order-preserving 64-bit keys (atomicMin).  Where
+ * a cell has both, d = map - scan.  A bitonic sort gives the lower median of d;
+ * the cells within 0.25 m of it enter the least-squares plane d = a x + b y + c
+ * (cell centres x = i - 15.5, y = j - 15.5): nine sums, each by a butterfly over
+ * a wave's 64 cells and then over the 16 waves in order, and Cramer's rule on one
+ * lane.  A rotation of the guess by Rx(roll) Ry(pitch) lifts the map in its frame
+ * by about pitch x - roll y, so the guess turns by (Rx(-b) Ry(a))^-1, both angles'
+ * cosines and sines from the series of the registration's yaw refinement.  The
+ * guess stays as it is with fewer than 32 cells in common or kept, a singular
+ * system, or an angle beyond 0.035 rad. */
+
+constexpr int LEV_N = 32, LEV_CELLS = LEV_N * LEV_N, LEV_MIN_CELLS = 32, LEV_SUMS = 9;
+constexpr double LEV_R = 16.0, LEV_Z0 = -8.0, LEV_Z1 = 8.0, LEV_TRIM = 0.25, LEV_MAX_ANGLE = 0.035;
+constexpr unsigned long long LEV_EMPTY = ~0ull;
+static_assert(LEV_CELLS == REG_BLOCK, "one cell per thread");
+
+/* order-preserving key of a double: a < b <=> key(a) < key(b) (no NaN) */
+__device__ inline unsigned long long lev_key(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double lev_value(unsigned long long k) {
+    const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    return __longlong_as_double((long long)u);
+}
+/* cell of a guess-frame point, -1 outside the window or its height range (NaN too) */
+__device__ inline int lev_cell(double x, double y, double z) {
+    const double fi = floor(x + LEV_R), fj = floor(y + LEV_R);
+    if (!(fi >= 0.0 && fi < (double)LEV_N && fj >= 0.0 && fj < (double)LEV_N && z >= LEV_Z0 && z < LEV_Z1))
+        return -1;
+    return (int)fj * LEV_N + (int)fi;
+}
+
+__global__ __launch_bounds__(REG_BLOCK) void k_bev_level(TileGridDev tg, const float4* __restrict__ spts,
+                                                         const int64_t* __restrict__ soff,
+                                                         const int32_t* __restrict__ scnt,
+                                                         const int32_t* __restrict__ idx, double* gl, double* gw) {
+    __shared__ unsigned long long zs[LEV_CELLS], zm[LEV_CELLS];
+    __shared__ double dv[LEV_CELLS];
+    __shared__ double wsum[REG_BLOCK / WAVE][LEV_SUMS];
+    __shared__ double s_g[12];
+    __shared__ int s_rng[4], s_n, s_keep;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
+    if (tid < 12) s_g[tid] = gl[16 * (int64_t)b + tid];
+    zs[tid] = LEV_EMPTY;
+    zm[tid] = LEV_EMPTY;
+    if (tid == 0) {
+        s_n = 0;
+        s_keep = 0;
+    }
+    __syncthreads();
+    const double r00 = s_g[0], r01 = s_g[1], r02 = s_g[2], t0 = s_g[3];
+    const double r10 = s_g[4], r11 = s_g[5], r12 = s_g[6], t1 = s_g[7];
+    const double r20 = s_g[8], r21 = s_g[9], r22 = s_g[10], t2 = s_g[11];
+    if (tid == 0) {
+        /* tiles under the window's box [-16, 16]^2 x [-8, 8] in the map frame */
+        double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+        for (int k = 0; k < 8; k++) {
+            const double qx = (k & 1) ? LEV_R : -LEV_R;
+            const double qy = (k & 2) ? LEV_R : -LEV_R;
+            const double qz = (k & 4) ? LEV_Z1 : LEV_Z0;
+            const double px = r00 * qx + r01 * qy + r02 * qz + t0;
+            const double py = r10 * qx + r11 * qy + r12 * qz + t1;
+            xmin = fmin(xmin, px);
+            xmax = fmax(xmax, px);
+            ymin = fmin(ymin, py);
+            ymax = fmax(ymax, py);
+        }
+        double fx0 = floor((xmin - 0.5 - tg.x0) / MAP_TILE), fx1 = floor((xmax + 0.5 - tg.x0) / MAP_TILE);
+        double fy0 = floor((ymin - 0.5 - tg.y0) / MAP_TILE), fy1 = floor((ymax + 0.5 - tg.y0) / MAP_TILE);
+        int tx0 = 0, tx1 = -1, ty0 = 0, ty1 = -1;
+        if (fx1 >= 0.0 && fx0 < (double)tg.ntx && fy1 >= 0.0 && fy0 < (double)tg.nty) {
+            tx0 = (int)fmax(fx0, 0.0);
+            tx1 = (int)fmin(fx1, (double)(tg.ntx - 1));
+            ty0 = (int)fmax(fy0, 0.0);
+            ty1 = (int)fmin(fy1, (double)(tg.nty - 1));
+        }
+        s_rng[0] = tx0;
+        s_rng[1] = tx1;
+        s_rng[2] = ty0;
+        s_rng[3] = ty1;
+    }
+    {
+        const int s = idx[b];
+        const float4* sp = spts + soff[s];
+        const int ns = scnt[s];
+        for (int i = tid; i < ns; i += REG_BLOCK) {
+            const float4 p = sp[i];
+            const int c = lev_cell((double)p.x, (double)p.y, (double)p.z);
+            if (c >= 0) atomicMin(&zs[c], lev_key((double)p.z));
+        }
+    }
+    __syncthreads();
+    const int tx0 = s_rng[0], tx1 = s_rng[1], ty0 = s_rng[2], ty1 = s_rng[3];
+    for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
+        const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
+        const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
+        for (uint32_t i = pb + tid; i < pe; i += REG_BLOCK) {
+            const float4 p = tg.pts[i];
+            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
+            const double qx = r00 * dx + r10 * dy + r20 * dz;
+            const double qy = r01 * dx + r11 * dy + r21 * dz;
+            const double qz = r02 * dx + r12 * dy + r22 * dz;
+            const int c = lev_cell(qx, qy, qz);
+            if (c >= 0) atomicMin(&zm[c], lev_key(qz));
+        }
+    }
+    __syncthreads();
+    const bool both = zs[tid] != LEV_EMPTY && zm[tid] != LEV_EMPTY;
+    const double d = both ? lev_value(zm[tid]) - lev_value(zs[tid]) : INFINITY;
+    dv[tid] = d;
+    if (both) atomicAdd(&s_n, 1);
+    __syncthreads();
+    const int n = s_n;
+    if (n < LEV_MIN_CELLS) return; /* uniform over the workgroup */
+    /* ascending bitonic sort of the 1024 differences (+inf where a cell lacks one) */
+    for (int k = 2; k <= LEV_CELLS; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int l = tid ^ j;
+            if (l > tid) {
+                const double a = dv[tid], c = dv[l];
+                if (((tid & k) == 0) ? (c < a) : (a < c)) {
+                    dv[tid] = c;
+                    dv[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const double med = dv[(n - 1) / 2];
+    const bool keep = both && fabs(d - med) < LEV_TRIM;
+    if (keep) atomicAdd(&s_keep, 1);
+    const double x = (double)(tid % LEV_N) - 15.5, y = (double)(tid / LEV_N) - 15.5;
+    const double one = keep ? 1.0 : 0.0, dd = keep ? d : 0.0;
+    double t[LEV_SUMS] = {one, one * x, one * y, one * x * x, one * x * y, one * y * y, dd, dd * x, dd * y};
+#pragma unroll
+    for (int q = 0; q < LEV_SUMS; q++) {
+        for (int off = WAVE / 2; off > 0; off >>= 1) t[q] = t[q] + __shfl_xor(t[q], off);
+        if (lane == 0) wsum[wave][q] = t[q];
+    }
+    __syncthreads();
+    if (tid != 0 || s_keep < LEV_MIN_CELLS) return;
+    double S[LEV_SUMS];
+    for (int q = 0; q < LEV_SUMS; q++) {
+        S[q] = wsum[0][q];
+        for (int w = 1; w < REG_BLOCK / WAVE; w++) S[q] = S[q] + wsum[w][q];
+    }
+    const double sn = S[0], sx = S[1], sy = S[2], sxx = S[3], sxy = S[4], syy = S[5], sd = S[6], sxd = S[7],
+                 syd = S[8];
+    const double m0 = syy * sn - sy * sy;
+    const double m1 = sxy * sn - sy * sx;
+    const double m2 = sxy * sy - syy * sx;
+    const double det = sxx * m0 - sxy * m1 + sx * m2;
+    if (!(det > 0.0)) return;
+    const double a = (sxd * m0 - sxy * (syd * sn - sy * sd) + sx * (syd * sy - syy * sd)) / det;
+    const double bb = (sxx * (syd * sn - sy * sd) - sxd * m1 + sx * (sxy * sd - syd * sx)) / det;
+    const double pitch = a, roll = -bb;
+    if (!(fabs(pitch) <= LEV_MAX_ANGLE && fabs(roll) <= LEV_MAX_ANGLE)) return;
+    double ct, st, cf, sf;
+    reg_small_cos_sin(pitch, ct, st);
+    reg_small_cos_sin(roll, cf, sf);
+    /* M = Ry(-pitch) Rx(-roll) */
+    const double M[3][3] = {{ct, st * sf, -(st * cf)}, {0.0, cf, sf}, {st, -(ct * sf), ct * cf}};
+    double* outs[2] = {gl + 16 * (int64_t)b, gw + 16 * (int64_t)b};
+    for (int q = 0; q < 2; q++) {
+        double* g = outs[q];
+        for (int r = 0; r < 3; r++) {
+            const double a0 = g[4 * r], a1 = g[4 * r + 1], a2 = g[4 * r + 2];
+            for (int k = 0; k < 3; k++) g[4 * r + k] = a0 * M[0][k] + a1 * M[1][k] + a2 * M[2][k];
+        }
+    }
+}
+
 int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const float4* pts, const int64_t* offsets,
                         const int32_t* counts, const int32_t* idx, const double* gl, const double* gw, int32_t B,
                         const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score, hipStream_t stream) {
@@ -579,6 +757,9 @@ int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const float4* pts,
         REG_LAUNCH(1) REG_LAUNCH(2) REG_LAUNCH(3) REG_LAUNCH(4) REG_LAUNCH(5) REG_LAUNCH(6) REG_LAUNCH(7) REG_LAUNCH(8)
     }
 #undef REG_LAUNCH
+    if (rp.level)
+        hipLaunchKernelGGL(k_bev_level, dim3(B), dim3(REG_BLOCK), 0, stream, tg, pts, offsets, counts, idx, gl_out,
+                           gw_out);
     return (int)hipGetLastError();
 }
 

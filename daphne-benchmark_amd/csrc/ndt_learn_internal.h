@@ -70,21 +70,22 @@ int launch_to_bf16(const float* x, int64_t n, uint16_t* out, hipStream_t stream)
 
 /* BEV registration of a guess (ndt_register_params): yaw k of the search is
  * (k - (yaws - 1) / 2) * yaw_step, its cosine, sine and tangent computed on the
- * host */
+ * host; `level` adds the leveling kernel after it */
 constexpr int REG_MAX_RADIUS = 8, REG_MAX_YAWS = 11;
 constexpr int REG_BLOCK = 1024;
 struct RegisterDev {
-    int32_t radius, yaws, refine_yaw, z_search;
+    int32_t radius, yaws, refine_yaw, z_search, level, pad;
     double yaw_step;
     double c[REG_MAX_YAWS], s[REG_MAX_YAWS], t[REG_MAX_YAWS];
 };
 /* Registers row b (B rows): scan grid sbev[idx[b]] (bf16 bits, 0 = no point),
- * and with z_search the heights of the scan's points (pts from offsets[idx[b]],
+ * and with z_search or level the scan's points (pts from offsets[idx[b]],
  * counts[idx[b]] of them), against the map around the map-local guess gl[b];
  * writes the registered
  * map-local guess to gl_out[b], the registered world guess (from gw[b]) to
- * gw_out[b] and, when `score` is not null, the best score.  gl_out may be gl and
- * gw_out may be gw (each row is read before it is written). */
+ * gw_out[b] and, when `score` is not null, the best score; with level, a second
+ * kernel then levels both against the map around gl_out[b].  gl_out may be gl
+ * and gw_out may be gw (each row is read before it is written). */
 int launch_bev_register(TileGridDev tg, const uint16_t* sbev, const float4* pts, const int64_t* offsets,
                         const int32_t* counts, const int32_t* idx, const double* gl, const double* gw, int32_t B,
                         const RegisterDev& rp, double* gl_out, double* gw_out, int32_t* score, hipStream_t stream);

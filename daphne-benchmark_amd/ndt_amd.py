@@ -412,7 +412,7 @@ class TrainParams(C.Structure):
 
 class RegisterParams(C.Structure):
     _fields_ = [("radius", C.c_int32), ("yaws", C.c_int32), ("yaw_step", C.c_double), ("refine_yaw", C.c_int32),
-                ("z_search", C.c_int32)]
+                ("z_search", C.c_int32), ("level", C.c_int32), ("pad", C.c_int32)]
 
 
 class NdtError(RuntimeError):

@@ -92,19 +92,27 @@ enum { NDT_LEARNED_RACE = 0, NDT_LEARNED_FALLBACK = 1, NDT_LEARNED_KEEP_ROTATION
  * scan's in its sensor frame
  * and the map's under the grid's footprint in the guess frame, correlated over
  * shifts of -1.5 .. 1.5 m; the best shift, refined by a parabola, moves the
- * registered guess along its z axis.  The model then sees the map around the
- * registered guess and corrects it. */
+ * registered guess along its z axis.  With level the roll and pitch follow: per
+ * 1 m cell of [-16, 16)^2 m the lowest scan point and the lowest map point in the
+ * registered guess's frame (heights in [-8, 8) m), their differences within
+ * 0.25 m of the median fitted by a plane, whose slopes give the guess's tilt
+ * against the scan (at most 0.035 rad each way; at least 32 cells; otherwise the
+ * guess keeps its rotation).  The model then sees the map around the registered
+ * guess and corrects it. */
 typedef struct ndt_register_params {
     int32_t radius;     /* cells (0.5 m) searched each way on x and y, 0..8; 0 = off */
     int32_t yaws;       /* yaw angles searched: odd, 1..11 */
     double yaw_step;    /* radians between them, 0 < yaw_step <= 0.05 */
     int32_t refine_yaw; /* 1: the yaw refined by a parabola through its neighbours */
     int32_t z_search;   /* 1: the height registered too */
+    int32_t level;      /* 1: roll and pitch leveled against the map after the search */
+    int32_t pad;        /* 0 */
 } ndt_register_params;
 
 enum { NDT_REGISTER_MAX_RADIUS = 8, NDT_REGISTER_MAX_YAWS = 11 };
 
-/* radius 6 (+-3 m), 7 yaws 1 degree apart (+-3 degrees), refine_yaw 1, z_search 1 */
+/* radius 6 (+-3 m), 7 yaws 1 degree apart (+-3 degrees), refine_yaw 1, z_search 1,
+ * level 1 */
 void ndt_register_params_default(ndt_register_params* rp);
 
 /* Buckets the map points (map-local coordinates, m x 4 floats) into 4 m x 4 m

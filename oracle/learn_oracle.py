@@ -193,8 +193,9 @@ def adam_step(w, m, v, g, t, lr, beta1=0.9, beta2=0.999, eps=1e-8, wd=0.0):
 # ---------------------------------------------------------------- registration
 
 def register_params_default() -> dict:
-    """radius 6 cells (+-3 m), 7 yaws 1 degree apart, yaw refined, height searched"""
-    return dict(radius=6, yaws=7, yaw_step=3.14159265358979323846 / 180.0, refine_yaw=1, z_search=1)
+    """radius 6 cells (+-3 m), 7 yaws 1 degree apart, yaw refined, height searched,
+    roll and pitch leveled"""
+    return dict(radius=6, yaws=7, yaw_step=3.14159265358979323846 / 180.0, refine_yaw=1, z_search=1, level=1)
 
 
 ZH_BINS, ZH_RANGE, ZH_Z0, ZH_BIN = 320, 30, -4.0, 0.05
@@ -354,7 +355,8 @@ def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tup
     through its neighbours' scores (half a cell at most), delta likewise between
     the neighbouring yaws when refine_yaw; unchanged when the best score is 0.
     With z_search the result then moves along its z axis by the best height shift
-    (z_scores of the scan's heights and the map's under the footprint of G)."""
+    (z_scores of the scan's heights and the map's under the footprint of G); with
+    level it is then leveled (`level`) against the map around it."""
     GL = np.asarray(guess_local, np.float64).reshape(4, 4).copy()
     GW = np.asarray(guess_world, np.float64).reshape(4, 4).copy()
     if rp["radius"] == 0:
@@ -379,6 +381,8 @@ def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tup
     v, k, dy, dx = register_pick(sc, rp)
     if v == 0:
         gl, gw = lift([GL, GW])
+        if rp.get("level", 0):
+            gl, gw = level(scan_xyzw, map_xyzw, gl, gw)[:2]
         return gl, gw, 0
     R, K = rp["radius"], len(cs[0])
     S = sc[k].astype(np.float64)
@@ -404,4 +408,122 @@ def register(scan_xyzw, map_xyzw, guess_local, guess_world, rp, lut=None) -> tup
             g[4 * r + 1] = bb * c - a * s
         out.append(np.array(g).reshape(4, 4))
     out = lift(out)
+    if rp.get("level", 0):
+        out = level(scan_xyzw, map_xyzw, out[0], out[1])[:2]
     return out[0], out[1], int(v)
+
+
+# ---------------------------------------------------------------- leveling
+LEV_N, LEV_R, LEV_Z0, LEV_Z1 = 32, 16.0, -8.0, 8.0
+LEV_TRIM, LEV_MIN_CELLS, LEV_MAX_ANGLE = 0.25, 32, 0.035
+
+
+def level_cells(x, y, z) -> np.ndarray:
+    """(1024,) float64 lowest height per 1 m cell of [-16, 16)^2 (cell (j, i) =
+    (floor(y + 16), floor(x + 16)), index j * 32 + i), heights in [-8, 8) only;
+    +inf where the cell holds no point"""
+    x, y, z = (np.asarray(v, np.float64) for v in (x, y, z))
+    with np.errstate(invalid="ignore"):
+        fi, fj = np.floor(x + LEV_R), np.floor(y + LEV_R)
+        ok = (fi >= 0) & (fi < LEV_N) & (fj >= 0) & (fj < LEV_N) & (z >= LEV_Z0) & (z < LEV_Z1)
+    out = np.full(LEV_N * LEV_N, np.inf)
+    np.minimum.at(out, fj[ok].astype(np.int64) * LEV_N + fi[ok].astype(np.int64), z[ok])
+    return out
+
+
+def _wave_tree(v) -> float:
+    """sum of 64 values as a 64-lane butterfly leaves it in lane 0"""
+    a = np.asarray(v, np.float64)
+    while a.shape[0] > 1:
+        h = a.shape[0] // 2
+        a = a[:h] + a[h:]
+    return float(a[0])
+
+
+def level_sums(d, keep) -> list:
+    """the nine sums n, x, y, xx, xy, yy, d, xd, yd over the kept cells (cell
+    centres x = i - 15.5, y = j - 15.5): per 64-cell block a butterfly tree, the 16
+    block sums then added in block order"""
+    i = np.arange(LEV_N * LEV_N)
+    x = (i % LEV_N).astype(np.float64) - 15.5
+    y = (i // LEV_N).astype(np.float64) - 15.5
+    k = np.asarray(keep, bool)
+    dd = np.where(k, np.asarray(d, np.float64), 0.0)
+    one = k.astype(np.float64)
+    terms = [one, one * x, one * y, one * x * x, one * x * y, one * y * y, dd, dd * x, dd * y]
+    out = []
+    for t in terms:
+        s = 0.0
+        for w in range(LEV_N * LEV_N // 64):
+            s = s + _wave_tree(t[64 * w:64 * w + 64]) if w else _wave_tree(t[:64])
+        out.append(s)
+    return out
+
+
+def level_solve(S) -> tuple:
+    """slopes (a, b) of the least-squares plane d = a x + b y + c from the sums, by
+    Cramer's rule; None when the system is singular"""
+    n, sx, sy, sxx, sxy, syy, sd, sxd, syd = S
+    m0 = syy * n - sy * sy
+    m1 = sxy * n - sy * sx
+    m2 = sxy * sy - syy * sx
+    det = sxx * m0 - sxy * m1 + sx * m2
+    if not det > 0.0:
+        return None
+    a = (sxd * m0 - sxy * (syd * n - sy * sd) + sx * (syd * sy - syy * sd)) / det
+    b = (sxx * (syd * n - sy * sd) - sxd * m1 + sx * (sxy * sd - syd * sx)) / det
+    return a, b
+
+
+def level(scan_xyzw, map_xyzw, guess_local, guess_world) -> tuple:
+    """Leveled (map-local guess, world guess, kept cells, (pitch, roll)) of a
+    registered guess.  Per 1 m cell of [-16, 16)^2 the lowest scan point (sensor
+    frame) and the lowest map point in the guess frame (q = R^T (p - t), heights in
+    [-8, 8)); d = map - scan where both exist.  With the cells within 0.25 m of the
+    lower median of d (at least 32 cells, and 32 kept), the plane d = a x + b y + c
+    gives the tilt of the guess against the scan: pitch a, roll -b (a rotation of
+    the guess by Rx(roll) Ry(pitch) lifts the map in its frame by about
+    pitch x - roll y).  The guess turns by (Rx(roll) Ry(pitch))^-1, each angle's
+    cosine and sine from the series the registration uses, unless an angle exceeds
+    0.035 rad; otherwise it is returned unchanged (pitch and roll then None)."""
+    GL = np.asarray(guess_local, np.float64).reshape(4, 4).copy()
+    GW = np.asarray(guess_world, np.float64).reshape(4, 4).copy()
+    s = np.asarray(scan_xyzw, np.float32)
+    zs = level_cells(s[:, 0], s[:, 1], s[:, 2])
+    m = np.asarray(map_xyzw, np.float32)
+    dx = m[:, 0].astype(np.float64) - GL[0, 3]
+    dy = m[:, 1].astype(np.float64) - GL[1, 3]
+    dz = m[:, 2].astype(np.float64) - GL[2, 3]
+    qx = GL[0, 0] * dx + GL[1, 0] * dy + GL[2, 0] * dz
+    qy = GL[0, 1] * dx + GL[1, 1] * dy + GL[2, 1] * dz
+    qz = GL[0, 2] * dx + GL[1, 2] * dy + GL[2, 2] * dz
+    zm = level_cells(qx, qy, qz)
+    both = np.isfinite(zs) & np.isfinite(zm)
+    n = int(both.sum())
+    if n < LEV_MIN_CELLS:
+        return GL, GW, 0, None
+    with np.errstate(invalid="ignore"):
+        d = np.where(both, zm - zs, np.inf)
+    med = np.sort(d)[(n - 1) // 2]
+    keep = both & (np.abs(d - med) < LEV_TRIM)
+    if int(keep.sum()) < LEV_MIN_CELLS:
+        return GL, GW, int(keep.sum()), None
+    ab = level_solve(level_sums(np.where(keep, d, 0.0), keep))
+    if ab is None:
+        return GL, GW, int(keep.sum()), None
+    pitch, roll = ab[0], -ab[1]
+    if not (abs(pitch) <= LEV_MAX_ANGLE and abs(roll) <= LEV_MAX_ANGLE):
+        return GL, GW, int(keep.sum()), None
+    ct, st = _small_cos_sin(pitch)
+    cf, sf = _small_cos_sin(roll)
+    # M = Ry(-pitch) Rx(-roll)
+    M = [[ct, st * sf, -(st * cf)], [0.0, cf, sf], [st, -(ct * sf), ct * cf]]
+    out = []
+    for G in (GL, GW):
+        g = [float(v) for v in G.reshape(-1)]
+        for r in range(3):
+            a0, a1, a2 = g[4 * r], g[4 * r + 1], g[4 * r + 2]
+            for k in range(3):
+                g[4 * r + k] = a0 * M[0][k] + a1 * M[1][k] + a2 * M[2][k]
+        out.append(np.array(g).reshape(4, 4))
+    return out[0], out[1], int(keep.sum()), (pitch, roll)

@@ -85,17 +85,19 @@ def test_features_world_origin(A, ctx, small_scene):
 
 
 def _rp_dict(rp):
-    return dict(radius=rp.radius, yaws=rp.yaws, yaw_step=rp.yaw_step, refine_yaw=rp.refine_yaw, z_search=rp.z_search)
+    return dict(radius=rp.radius, yaws=rp.yaws, yaw_step=rp.yaw_step, refine_yaw=rp.refine_yaw, z_search=rp.z_search,
+                level=rp.level)
 
 
 @pytest.mark.parametrize("kw,origin", [(dict(), (0.0, 0.0, 0.0)), (dict(refine_yaw=0, radius=8, yaws=11),
                                                                     (4.2e5, 5.6e6, 80.0)),
-                                       (dict(yaws=1, radius=3, z_search=0), (0.0, 0.0, 0.0))])
+                                       (dict(yaws=1, radius=3, z_search=0, level=0), (0.0, 0.0, 0.0))])
 def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
-    """Registered guesses and best scores bit for bit equal to the oracle's, for
-    guesses off by up to +-1 m / +-3 degrees, a guess beside the map (no cell in
-    common: the guess comes back unchanged with score 0) and a scan above the grid's
-    height range, with a world origin far from the map-local frame."""
+    """Registered (and leveled) guesses and best scores bit for bit equal to the
+    oracle's, for guesses off by up to +-1 m / +-3 degrees and +-0.5 degrees of roll
+    and pitch, a guess beside the map (no cell in common: the guess comes back
+    unchanged with score 0) and a scan above the grid's height range, with a world
+    origin far from the map-local frame."""
     w = small_scene
     high = w.scans[0].copy()
     high[:, 2] += 40.0
@@ -103,7 +105,7 @@ def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
     sc = A.Scans(ctx, scans)
     o = np.array(origin)
     bm = A.BevMap(ctx, w.map_xyzw, o)
-    g = A.perturb_poses(w.gt, np.random.default_rng(4), xy=1.0, z=0.3, yaw_deg=3.0)
+    g = A.perturb_poses(w.gt, np.random.default_rng(4), xy=1.0, z=0.3, yaw_deg=3.0, rollpitch_deg=0.5)
     gl = np.concatenate([g, g[:1], w.gt[:1]])
     gl[len(w.scans), 0, 3] += 500.0
     idx = np.array(list(range(len(w.scans))) + [0, len(w.scans)], np.int32)
@@ -118,9 +120,22 @@ def test_register_matches_oracle(A, ctx, small_scene, kw, origin):
         el, ew, v = L.register(scans[s], w.map_xyzw, gl_b, gw[b], _rp_dict(rp), lut)
         assert score[b] == v and np.array_equal(out[b], ew), (b, score[b], v)
     # no cell in common: the guess stays, apart from the height search (the map
-    # beside it is empty, the scan above the grid still has heights)
+    # beside it is empty, the scan above the grid still has heights, and above the
+    # leveling's height range too)
     assert score[len(w.scans)] == 0 and np.array_equal(out[len(w.scans)], gw[len(w.scans)])
     assert score[-1] == 0 and np.array_equal(out[-1][:, :3], gw[-1][:, :3])
+    if rp.level:
+        # leveling turned the registered guesses towards the truth's roll and pitch
+        # (on these 2 k-point scans from 0.42 to 0.18 degrees: most 1 m cells hold
+        # one or two scan points, so their lowest heights are noisy)
+        def tilt(G, T):
+            E = np.linalg.inv(T) @ G
+            return np.hypot(np.arctan2(E[2, 1], E[2, 2]), np.arcsin(np.clip(-E[2, 0], -1, 1)))
+        gt_w = w.gt.copy()
+        gt_w[:, :3, 3] += o
+        before = np.mean([tilt(gw[b], gt_w[b]) for b in range(len(w.scans))])
+        after = np.mean([tilt(out[b], gt_w[b]) for b in range(len(w.scans))])
+        assert after < 0.5 * before, (before, after)
     assert (score[:len(w.scans)] > 100).all()
     off = bm.register(sc, idx, gw, A.register_params(radius=0))[0]
     assert np.array_equal(off, gw)
@@ -177,10 +192,21 @@ def test_model_save_load(A, ctx, tmp_path):
     m.set_register(A.register_params(radius=5, yaws=7))
     m.save(str(p))
     r3 = A.Model(ctx, path=str(p)).get_register()
-    assert (r3.radius, r3.yaws, r3.yaw_step, r3.refine_yaw, r3.z_search) == (5, 7, A.register_params().yaw_step, 1, 1)
-    m.set_register(A.register_params(z_search=0))
+    assert (r3.radius, r3.yaws, r3.yaw_step, r3.refine_yaw, r3.z_search, r3.level) == \
+        (5, 7, A.register_params().yaw_step, 1, 1, 1)
+    m.set_register(A.register_params(z_search=0, level=0))
     m.save(str(p))
-    assert A.Model(ctx, path=str(p)).get_register().z_search == 0
+    r4 = A.Model(ctx, path=str(p)).get_register()
+    assert (r4.z_search, r4.level) == (0, 0)
+    # a file with the earlier trailer (no level) loads with level 0
+    raw = bytearray(p.read_bytes())
+    k = raw.rindex(b"NDTREG03")
+    old = raw[:k] + b"NDTREG02" + raw[k + 8:k + 8 + 16] + raw[k + 8 + 20:]
+    p.write_bytes(bytes(old))
+    r5 = A.Model(ctx, path=str(p)).get_register()
+    assert (r5.radius, r5.z_search, r5.level, r5.yaw_step) == (6, 0, 0, A.register_params().yaw_step)
+    with pytest.raises(A.NdtError):
+        m.set_register(A.register_params(level=2))
     xb = L.bf16_bits(np.abs(r.normal(size=(128, 256))).astype(np.float32))
     assert np.array_equal(m.forward(xb), m2.forward(xb))
 
@@ -218,7 +244,7 @@ def trained(A, ctx):
         guesses = np.where(wide[:, None, None], A.perturb_poses(tr_gt[idx], rng),
                            A.perturb_poses(tr_gt[idx], rng, xy=1.0, z=1.0, yaw_deg=3.0))
         # registered as bench.py does: a coarse search (+-4 m, +-10 deg), then rp
-        coarse = A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)))
+        coarse = A.register_params(radius=8, yaws=11, yaw_step=float(np.radians(2.0)), level=0)
         reg, _ = bm.register(trS, idx, bm.register(trS, idx, guesses, coarse)[0], rp)
         if ds is None:
             ds = A.Dataset(ctx, bm, trS, idx, guesses, labels[idx])

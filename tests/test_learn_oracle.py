@@ -3,7 +3,9 @@ rounding against torch, grids against a per-axis linear search and a frame
 invariance, gradients against finite differences, the pose correction against its
 round trip; the registration's scores against a cell-by-cell count, its choice
 among ties, its parabola and small-angle series against closed forms and libm,
-and its result against planted shifts and yaws."""
+and its result against planted shifts and yaws; the leveling's cells against a
+linear search, its plane against least squares, and its result against planted
+tilts of level and of tilted true poses."""
 import numpy as np
 import pytest
 
@@ -215,7 +217,7 @@ def test_register_recovers_planted_offsets(A, shift, steps):
     p = L.correction_params(gl, T)
     assert np.hypot(p[0], p[1]) < 0.03 and abs(p[2]) < 0.01, p
     assert abs(p[5]) < np.radians(0.05), p
-    gl2, _, _ = L.register(scan, mp, G, G, dict(rp, z_search=0))
+    gl2, _, _ = L.register(scan, mp, G, G, dict(rp, z_search=0, level=0))
     assert abs(L.correction_params(gl2, T)[2] - shift[2]) < 1e-9
 
 
@@ -243,3 +245,68 @@ def test_register_without_common_cells_keeps_the_guess(A):
     high[:, 2] += 40.0
     gl, _, v = L.register(high, mp, T, T, rp)
     assert v == 0 and np.array_equal(gl, T)
+
+
+def test_level_cells_against_linear_search():
+    rng = np.random.default_rng(2)
+    x, y, z = rng.uniform(-18, 18, 3000), rng.uniform(-18, 18, 3000), rng.uniform(-10, 10, 3000)
+    x[:5] = [np.nan, 15.999999, -16.0, 16.0, 0.0]
+    z[5], z[6], z[7] = 8.0, -8.0, np.nan
+    out = L.level_cells(x, y, z)
+    ref = np.full(L.LEV_N * L.LEV_N, np.inf)
+    for xi, yi, zi in zip(x, y, z):
+        i, j = _linear_search_cell(xi, -16.0, 1.0, 32), _linear_search_cell(yi, -16.0, 1.0, 32)
+        if i >= 0 and j >= 0 and -8.0 <= zi < 8.0:
+            ref[j * 32 + i] = min(ref[j * 32 + i], zi)
+    assert np.array_equal(out, ref)
+
+
+def test_level_plane_against_least_squares():
+    """The tree sums and Cramer's rule give numpy's least-squares slopes; cells on
+    one row leave the system singular (no tilt)."""
+    rng = np.random.default_rng(3)
+    i = np.arange(L.LEV_N * L.LEV_N)
+    x, y = (i % 32) - 15.5, (i // 32) - 15.5
+    for _ in range(6):
+        keep = rng.uniform(size=i.size) < rng.uniform(0.05, 0.95)
+        d = rng.normal(0, 0.02) * x + rng.normal(0, 0.02) * y + rng.normal(0, 0.5) + rng.normal(0, 0.01, i.size)
+        a, b = L.level_solve(L.level_sums(np.where(keep, d, 0.0), keep))
+        ref = np.linalg.lstsq(np.column_stack([x[keep], y[keep], np.ones(keep.sum())]), d[keep], rcond=None)[0]
+        assert abs(a - ref[0]) < 1e-12 and abs(b - ref[1]) < 1e-12
+    row = i < 32
+    assert L.level_solve(L.level_sums(np.where(row, 0.1 * x, 0.0), row)) is None
+
+
+def _tilt(G, T):
+    """roll and pitch of T^-1 G"""
+    E = np.linalg.inv(T) @ G
+    return np.arctan2(E[2, 1], E[2, 2]), np.arcsin(np.clip(-E[2, 0], -1.0, 1.0))
+
+
+@pytest.mark.parametrize("truth,err", [((0.0, 0.0), (0.4, -0.3)), ((1.5, -1.0), (-0.45, 0.2)),
+                                       ((-2.0, 0.5), (0.3, 0.45)), ((0.7, 2.2), (0.0, -0.4))])
+def test_level_recovers_planted_tilt(A, truth, err):
+    """A scan made of the map's own points seen from a true pose T with the given
+    roll and pitch (degrees; the scene's ground is level, so a tilted T sees it
+    tilted), and the guess T turned by a further roll and pitch, 20 cm off on x:
+    leveling returns T's roll and pitch within 0.02 degrees (the error planted is
+    0.3-0.45), whatever T's own tilt, and leaves the translation alone; a tilt
+    of the wrong sign would double the error."""
+    mp = A.make_map(A.scene_cfg(4, extent=80.0), 80000)
+    T = L.pose_matrix([3.0, -5.0, 1.6, np.radians(truth[0]), np.radians(truth[1]), 0.4])
+    p = np.column_stack([mp[:, :3].astype(np.float64), np.ones(len(mp))]) @ np.linalg.inv(T).T
+    scan = np.column_stack([p[:, :3], np.ones(len(p))]).astype(np.float32)
+    G = T @ L.pose_matrix([0.2, 0.0, 0.0, np.radians(err[0]), np.radians(err[1]), 0.0])
+    gl, gw, kept, pr = L.level(scan, mp, G, G)
+    assert np.array_equal(gl, gw) and kept > 300 and pr is not None
+    r, q = _tilt(gl, T)
+    assert abs(r) < np.radians(0.02) and abs(q) < np.radians(0.02), np.degrees([r, q])
+    assert np.array_equal(gl[:3, 3], G[:3, 3])
+
+
+def test_level_keeps_a_guess_without_common_cells(A):
+    mp, T, scan = _scene(A)
+    far = T.copy()
+    far[0, 3] += 500.0
+    gl, gw, kept, pr = L.level(scan, mp, far, far)
+    assert pr is None and kept == 0 and np.array_equal(gl, far) and np.array_equal(gw, far)
