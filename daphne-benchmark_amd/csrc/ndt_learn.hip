@@ -570,7 +570,9 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
  * [-16, 16)^2 (cell (j, i) = (floor(y + 16), floor(x + 16)), index j * 32 + i).
  * The lowest scan point (sensor frame) and the lowest map point in the guess frame
  * (q = R^T (p - t), as the registration computes it) of each cell, heights in
- * [-8, 8) only, are kept in LDS as order-preserving 64-bit keys (atomicMin).  Where
+ * [-8, 8) only, are kept in LDS as order-preserving 64-bit keys (atomicMin, issued
+ * only by a point below the cell's current lowest: most points of the dense cells
+ * near the sensor then skip the contended atomic).  Where
  * a cell has both, d = map - scan.  A bitonic sort gives the lower median of d;
  * the cells within 0.25 m of it enter the least-squares plane d = a x + b y + c
  * (cell centres x = i - 15.5, y = j - 15.5): nine sums, each by a butterfly over
@@ -581,7 +583,7 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_register(TileGridDev tg, cons
  * guess stays as it is with fewer than 32 cells in common or kept, a singular
  * system, or an angle beyond 0.035 rad. */
 
-constexpr int LEV_N = 32, LEV_CELLS = LEV_N * LEV_N, LEV_MIN_CELLS = 32, LEV_SUMS = 9;
+constexpr int LEV_N = 32, LEV_CELLS = LEV_N * LEV_N, LEV_MIN_CELLS = 32, LEV_SUMS = 9, LEV_UNROLL = 4;
 constexpr double LEV_R = 16.0, LEV_Z0 = -8.0, LEV_Z1 = 8.0, LEV_TRIM = 0.25, LEV_MAX_ANGLE = 0.035;
 constexpr unsigned long long LEV_EMPTY = ~0ull;
 static_assert(LEV_CELLS == REG_BLOCK, "one cell per thread");
@@ -608,7 +610,7 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_level(TileGridDev tg, const f
                                                          const int32_t* __restrict__ scnt,
                                                          const int32_t* __restrict__ idx, double* gl, double* gw) {
     __shared__ unsigned long long zs[LEV_CELLS], zm[LEV_CELLS];
-    __shared__ double dv[LEV_CELLS];
+    __shared__ double dv[2][LEV_CELLS];
     __shared__ double wsum[REG_BLOCK / WAVE][LEV_SUMS];
     __shared__ double s_g[12];
     __shared__ int s_rng[4], s_n, s_keep;
@@ -653,13 +655,23 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_level(TileGridDev tg, const f
         s_rng[3] = ty1;
     }
     {
+        /* LEV_UNROLL points in flight per thread */
         const int s = idx[b];
         const float4* sp = spts + soff[s];
         const int ns = scnt[s];
-        for (int i = tid; i < ns; i += REG_BLOCK) {
-            const float4 p = sp[i];
-            const int c = lev_cell((double)p.x, (double)p.y, (double)p.z);
-            if (c >= 0) atomicMin(&zs[c], lev_key((double)p.z));
+        for (int i0 = tid; i0 < ns; i0 += LEV_UNROLL * REG_BLOCK) {
+            float4 p[LEV_UNROLL];
+#pragma unroll
+            for (int u = 0; u < LEV_UNROLL; u++)
+                if (i0 + u * REG_BLOCK < ns) p[u] = sp[i0 + u * REG_BLOCK];
+#pragma unroll
+            for (int u = 0; u < LEV_UNROLL; u++) {
+                if (i0 + u * REG_BLOCK >= ns) break;
+                const int c = lev_cell((double)p[u].x, (double)p[u].y, (double)p[u].z);
+                if (c < 0) continue;
+                const unsigned long long key = lev_key((double)p[u].z);
+                if (key < zs[c]) atomicMin(&zs[c], key);
+            }
         }
     }
     __syncthreads();
@@ -667,39 +679,55 @@ __global__ __launch_bounds__(REG_BLOCK) void k_bev_level(TileGridDev tg, const f
     for (int ty = ty0; ty <= ty1 && tx0 <= tx1; ty++) {
         const uint32_t pb = tg.start[(int64_t)ty * tg.ntx + tx0];
         const uint32_t pe = tg.start[(int64_t)ty * tg.ntx + tx1 + 1];
-        for (uint32_t i = pb + tid; i < pe; i += REG_BLOCK) {
-            const float4 p = tg.pts[i];
-            const double dx = (double)p.x - t0, dy = (double)p.y - t1, dz = (double)p.z - t2;
-            const double qx = r00 * dx + r10 * dy + r20 * dz;
-            const double qy = r01 * dx + r11 * dy + r21 * dz;
-            const double qz = r02 * dx + r12 * dy + r22 * dz;
-            const int c = lev_cell(qx, qy, qz);
-            if (c >= 0) atomicMin(&zm[c], lev_key(qz));
+        for (uint32_t i0 = pb + tid; i0 < pe; i0 += LEV_UNROLL * REG_BLOCK) {
+            float4 p[LEV_UNROLL];
+#pragma unroll
+            for (int u = 0; u < LEV_UNROLL; u++)
+                if (i0 + u * REG_BLOCK < pe) p[u] = tg.pts[i0 + u * REG_BLOCK];
+#pragma unroll
+            for (int u = 0; u < LEV_UNROLL; u++) {
+                if (i0 + u * REG_BLOCK >= pe) break;
+                const double dx = (double)p[u].x - t0, dy = (double)p[u].y - t1, dz = (double)p[u].z - t2;
+                const double qx = r00 * dx + r10 * dy + r20 * dz;
+                const double qy = r01 * dx + r11 * dy + r21 * dz;
+                const double qz = r02 * dx + r12 * dy + r22 * dz;
+                const int c = lev_cell(qx, qy, qz);
+                if (c < 0) continue;
+                const unsigned long long key = lev_key(qz);
+                if (key < zm[c]) atomicMin(&zm[c], key);
+            }
         }
     }
     __syncthreads();
     const bool both = zs[tid] != LEV_EMPTY && zm[tid] != LEV_EMPTY;
     const double d = both ? lev_value(zm[tid]) - lev_value(zs[tid]) : INFINITY;
-    dv[tid] = d;
     if (both) atomicAdd(&s_n, 1);
     __syncthreads();
     const int n = s_n;
     if (n < LEV_MIN_CELLS) return; /* uniform over the workgroup */
-    /* ascending bitonic sort of the 1024 differences (+inf where a cell lacks one) */
+    /* ascending bitonic sort of the 1024 differences (+inf where a cell lacks one),
+     * one value per thread: partners within a wave by lane shuffles, the rest
+     * through two LDS buffers in turn; a thread keeps its own value on a tie */
+    double v = d;
+    int buf = 0;
     for (int k = 2; k <= LEV_CELLS; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const int l = tid ^ j;
-            if (l > tid) {
-                const double a = dv[tid], c = dv[l];
-                if (((tid & k) == 0) ? (c < a) : (a < c)) {
-                    dv[tid] = c;
-                    dv[l] = a;
-                }
+            double o;
+            if (j >= WAVE) {
+                dv[buf][tid] = v;
+                __syncthreads();
+                o = dv[buf][tid ^ j];
+                buf ^= 1;
+            } else {
+                o = __shfl_xor(v, j);
             }
-            __syncthreads();
+            const bool take_min = ((tid & j) == 0) == ((tid & k) == 0);
+            v = take_min ? ((o < v) ? o : v) : ((v < o) ? o : v);
         }
     }
-    const double med = dv[(n - 1) / 2];
+    dv[buf][tid] = v;
+    __syncthreads();
+    const double med = dv[buf][(n - 1) / 2];
     const bool keep = both && fabs(d - med) < LEV_TRIM;
     if (keep) atomicAdd(&s_keep, 1);
     const double x = (double)(tid % LEV_N) - 15.5, y = (double)(tid / LEV_N) - 15.5;
