@@ -65,7 +65,7 @@ def profiled_traffic(args, prm, kind="plain"):
                   r"chunk (\d+), fine_pass (\d+)\)", text)
     if not m:
         return None
-    shape = (prm.block_size or 256, prm.points_per_wg or 2048, prm.fine_pass or 28)
+    shape = (prm.block_size or 256, prm.points_per_wg or 2048, prm.fine_pass or 24)
     if (m.group(2), int(m.group(3)), m.group(4), (int(m.group(5)), int(m.group(6)), int(m.group(7)))) != (
             args.workload, args.scans, args.precision, shape):
         return None
@@ -656,7 +656,7 @@ def main():
         out["batched_hessian_at_accept"] = {
             "value": S * max(1, min(args.steps, 5)) / (time.perf_counter() - t_h), "unit": "alignments/s",
             "mean_passes": float(np.mean([r["passes"] for r in res_h])),
-            # the extra passes move the late-pass chunking (pass 28 on), so sums round
+            # the extra passes move the late-pass chunking (pass 24 on), so sums round
             # differently there
             "same_poses_as_default": int(sum(np.array_equal(a["final_f64"], b["final_f64"])
                                              for a, b in zip(res_h, res_b))),

@@ -81,7 +81,7 @@ typedef struct ndt_params {
                                   more passes. */
     int32_t fine_pass;        /* ndt align: passes of a run from this one on cut a scan into
                                  chunks a quarter the size (at least 256 points), spreading the
-                                 late passes of the slowest scans over more workgroups; 0 = 28.
+                                 late passes of the slowest scans over more workgroups; 0 = 24.
                                  Part of the launch shape: results depend on it at the level of
                                  rounding, as on points_per_wg */
 } ndt_params;

@@ -155,11 +155,11 @@ def test_schedulers_and_batch_composition_agree(A, gpu_grid, cfg0):
 
 @pytest.mark.parametrize("fine_pass", [0, 12, 6])
 def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
-    """From pass fine_pass of a run on (0: the 28th) a scan's chunks are a quarter
+    """From pass fine_pass of a run on (0: the 24th) a scan's chunks are a quarter
     the size.  Some scans run past it (guesses +-4 m / +-15 deg off for the default,
     +-3 m / +-10 deg otherwise): queue and rounds still give bit-identical results
     (both derive the chunking from the pass number), a scan's result does not
-    depend on its batch, 0 is 28, and another fine_pass moves results only at the
+    depend on its batch, 0 is 24, and another fine_pass moves results only at the
     level of rounding (on starts whose results are stable to that level)."""
     scene, mp, _ = cfg0
     sizes = [2600, 4100, 1800, 3300, 2200, 5200, 1500, 3900]
@@ -173,7 +173,7 @@ def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
         kw = dict(points_per_wg=ppw, fine_pass=fine_pass)
         q = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_QUEUE, **kw))
         r = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, scheduler=A.NDT_SCHED_ROUNDS, **kw))
-        assert max(x["passes"] for x in q) > (fine_pass or 28), [x["passes"] for x in q]
+        assert max(x["passes"] for x in q) > (fine_pass or 24), [x["passes"] for x in q]
         for x, y in zip(q, r):
             assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"], ppw
         k = int(np.argmax([x["passes"] for x in q]))
@@ -184,7 +184,7 @@ def test_late_pass_chunking_agrees(A, gpu_grid, cfg0, fine_pass):
             d = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw))
             assert max(np.abs(x["final_f64"][:3, 3] - y["final_f64"][:3, 3]).max() for x, y in zip(q, d)) < 1e-3
         else:
-            e = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw, fine_pass=28))
+            e = gpu_grid.align_batch(pts, gs, prm=A.params(A.NDT_MODE_NDT, points_per_wg=ppw, fine_pass=24))
             for x, y in zip(q, e):
                 assert np.array_equal(x["final_f64"], y["final_f64"]) and x["passes"] == y["passes"], ppw
 
