@@ -301,6 +301,44 @@ static int upload_staged(ndt_ctx* c, void* dst, const void* src, size_t bytes) {
     return upload_staged2(c, dst, src, bytes, nullptr, nullptr, 0);
 }
 
+/* m points (x, y, z, w floats) to dst through the pinned staging buffer in chunks
+ * of 16,384 points: each chunk is copied and checked on the host, then its copy to
+ * the device is queued, so the transfer of one chunk overlaps the host work on the
+ * next.  Returns NDT_ERR_INVALID at the first chunk holding a point whose x, y or
+ * z is not finite (chunks before it may already be on the device). */
+static int upload_points_finite(ndt_ctx* c, void* dst, const float* src, int64_t m) {
+    if (c->pin_pending) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        c->pin_pending = false;
+    }
+    int e = pinned_reserve(c, sizeof(float4) * (size_t)m);
+    if (e) return e;
+    constexpr int64_t CHUNK = 16384;
+    for (int64_t p0 = 0; p0 < m; p0 += CHUNK) {
+        const int64_t n = std::min(CHUNK, m - p0);
+        const float* sp = src + 4 * p0;
+        float* hp = (float*)c->pin + 4 * p0;
+        bool finite = true;
+        for (int64_t i = 0; i < 4 * n; i += 4) {
+            const float x = sp[i], y = sp[i + 1], z = sp[i + 2];
+            hp[i] = x;
+            hp[i + 1] = y;
+            hp[i + 2] = z;
+            hp[i + 3] = sp[i + 3];
+            /* v - v is 0 for a finite v and NaN for an infinite or NaN one */
+            finite &= (x - x == 0.0f) & (y - y == 0.0f) & (z - z == 0.0f);
+        }
+        if (!finite) {
+            c->pin_pending = p0 > 0;
+            return fail(NDT_ERR_INVALID, "compat mode needs finite map points");
+        }
+        HIP_TRY(hipMemcpyAsync((char*)dst + sizeof(float4) * p0, hp, sizeof(float4) * n, hipMemcpyHostToDevice,
+                               c->stream));
+        c->pin_pending = true;
+    }
+    return 0;
+}
+
 static int grid_build_ndt(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
     const double res = (double)g->params.resolution;
     BUF_TRY(c->pts, sizeof(float4) * m);
@@ -474,12 +512,9 @@ static uint64_t next_grid_serial() {
 
 static int grid_build_compat(ndt_ctx* c, const float* map, int64_t m, ndt_grid* g) {
     /* initCompute, src/CPU/ndt_mapping/kernel.cpp:1302-1373 */
-    for (int64_t i = 0; i < m; i++)
-        if (!std::isfinite(map[4 * i]) || !std::isfinite(map[4 * i + 1]) || !std::isfinite(map[4 * i + 2]))
-            return fail(NDT_ERR_INVALID, "compat mode needs finite map points");
     const float res = g->params.resolution;
     BUF_TRY(c->pts, sizeof(float4) * m);
-    int e = upload_staged(c, c->pts.p, map, sizeof(float4) * m);
+    int e = upload_points_finite(c, c->pts.p, map, m);
     if (e) return e;
     DevBuf& mm = c->bk_mm;
     BUF_TRY(mm, 64);

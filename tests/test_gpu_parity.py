@@ -437,6 +437,23 @@ def test_compat_callback_matches_reference(A, orc, ctx, compat_cases):
     print(f"compat bit-exact with the reference on {exact}/{len(compat_cases)} cases")
 
 
+def test_compat_callback_rejects_non_finite_maps(A, ctx, compat_cases):
+    """A map with a NaN or an infinite coordinate, in the first chunk of the upload
+    or a later one, is refused before any kernel runs; the next callback on the
+    same context still matches the one before it bit for bit."""
+    c = compat_cases[0]
+    first = ctx.compat_callback(c["scan"], c["guess"], c["map"])
+    big = np.concatenate([c["map"]] * max(1, -(-40000 // len(c["map"]))))[:40000].copy()
+    for at, v in ((5, np.nan), (20000, np.inf), (39999, -np.inf)):
+        bad = big.copy()
+        bad[at, 1] = v
+        with pytest.raises(A.NdtError) as e:
+            ctx.compat_callback(c["scan"], c["guess"], bad)
+        assert e.value.code == 1 and "finite" in str(e.value), at
+    again = ctx.compat_callback(c["scan"], c["guess"], c["map"])
+    assert np.array_equal(first["final_f32"], again["final_f32"]) and first["iterations"] == again["iterations"]
+
+
 def test_daphne_harness_end_to_end(tmp_path):
     """The DAPHNE testcase binary (same interface, files and check as the reference)
     on a reference-produced output file: exits 0 with "result ok"."""
