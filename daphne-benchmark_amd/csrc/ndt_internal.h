@@ -348,7 +348,8 @@ struct ScanDesc {
  * guesses: 6.45 ms against 7.0 ms with one chunk size; passes from 20, 24, 32 or 36
  * on, chunks an eighth or half the size, or a further level of sixteenth-size
  * chunks from pass 36, 40 or 44 on, gave 6.45-7.0 ms; with starts a registration
- * put near the optimum, from pass 12 on).  Once a line-search trial at the step
+ * put near the optimum, from pass 12 on, and pass 8 once leveled starts shortened
+ * the runs further).  Once a line-search trial at the step
  * of the trial before it no longer made a pass (runs about 12 % shorter), pass 24
  * did best on that batch: 5.55 ms against 5.65 at pass 28, 5.63 at 22, 5.76 at 20
  * and 6.10 at 16.  The chunking is a function of the pass number only, so results
